@@ -1,0 +1,171 @@
+/*
+ * netrax_gpu.h -- C ABI of the MI355X network-likelihood engine.
+ *
+ * Drop-in boundary for NetRAX's likelihood layer (src/likelihood/).  Every
+ * entry point is extern "C", takes plain pointers and sizes, returns an
+ * integer status (NRX_OK == 0) and never throws; the message of the last
+ * failure on the calling thread is available from nrx_last_error().
+ *
+ * Reference interfaces replaced (paths relative to the NetRAX source tree):
+ *   nrx_create / nrx_destroy
+ *       AnnotatedNetwork construction + createNetworkPllTreeinfo
+ *       (src/graph/AnnotatedNetwork.cpp:80-185, src/RaxmlWrapper.cpp:539-669):
+ *       the engine owns CLVs, scalers and P matrices in HBM.
+ *   nrx_set_model
+ *       pll model setup (subst rates, frequencies, Gamma alpha) done by
+ *       raxml-ng's Model / pllmod_treeinfo (RaxmlWrapper.cpp:189-214).
+ *   nrx_set_branch_lengths
+ *       fake_treeinfo->linked_branch_lengths + invalidatePmatrixIndex
+ *       (RaxmlWrapper.cpp:652-663, src/helper/InvalidationHelper.cpp:157-172).
+ *   nrx_set_reticulation_probs
+ *       setReticulationProb (src/optimization/ReticulationOptimization.cpp:25-38).
+ *   nrx_loglikelihood
+ *       double computeLoglikelihood(AnnotatedNetwork&, int incremental,
+ *       int update_pmatrices)  (src/likelihood/LikelihoodComputation.hpp:17-18)
+ *       and the pll-modules hook network_logl_wrapper (RaxmlWrapper.cpp:21-26).
+ *   nrx_gradient
+ *       no single reference function: replaces the E calls of
+ *       computePartitionSumtables + computeLoglikelihoodDerivatives that a
+ *       full gradient costs in the reference, and adds d lnL / d p_r
+ *       (the reference optimises p_r by Brent, ReticulationOptimization.cpp:68-101).
+ *   nrx_brlen_derivatives
+ *       computePartitionSumtables(ann_network, pmatrix_index) followed by
+ *       computeLoglikelihoodDerivatives(ann_network, sumtables, pmatrix_index)
+ *       (src/likelihood/LikelihoodDerivatives.hpp:83-88), evaluated at a given
+ *       branch length as pll's Newton-Raphson callback does
+ *       (src/optimization/BranchLengthOptimization.cpp:171-200).
+ */
+#ifndef NETRAX_GPU_H
+#define NETRAX_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NRX_OK 0
+#define NRX_ERR_ARG 1        /* invalid argument / shape */
+#define NRX_ERR_NEGINF 2     /* network likelihood is -infinity */
+#define NRX_ERR_DEVICE 3     /* HIP runtime failure */
+#define NRX_ERR_UNSUPPORTED 4
+#define NRX_ERR_STATE 5      /* call out of order (e.g. no model set) */
+
+/* LikelihoodVariant (src/likelihood/LikelihoodVariant.hpp:3-7) */
+#define NRX_AVERAGE_DISPLAYED_TREES 0
+#define NRX_BEST_DISPLAYED_TREE 1
+
+/* arithmetic of the CLV products */
+#define NRX_FP64 0 /* plain form: FP64 CLVs, one launch per pruning level */
+#define NRX_FP32 1 /* batched form: FP32 CLVs + per-site power-of-two
+                      exponents, persistent site-tile kernels, FP64 sums */
+
+/* branch-derivative normalisation for nrx_brlen_derivatives */
+#define NRX_DERIV_REFERENCE 0 /* sums over trees containing the branch only
+                                 (LikelihoodDerivatives.cpp:153-183, 325) */
+#define NRX_DERIV_TRUE 1      /* true derivatives of the network lnL */
+
+typedef struct nrx_engine nrx_engine;
+
+/* Network topology with NetRAX's index layout (src/io/NetworkIO.cpp:114-185):
+ * edge e goes edge_parent[e] -> edge_child[e]; reticulation r has parent
+ * edges retic_first_edge[r] and retic_second_edge[r]. */
+typedef struct {
+  int32_t n_nodes;
+  int32_t n_tips; /* tips are nodes 0 .. n_tips-1 */
+  int32_t root;
+  int32_t n_edges;
+  int32_t n_retic;
+  const int32_t *edge_parent;       /* [n_edges] */
+  const int32_t *edge_child;        /* [n_edges] */
+  const int32_t *retic_first_edge;  /* [n_retic] */
+  const int32_t *retic_second_edge; /* [n_retic] */
+} nrx_network_desc;
+
+/* One partition of compressed site patterns. */
+typedef struct {
+  int32_t n_sites;           /* number of patterns */
+  int32_t states;            /* 4 (DNA) or 20 (protein) */
+  int32_t rate_cats;         /* discrete Gamma categories (1..8) */
+  const uint32_t *tip_masks; /* [n_tips][n_sites]; bit i set = state i allowed */
+  const uint32_t *weights;   /* [n_sites] pattern weights */
+} nrx_partition_desc;
+
+typedef struct {
+  int32_t precision;       /* NRX_FP64 or NRX_FP32 */
+  int32_t device;          /* HIP device ordinal */
+  double min_tree_logprob; /* NetraxOptions::min_interesting_tree_logprob, log(1e-6) */
+  double brlen_min;        /* branch lengths are clamped to [brlen_min, brlen_max] */
+  double brlen_max;
+  int32_t use_graph;       /* capture the evaluation in a hipGraph (FP32 form) */
+  int32_t timing;          /* record HIP events around kernels (nrx_last_timing) */
+} nrx_options;
+
+typedef struct {
+  int32_t n_variants;      /* distinct (node, displayed subtree) CLVs */
+  int32_t n_ops_needed;    /* CLV ops evaluated per full forward pass */
+  int32_t n_levels;        /* pruning levels */
+  int32_t n_classes;       /* root variants (distinct displayed trees) */
+  int32_t n_root_clvs;     /* distinct CLVs the root kernel reads */
+  int32_t n_uses;          /* (parent op, child) pairs walked by the adjoint */
+  int64_t n_sites_padded;  /* sites over all partitions after tile padding */
+  double fwd_bytes;        /* algorithmic HBM bytes of one forward pass */
+  double grad_bytes;       /* algorithmic HBM bytes of forward + adjoint */
+  double clv_bytes;        /* device bytes held by CLVs, scalers and adjoints */
+} nrx_stats;
+
+void nrx_default_options(nrx_options *opt);
+const char *nrx_last_error(void);
+
+int nrx_create(const nrx_network_desc *net, const nrx_partition_desc *parts,
+               int32_t n_parts, const nrx_options *opt, nrx_engine **out);
+int nrx_destroy(nrx_engine *eng);
+
+/* subst_rates: states*(states-1)/2 exchangeabilities (upper triangle,
+ * row-major: AC AG AT CG CT GT for DNA); freqs: states values > 0;
+ * alpha: Gamma shape (ignored for one category). */
+int nrx_set_model(nrx_engine *eng, int32_t partition, const double *subst_rates,
+                  const double *freqs, double alpha);
+/* explicit category rates (mean 1 expected) instead of Gamma(alpha) */
+int nrx_set_category_rates(nrx_engine *eng, int32_t partition, const double *rates);
+int nrx_set_branch_lengths(nrx_engine *eng, const double *brlen /* [n_edges] */);
+int nrx_set_reticulation_probs(nrx_engine *eng,
+                               const double *first_parent_prob /* [n_retic] */);
+
+/* Network lnL.  incremental = 0 recomputes every CLV; 1 recomputes only
+ * CLVs depending on branch lengths changed since the last call.
+ * update_pmatrices = 0 keeps the current P matrices.  per_partition may be
+ * NULL.  Returns NRX_ERR_NEGINF when the result is -infinity
+ * (ImprovedLoglikelihood.cpp:637-640). */
+int nrx_loglikelihood(nrx_engine *eng, int32_t variant, int32_t incremental,
+                      int32_t update_pmatrices, double *total, double *per_partition);
+
+/* lnL plus dlnL/d(branch length) for every edge and dlnL/d(first-parent
+ * probability) for every reticulation, from one forward and one adjoint
+ * pass.  per_partition_dbrlen ([n_parts][n_edges]) may be NULL. */
+int nrx_gradient(nrx_engine *eng, int32_t variant, double *total, double *dbrlen,
+                 double *dprob, double *per_partition_dbrlen);
+
+/* First and second derivative of lnL w.r.t. the length of one branch,
+ * evaluated at branch length `brlen` with every other parameter fixed.
+ * mode: NRX_DERIV_REFERENCE or NRX_DERIV_TRUE.  part_d1/part_d2 may be NULL. */
+int nrx_brlen_derivatives(nrx_engine *eng, int32_t variant, int32_t edge,
+                          int32_t mode, double brlen, double *d1, double *d2,
+                          double *part_d1, double *part_d2);
+
+/* Per-(partition, displayed tree) log-likelihoods of the last forward pass:
+ * tree_logl [n_parts][n_classes], class_logprob [n_classes]. */
+int nrx_tree_logls(nrx_engine *eng, double *tree_logl, double *class_logprob);
+
+int nrx_get_stats(nrx_engine *eng, nrx_stats *out);
+
+/* Milliseconds of the device work of the last nrx_loglikelihood /
+ * nrx_gradient call, and of its dominant kernel summed over launches. */
+int nrx_last_timing(nrx_engine *eng, double *total_ms, double *dominant_ms,
+                    int32_t *dominant_launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NETRAX_GPU_H */

@@ -1,0 +1,784 @@
+// Host orchestration and C ABI of the network-likelihood engine.
+//
+// The call sequence mirrors NetRAX's computeLoglikelihoodImproved
+// (src/likelihood/ImprovedLoglikelihood.cpp:646-671): invalidate (when not
+// incremental), update P matrices of invalid edges, recompute invalid CLVs
+// level by level, evaluate every displayed-tree root, mix per partition,
+// sum partitions, and fail on -infinity.  A result is reused when nothing
+// changed since the last call (reuseOldDisplayedTreesCheck,
+// src/graph/AnnotatedNetwork.cpp:359-394).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <exception>
+#include <stdexcept>
+#include <string>
+
+#include "engine_state.hpp"
+
+namespace nrx {
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg) {
+  g_last_error = msg;
+  return code;
+}
+
+template <typename F>
+int guarded(F &&f) {
+  try {
+    return f();
+  } catch (const std::invalid_argument &ex) {
+    return fail(NRX_ERR_ARG, ex.what());
+  } catch (const std::runtime_error &ex) {
+    const std::string m = ex.what();
+    return fail(m.rfind("HIP error", 0) == 0 ? NRX_ERR_DEVICE : NRX_ERR_STATE, m);
+  } catch (const std::exception &ex) {
+    return fail(NRX_ERR_STATE, ex.what());
+  }
+}
+
+// Host-to-device copy of a (pageable, possibly short-lived) host vector;
+// waits for completion so the source may be freed right after.
+template <typename T>
+void upload(DevBuf &b, const std::vector<T> &v, hipStream_t s) {
+  b.alloc(std::max<size_t>(sizeof(T) * v.size(), 16));
+  if (!v.empty()) {
+    NRX_HIP(hipMemcpyAsync(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, s));
+    NRX_HIP(hipStreamSynchronize(s));
+  }
+}
+
+}  // namespace
+
+Engine::~Engine() {
+  if (stream) (void)hipStreamSynchronize(stream);
+  if (h_result) (void)hipHostFree(h_result);
+  if (ev_begin) (void)hipEventDestroy(ev_begin);
+  if (ev_end) (void)hipEventDestroy(ev_end);
+  for (auto ev : ev_dom) (void)hipEventDestroy(ev);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+// ------------------------------------------------------------ host logic
+
+static void compute_class_probs(Engine &e) {
+  const int nc = static_cast<int>(e.dag.classes.size()), R = e.R;
+  e.class_logprob.assign(nc, 0.0);
+  e.class_dlogprob.assign(static_cast<size_t>(std::max(nc, 1)) * std::max(R, 1), 0.0);
+  e.class_use.assign(nc, 0);
+  for (int k = 0; k < nc; ++k) {
+    const auto &cfgs = e.dag.classes[k].configs;
+    // log P of one configuration: sum over reticulations of log p or
+    // log(1 - p) (ReticulationConfigSet.cpp:70-88); several member
+    // configurations are summed in probability space (:98-113)
+    std::vector<double> lps(cfgs.size());
+    double mx = -INFINITY;
+    for (size_t m = 0; m < cfgs.size(); ++m) {
+      double lp = 0.0;
+      for (int r = 0; r < R; ++r)
+        lp += ((cfgs[m] >> r) & 1u) ? std::log(1.0 - e.probs[r]) : std::log(e.probs[r]);
+      lps[m] = lp;
+      mx = std::max(mx, lp);
+    }
+    double s = 0.0;
+    std::vector<double> d(R, 0.0);
+    for (size_t m = 0; m < cfgs.size(); ++m) {
+      const double w = std::exp(lps[m] - mx);
+      s += w;
+      for (int r = 0; r < R; ++r)
+        d[r] += w * (((cfgs[m] >> r) & 1u) ? -1.0 / (1.0 - e.probs[r]) : 1.0 / e.probs[r]);
+    }
+    e.class_logprob[k] = cfgs.size() == 1 ? lps[0] : mx + std::log(s);
+    for (int r = 0; r < R; ++r) e.class_dlogprob[static_cast<size_t>(k) * R + r] = d[r] / s;
+    e.class_use[k] = e.class_logprob[k] >= e.opt.min_tree_logprob;
+  }
+}
+
+// Builds the level schedule, root CLV set and adjoint use lists for the
+// classes that pass the log-probability cut, and uploads them.
+static void build_schedule(Engine &e) {
+  const int nv = static_cast<int>(e.dag.var.size());
+  const int nc = static_cast<int>(e.dag.classes.size());
+  e.needed.assign(nv, 0);
+  for (int k = 0; k < nc; ++k)
+    if (e.class_use[k]) e.needed[e.dag.classes[k].dtroot_slot] = 1;
+  for (int s = nv - 1; s >= 0; --s)  // parents have larger slot ids
+    if (e.needed[s])
+      for (int k = 0; k < e.dag.var[s].nkids; ++k)
+        if (e.dag.var[s].kid[k] >= 0) e.needed[e.dag.var[s].kid[k]] = 1;
+  int max_level = 0;
+  for (int s = 0; s < nv; ++s)
+    if (e.needed[s]) max_level = std::max(max_level, e.dag.var[s].level);
+  e.level_ops.assign(max_level + 1, {});
+  for (int s = 0; s < nv; ++s)
+    if (e.needed[s]) e.level_ops[e.dag.var[s].level].push_back(s);
+  e.ops_all.clear();
+  e.level_begin.assign(max_level + 2, 0);
+  for (int l = 0; l <= max_level; ++l) {
+    e.level_begin[l] = static_cast<int>(e.ops_all.size());
+    for (int s : e.level_ops[l]) {
+      const Variant &v = e.dag.var[s];
+      DevOp op{};
+      op.out = s;
+      op.nk = v.nkids;
+      for (int k = 0; k < 2; ++k) {
+        op.kid[k] = v.kid[k];
+        op.edge[k] = v.edge[k] < 0 ? 0 : v.edge[k];
+      }
+      e.ops_all.push_back(op);
+    }
+  }
+  e.level_begin[max_level + 1] = static_cast<int>(e.ops_all.size());
+
+  // root CLVs
+  e.root_slots.clear();
+  e.class_root_index.assign(nc, -1);
+  for (int k = 0; k < nc; ++k) {
+    if (!e.class_use[k]) continue;
+    const int s = e.dag.classes[k].dtroot_slot;
+    auto it = std::find(e.root_slots.begin(), e.root_slots.end(), s);
+    if (it == e.root_slots.end()) {
+      e.root_slots.push_back(s);
+      it = e.root_slots.end() - 1;
+    }
+    e.class_root_index[k] = static_cast<int>(it - e.root_slots.begin());
+  }
+
+  // adjoint: one work item per needed CLV and per tip, consumers in order
+  std::vector<std::vector<DevUse>> uses_of_slot(nv), uses_of_tip(e.T);
+  for (int s = 0; s < nv; ++s) {
+    if (!e.needed[s]) continue;
+    const Variant &v = e.dag.var[s];
+    for (int k = 0; k < v.nkids; ++k) {
+      DevUse u{};
+      u.parent = s;
+      u.edge = v.edge[k];
+      u.sib = v.nkids == 2 ? v.kid[1 - k] : NO_CHILD;
+      u.sib_edge = v.nkids == 2 ? v.edge[1 - k] : 0;
+      if (v.kid[k] >= 0)
+        uses_of_slot[v.kid[k]].push_back(u);
+      else
+        uses_of_tip[tip_of(v.kid[k])].push_back(u);
+    }
+  }
+  e.children.clear();
+  e.uses.clear();
+  std::vector<std::vector<int>> child_by_level(max_level + 1);
+  std::vector<int> root_index_of_slot(nv, -1);
+  for (size_t r = 0; r < e.root_slots.size(); ++r) root_index_of_slot[e.root_slots[r]] = r;
+  auto add_child = [&](int32_t code, const std::vector<DevUse> &us, int level) {
+    DevChild ch{};
+    ch.child = code;
+    ch.use_begin = static_cast<int>(e.uses.size());
+    for (DevUse u : us) {
+      u.grad_slot = static_cast<int>(e.uses.size());
+      e.uses.push_back(u);
+    }
+    ch.use_end = static_cast<int>(e.uses.size());
+    ch.root_index = code >= 0 ? root_index_of_slot[code] : -1;
+    child_by_level[level].push_back(static_cast<int>(e.children.size()));
+    e.children.push_back(ch);
+  };
+  for (int s = 0; s < nv; ++s)
+    if (e.needed[s]) add_child(s, uses_of_slot[s], e.dag.var[s].level);
+  for (int t = 0; t < e.T; ++t)
+    if (!uses_of_tip[t].empty()) add_child(tip_code(t), uses_of_tip[t], 0);
+  e.n_uses = static_cast<int>(e.uses.size());
+  e.child_lists.clear();
+  e.child_level_begin.assign(max_level + 2, 0);
+  // reverse order: highest level first
+  for (int l = max_level, i = 0; l >= 0; --l, ++i) {
+    e.child_level_begin[i] = static_cast<int>(e.child_lists.size());
+    for (int c : child_by_level[l]) e.child_lists.push_back(c);
+  }
+  e.child_level_begin[max_level + 1] = static_cast<int>(e.child_lists.size());
+  // uses grouped by edge
+  std::vector<std::vector<int>> by_edge(e.E);
+  for (int u = 0; u < e.n_uses; ++u) by_edge[e.uses[u].edge].push_back(u);
+  e.edge_use_begin.assign(e.E + 1, 0);
+  e.edge_use_list.clear();
+  for (int ed = 0; ed < e.E; ++ed) {
+    e.edge_use_begin[ed] = static_cast<int>(e.edge_use_list.size());
+    for (int u : by_edge[ed]) e.edge_use_list.push_back(u);
+  }
+  e.edge_use_begin[e.E] = static_cast<int>(e.edge_use_list.size());
+
+  std::vector<DevClass> dc(nc);
+  for (int k = 0; k < nc; ++k) {
+    dc[k].logprob = e.class_logprob[k];
+    dc[k].root_index = e.class_root_index[k];
+    dc[k].use = e.class_use[k];
+  }
+  upload(e.d_ops, e.ops_all, e.stream);
+  upload(e.d_children, e.children, e.stream);
+  upload(e.d_child_lists, e.child_lists, e.stream);
+  upload(e.d_uses, e.uses, e.stream);
+  upload(e.d_edge_use_begin, e.edge_use_begin, e.stream);
+  upload(e.d_edge_use_list, e.edge_use_list, e.stream);
+  upload(e.d_classes, dc, e.stream);
+  upload(e.d_class_dlogprob, e.class_dlogprob, e.stream);
+  std::vector<int32_t> rs(e.root_slots.begin(), e.root_slots.end());
+  upload(e.d_root_slots, rs, e.stream);
+  e.d_root_partials.alloc(sizeof(double) * std::max<size_t>(1, rs.size()) * e.L.n_tiles);
+  e.d_beta.alloc(sizeof(double) * std::max<size_t>(1, rs.size()) * e.P);
+  e.d_grad_partials.alloc(sizeof(double) * std::max(1, e.n_uses) * static_cast<size_t>(e.L.n_tiles));
+  NRX_HIP(hipStreamSynchronize(e.stream));
+}
+
+static void upload_classes_only(Engine &e) {
+  const int nc = static_cast<int>(e.dag.classes.size());
+  std::vector<DevClass> dc(nc);
+  for (int k = 0; k < nc; ++k) {
+    dc[k].logprob = e.class_logprob[k];
+    dc[k].root_index = e.class_root_index[k];
+    dc[k].use = e.class_use[k];
+  }
+  upload(e.d_classes, dc, e.stream);
+  upload(e.d_class_dlogprob, e.class_dlogprob, e.stream);
+  NRX_HIP(hipStreamSynchronize(e.stream));
+}
+
+static void set_probs(Engine &e, const double *p) {
+  for (int r = 0; r < e.R; ++r) {
+    if (!(p[r] > 0.0 && p[r] < 1.0))
+      throw std::invalid_argument("reticulation probabilities must lie in (0, 1)");
+    e.probs[r] = p[r];
+  }
+  const std::vector<char> old_use = e.class_use;
+  compute_class_probs(e);
+  if (old_use != e.class_use) {
+    build_schedule(e);
+    std::fill(e.clv_dirty.begin(), e.clv_dirty.end(), 1);
+  } else {
+    upload_classes_only(e);
+  }
+  e.probs_dirty = true;
+}
+
+static void mark_clv_dirty_for_edges(Engine &e) {
+  const int nv = static_cast<int>(e.dag.var.size());
+  for (int s = 0; s < nv; ++s) {
+    const Variant &v = e.dag.var[s];
+    bool d = e.clv_dirty[s];
+    for (int k = 0; k < v.nkids && !d; ++k) {
+      if (e.edge_dirty[v.edge[k]]) d = true;
+      if (v.kid[k] >= 0 && e.clv_dirty[v.kid[k]]) d = true;
+    }
+    e.clv_dirty[s] = d;
+  }
+}
+
+static void check_ready(const Engine &e) {
+  for (int p = 0; p < e.P; ++p)
+    if (!e.parts[p].model_set) throw std::runtime_error("model of a partition is not set");
+}
+
+static void upload_models(Engine &e) {
+  const int S = e.S, C = e.C, P = e.P;
+  std::vector<double> V(P * S * S), Vi(P * S * S), ev(P * S), rates(P * C), fr(P * S);
+  for (int p = 0; p < P; ++p) {
+    const PartInfo &pi = e.parts[p];
+    std::copy(pi.model.V.begin(), pi.model.V.end(), V.begin() + p * S * S);
+    std::copy(pi.model.Vinv.begin(), pi.model.Vinv.end(), Vi.begin() + p * S * S);
+    std::copy(pi.model.eval.begin(), pi.model.eval.end(), ev.begin() + p * S);
+    std::copy(pi.rates.begin(), pi.rates.end(), rates.begin() + p * C);
+    std::copy(pi.model.freqs.begin(), pi.model.freqs.end(), fr.begin() + p * S);
+  }
+  upload(e.d_V, V, e.stream);
+  upload(e.d_Vinv, Vi, e.stream);
+  upload(e.d_eval, ev, e.stream);
+  upload(e.d_rates, rates, e.stream);
+  upload(e.d_freqs, fr, e.stream);
+}
+
+// P matrices for changed edges / models; marks dependent CLVs invalid.
+static void update_pmatrices(Engine &e, bool do_update) {
+  const bool any_edge = std::any_of(e.edge_dirty.begin(), e.edge_dirty.end(), [](char c) { return c; });
+  if (!(e.model_dirty || any_edge)) return;
+  if (e.model_dirty) std::fill(e.clv_dirty.begin(), e.clv_dirty.end(), 1);
+  mark_clv_dirty_for_edges(e);
+  if (!do_update) return;
+  if (e.model_dirty) upload_models(e);
+  upload(e.d_brlen, e.brlen, e.stream);
+  launch_pmatrices(e);
+  e.model_dirty = false;
+  std::fill(e.edge_dirty.begin(), e.edge_dirty.end(), 0);
+}
+
+static void record_dom_begin(Engine &e) {
+  if (!e.opt.timing) return;
+  if (static_cast<int>(e.ev_dom.size()) < 2 * (e.n_dom + 1)) {
+    for (int i = 0; i < 2; ++i) {
+      hipEvent_t ev;
+      NRX_HIP(hipEventCreate(&ev));
+      e.ev_dom.push_back(ev);
+    }
+  }
+  NRX_HIP(hipEventRecord(e.ev_dom[2 * e.n_dom], e.stream));
+}
+
+static void record_dom_end(Engine &e) {
+  if (!e.opt.timing) return;
+  NRX_HIP(hipEventRecord(e.ev_dom[2 * e.n_dom + 1], e.stream));
+  e.n_dom++;
+}
+
+static void forward(Engine &e, bool incremental) {
+  const int n_levels = static_cast<int>(e.level_ops.size());
+  if (!incremental) std::fill(e.clv_dirty.begin(), e.clv_dirty.end(), 1);
+  bool all = true;
+  for (size_t i = 0; i < e.ops_all.size(); ++i)
+    if (!e.clv_dirty[e.ops_all[i].out]) {
+      all = false;
+      break;
+    }
+  if (all) {
+    for (int l = 1; l < n_levels; ++l) {
+      const int b = e.level_begin[l], n = e.level_begin[l + 1] - b;
+      record_dom_begin(e);
+      launch_forward_level(e, e.d_ops.as<DevOp>() + b, n);
+      record_dom_end(e);
+    }
+  } else {
+    std::vector<DevOp> dirty;
+    std::vector<int> begin(n_levels + 1, 0);
+    for (int l = 0; l < n_levels; ++l) {
+      begin[l] = static_cast<int>(dirty.size());
+      for (int i = e.level_begin[l]; i < e.level_begin[l + 1]; ++i)
+        if (e.clv_dirty[e.ops_all[i].out]) dirty.push_back(e.ops_all[i]);
+    }
+    begin[n_levels] = static_cast<int>(dirty.size());
+    if (!dirty.empty()) {
+      upload(e.d_ops_dirty, dirty, e.stream);
+      for (int l = 1; l < n_levels; ++l) {
+        const int b = begin[l], n = begin[l + 1] - b;
+        if (n == 0) continue;
+        record_dom_begin(e);
+        launch_forward_level(e, e.d_ops_dirty.as<DevOp>() + b, n);
+        record_dom_end(e);
+      }
+    }
+  }
+  for (size_t i = 0; i < e.ops_all.size(); ++i) e.clv_dirty[e.ops_all[i].out] = 0;
+}
+
+static void begin_timing(Engine &e) {
+  e.n_dom = 0;
+  if (e.opt.timing) NRX_HIP(hipEventRecord(e.ev_begin, e.stream));
+}
+
+static void end_timing(Engine &e) {
+  if (!e.opt.timing) return;
+  NRX_HIP(hipEventRecord(e.ev_end, e.stream));
+  NRX_HIP(hipEventSynchronize(e.ev_end));
+  float ms = 0.f;
+  NRX_HIP(hipEventElapsedTime(&ms, e.ev_begin, e.ev_end));
+  e.last_total_ms = ms;
+  double dom = 0.0;
+  for (int i = 0; i < e.n_dom; ++i) {
+    NRX_HIP(hipEventElapsedTime(&ms, e.ev_dom[2 * i], e.ev_dom[2 * i + 1]));
+    dom += ms;
+  }
+  e.last_dom_ms = dom;
+  e.last_dom_launches = e.n_dom;
+}
+
+static void fetch_results(Engine &e) {
+  NRX_HIP(hipMemcpyAsync(e.h_result, e.d_result.p, sizeof(double) * e.rl.size,
+                         hipMemcpyDeviceToHost, e.stream));
+  NRX_HIP(hipStreamSynchronize(e.stream));
+}
+
+static int evaluate(Engine &e, int variant, bool incremental, bool update_pm, bool gradient) {
+  check_ready(e);
+  if (variant != NRX_AVERAGE_DISPLAYED_TREES && variant != NRX_BEST_DISPLAYED_TREE)
+    throw std::invalid_argument("unknown likelihood variant");
+  begin_timing(e);
+  update_pmatrices(e, update_pm);
+  const bool any_dirty = !incremental || std::any_of(e.ops_all.begin(), e.ops_all.end(),
+                                                     [&](const DevOp &o) { return e.clv_dirty[o.out]; });
+  if (!gradient && !any_dirty && !e.probs_dirty && e.cache_valid && e.cache_variant == variant) {
+    end_timing(e);
+    return NRX_OK;  // nothing changed since the last evaluation
+  }
+  forward(e, incremental);
+  launch_root_partials(e);
+  launch_mixture(e, variant);
+  if (gradient) {
+    e.d_adj.alloc(sizeof(double) * e.clv_elems_per_slot * e.dag.var.size());
+    const int nl = static_cast<int>(e.child_level_begin.size()) - 1;
+    for (int i = 0; i < nl; ++i) {
+      const int b = e.child_level_begin[i], n = e.child_level_begin[i + 1] - b;
+      launch_adjoint_level(e, e.d_child_lists.as<int32_t>() + b, n);
+    }
+    launch_grad_reduce(e);
+  }
+  end_timing(e);
+  fetch_results(e);
+  e.probs_dirty = false;
+  e.cache_valid = true;
+  e.cache_variant = variant;
+  e.cached_total = e.h_result[e.rl.total];
+  e.cached_per_part.assign(e.h_result + e.rl.per_part, e.h_result + e.rl.per_part + e.P);
+  if (!(e.cached_total > -INFINITY) || std::isnan(e.cached_total)) {
+    e.cache_valid = false;
+    return fail(NRX_ERR_NEGINF, "Invalid network likelihood: negative infinity");
+  }
+  return NRX_OK;
+}
+
+}  // namespace nrx
+
+using namespace nrx;
+
+extern "C" {
+
+void nrx_default_options(nrx_options *opt) {
+  if (!opt) return;
+  std::memset(opt, 0, sizeof(*opt));
+  opt->precision = NRX_FP64;
+  opt->device = 0;
+  opt->min_tree_logprob = std::log(1e-6);
+  opt->brlen_min = 1e-6;
+  opt->brlen_max = 100.0;
+  opt->use_graph = 0;
+  opt->timing = 0;
+}
+
+const char *nrx_last_error(void) { return g_last_error.c_str(); }
+
+int nrx_create(const nrx_network_desc *net, const nrx_partition_desc *parts, int32_t n_parts,
+               const nrx_options *opt, nrx_engine **out) {
+  return guarded([&]() -> int {
+    if (!net || !parts || n_parts <= 0 || !out) throw std::invalid_argument("null argument");
+    *out = nullptr;
+    auto e = std::make_unique<Engine>();
+    if (opt)
+      e->opt = *opt;
+    else
+      nrx_default_options(&e->opt);
+    if (e->opt.precision != NRX_FP64 && e->opt.precision != NRX_FP32)
+      throw std::invalid_argument("unknown precision");
+    if (e->opt.precision == NRX_FP32)
+      return fail(NRX_ERR_UNSUPPORTED, "FP32 form not available in this build");
+    e->N = net->n_nodes;
+    e->E = net->n_edges;
+    e->R = net->n_retic;
+    e->T = net->n_tips;
+    e->root = net->root;
+    e->edge_parent.assign(net->edge_parent, net->edge_parent + net->n_edges);
+    e->edge_child.assign(net->edge_child, net->edge_child + net->n_edges);
+    if (e->R > 0) {
+      e->retic_first.assign(net->retic_first_edge, net->retic_first_edge + e->R);
+      e->retic_second.assign(net->retic_second_edge, net->retic_second_edge + e->R);
+    }
+    nrx_network_desc nd = *net;
+    nd.edge_parent = e->edge_parent.data();
+    nd.edge_child = e->edge_child.data();
+    nd.retic_first_edge = e->retic_first.data();
+    nd.retic_second_edge = e->retic_second.data();
+    e->dag = compile_dag(nd);
+
+    e->P = n_parts;
+    e->S = parts[0].states;
+    e->C = parts[0].rate_cats;
+    if (e->S != 4 && e->S != 20) throw std::invalid_argument("states must be 4 or 20");
+    if (e->C != 1 && e->C != 2 && e->C != 4 && e->C != 8)
+      throw std::invalid_argument("rate categories must be 1, 2, 4 or 8");
+    e->parts.resize(n_parts);
+    int64_t off = 0;
+    int tile = 0;
+    for (int p = 0; p < n_parts; ++p) {
+      if (parts[p].states != e->S || parts[p].rate_cats != e->C)
+        throw std::invalid_argument("all partitions must share states and rate categories");
+      if (parts[p].n_sites <= 0 || !parts[p].tip_masks || !parts[p].weights)
+        throw std::invalid_argument("empty partition");
+      PartInfo &pi = e->parts[p];
+      pi.n_sites = parts[p].n_sites;
+      pi.site_off = off;
+      const int nt = (pi.n_sites + TILE - 1) / TILE;
+      pi.tile_begin = tile;
+      pi.tile_end = tile + nt;
+      tile += nt;
+      off += static_cast<int64_t>(nt) * TILE;
+    }
+    DevLayout &L = e->L;
+    L.n_sites_pad = off;
+    L.n_tiles = tile;
+    L.n_tiles_grid = (tile + 7) / 8 * 8;
+    L.n_parts = n_parts;
+    L.n_edges = e->E;
+    L.S = e->S;
+    L.C = e->C;
+
+    NRX_HIP(hipSetDevice(e->opt.device));
+    NRX_HIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    NRX_HIP(hipEventCreate(&e->ev_begin));
+    NRX_HIP(hipEventCreate(&e->ev_end));
+
+    // tips, weights, tile -> partition
+    const int64_t n = L.n_sites_pad;
+    const uint32_t full = e->S == 4 ? 0xFu : ((1u << 20) - 1u);
+    std::vector<double> w(n, 0.0);
+    std::vector<int32_t> tile_part(tile), ptb(n_parts + 1);
+    for (int p = 0; p < n_parts; ++p) {
+      const PartInfo &pi = e->parts[p];
+      for (int s = 0; s < pi.n_sites; ++s) w[pi.site_off + s] = parts[p].weights[s];
+      for (int t = pi.tile_begin; t < pi.tile_end; ++t) tile_part[t] = p;
+      ptb[p] = pi.tile_begin;
+    }
+    ptb[n_parts] = tile;
+    if (e->S == 4) {
+      std::vector<uint8_t> tips(static_cast<size_t>(e->T) * n, static_cast<uint8_t>(full));
+      for (int p = 0; p < n_parts; ++p) {
+        const PartInfo &pi = e->parts[p];
+        for (int t = 0; t < e->T; ++t)
+          for (int s = 0; s < pi.n_sites; ++s) {
+            const uint32_t m = parts[p].tip_masks[static_cast<size_t>(t) * pi.n_sites + s] & full;
+            if (m == 0) throw std::invalid_argument("tip state mask is empty");
+            tips[static_cast<size_t>(t) * n + pi.site_off + s] = static_cast<uint8_t>(m);
+          }
+      }
+      upload(e->d_tips, tips, e->stream);
+    } else {
+      std::vector<uint32_t> tips(static_cast<size_t>(e->T) * n, full);
+      for (int p = 0; p < n_parts; ++p) {
+        const PartInfo &pi = e->parts[p];
+        for (int t = 0; t < e->T; ++t)
+          for (int s = 0; s < pi.n_sites; ++s) {
+            const uint32_t m = parts[p].tip_masks[static_cast<size_t>(t) * pi.n_sites + s] & full;
+            if (m == 0) throw std::invalid_argument("tip state mask is empty");
+            tips[static_cast<size_t>(t) * n + pi.site_off + s] = m;
+          }
+      }
+      upload(e->d_tips, tips, e->stream);
+    }
+    upload(e->d_weights, w, e->stream);
+    upload(e->d_tile_part, tile_part, e->stream);
+    upload(e->d_part_tile_begin, ptb, e->stream);
+
+    const size_t pm = static_cast<size_t>(n_parts) * e->E * e->C * e->S * e->S;
+    e->d_P.alloc(sizeof(double) * pm);
+    e->d_dP.alloc(sizeof(double) * pm);
+    e->d_d2P.alloc(sizeof(double) * pm);
+    e->clv_elems_per_slot = static_cast<size_t>(n) * e->C * e->S;
+    const size_t nv = e->dag.var.size();
+    e->d_clv.alloc(sizeof(double) * e->clv_elems_per_slot * std::max<size_t>(nv, 1));
+    e->d_scl.alloc(sizeof(int32_t) * static_cast<size_t>(n) * std::max<size_t>(nv, 1));
+
+    const int nc = static_cast<int>(e->dag.classes.size());
+    ResultLayout &rl = e->rl;
+    int o = 0;
+    rl.total = o++;
+    rl.per_part = o;
+    o += n_parts;
+    rl.root_logl = o;
+    o += n_parts * nc;
+    rl.tree_logl = o;
+    o += n_parts * nc;
+    rl.alpha = o;
+    o += n_parts * nc;
+    rl.dbrlen = o;
+    o += e->E;
+    rl.part_dbrlen = o;
+    o += n_parts * e->E;
+    rl.dprob = o;
+    o += std::max(e->R, 1);
+    rl.size = o;
+    e->d_result.alloc(sizeof(double) * rl.size);
+    NRX_HIP(hipHostMalloc(reinterpret_cast<void **>(&e->h_result), sizeof(double) * rl.size,
+                          hipHostMallocDefault));
+
+    e->brlen.assign(e->E, 0.1);
+    e->probs.assign(e->R, 0.5);
+    e->edge_dirty.assign(e->E, 1);
+    e->clv_dirty.assign(nv, 1);
+    compute_class_probs(*e);
+    build_schedule(*e);
+    *out = e.release();
+    return NRX_OK;
+  });
+}
+
+int nrx_destroy(nrx_engine *eng) {
+  return guarded([&]() -> int {
+    delete static_cast<Engine *>(eng);
+    return NRX_OK;
+  });
+}
+
+int nrx_set_model(nrx_engine *eng, int32_t partition, const double *subst_rates,
+                  const double *freqs, double alpha) {
+  return guarded([&]() -> int {
+    Engine &e = *static_cast<Engine *>(eng);
+    if (!eng || partition < 0 || partition >= e.P || !subst_rates || !freqs)
+      throw std::invalid_argument("bad model arguments");
+    PartInfo &pi = e.parts[partition];
+    build_model(e.S, subst_rates, freqs, pi.model);
+    pi.rates = gamma_mean_rates(alpha, e.C);
+    pi.model_set = true;
+    e.model_dirty = true;
+    return NRX_OK;
+  });
+}
+
+int nrx_set_category_rates(nrx_engine *eng, int32_t partition, const double *rates) {
+  return guarded([&]() -> int {
+    Engine &e = *static_cast<Engine *>(eng);
+    if (!eng || partition < 0 || partition >= e.P || !rates)
+      throw std::invalid_argument("bad arguments");
+    PartInfo &pi = e.parts[partition];
+    if (!pi.model_set) throw std::runtime_error("set the model before the category rates");
+    for (int c = 0; c < e.C; ++c)
+      if (!(rates[c] >= 0.0)) throw std::invalid_argument("category rates must be >= 0");
+    pi.rates.assign(rates, rates + e.C);
+    e.model_dirty = true;
+    return NRX_OK;
+  });
+}
+
+int nrx_set_branch_lengths(nrx_engine *eng, const double *brlen) {
+  return guarded([&]() -> int {
+    Engine &e = *static_cast<Engine *>(eng);
+    if (!eng || !brlen) throw std::invalid_argument("null argument");
+    for (int i = 0; i < e.E; ++i) {
+      if (std::isnan(brlen[i])) throw std::invalid_argument("branch length is NaN");
+      // clamp as createNetworkPllTreeinfo does (RaxmlWrapper.cpp:652-663)
+      const double b = std::min(std::max(brlen[i], e.opt.brlen_min), e.opt.brlen_max);
+      if (b != e.brlen[i]) {
+        e.brlen[i] = b;
+        e.edge_dirty[i] = 1;
+      }
+    }
+    return NRX_OK;
+  });
+}
+
+int nrx_set_reticulation_probs(nrx_engine *eng, const double *p) {
+  return guarded([&]() -> int {
+    Engine &e = *static_cast<Engine *>(eng);
+    if (!eng || (e.R > 0 && !p)) throw std::invalid_argument("null argument");
+    set_probs(e, p);
+    return NRX_OK;
+  });
+}
+
+int nrx_loglikelihood(nrx_engine *eng, int32_t variant, int32_t incremental,
+                      int32_t update_pmatrices, double *total, double *per_partition) {
+  return guarded([&]() -> int {
+    if (!eng || !total) throw std::invalid_argument("null argument");
+    Engine &e = *static_cast<Engine *>(eng);
+    const int rc = evaluate(e, variant, incremental != 0, update_pmatrices != 0, false);
+    if (rc != NRX_OK) return rc;
+    *total = e.cached_total;
+    if (per_partition)
+      std::copy(e.cached_per_part.begin(), e.cached_per_part.end(), per_partition);
+    return NRX_OK;
+  });
+}
+
+int nrx_gradient(nrx_engine *eng, int32_t variant, double *total, double *dbrlen, double *dprob,
+                 double *per_partition_dbrlen) {
+  return guarded([&]() -> int {
+    if (!eng || !total || !dbrlen) throw std::invalid_argument("null argument");
+    Engine &e = *static_cast<Engine *>(eng);
+    const int rc = evaluate(e, variant, true, true, true);
+    if (rc != NRX_OK) return rc;
+    *total = e.cached_total;
+    std::copy(e.h_result + e.rl.dbrlen, e.h_result + e.rl.dbrlen + e.E, dbrlen);
+    if (dprob && e.R > 0) std::copy(e.h_result + e.rl.dprob, e.h_result + e.rl.dprob + e.R, dprob);
+    if (per_partition_dbrlen)
+      std::copy(e.h_result + e.rl.part_dbrlen, e.h_result + e.rl.part_dbrlen + e.P * e.E,
+                per_partition_dbrlen);
+    return NRX_OK;
+  });
+}
+
+int nrx_brlen_derivatives(nrx_engine *eng, int32_t variant, int32_t edge, int32_t mode,
+                          double brlen, double *d1, double *d2, double *part_d1,
+                          double *part_d2) {
+  (void)eng;
+  (void)variant;
+  (void)edge;
+  (void)mode;
+  (void)brlen;
+  (void)d1;
+  (void)d2;
+  (void)part_d1;
+  (void)part_d2;
+  return fail(NRX_ERR_UNSUPPORTED, "nrx_brlen_derivatives is not available in this build");
+}
+
+int nrx_tree_logls(nrx_engine *eng, double *tree_logl, double *class_logprob) {
+  return guarded([&]() -> int {
+    if (!eng) throw std::invalid_argument("null argument");
+    Engine &e = *static_cast<Engine *>(eng);
+    if (!e.cache_valid) throw std::runtime_error("no evaluation yet");
+    const int nc = static_cast<int>(e.dag.classes.size());
+    if (tree_logl)
+      std::copy(e.h_result + e.rl.tree_logl, e.h_result + e.rl.tree_logl + e.P * nc, tree_logl);
+    if (class_logprob) std::copy(e.class_logprob.begin(), e.class_logprob.end(), class_logprob);
+    return NRX_OK;
+  });
+}
+
+int nrx_get_stats(nrx_engine *eng, nrx_stats *out) {
+  return guarded([&]() -> int {
+    if (!eng || !out) throw std::invalid_argument("null argument");
+    Engine &e = *static_cast<Engine *>(eng);
+    std::memset(out, 0, sizeof(*out));
+    out->n_variants = static_cast<int>(e.dag.var.size());
+    out->n_ops_needed = static_cast<int>(e.ops_all.size());
+    out->n_levels = static_cast<int>(e.level_ops.size()) - 1;
+    out->n_classes = static_cast<int>(e.dag.classes.size());
+    out->n_root_clvs = static_cast<int>(e.root_slots.size());
+    out->n_uses = e.n_uses;
+    out->n_sites_padded = e.L.n_sites_pad;
+    // algorithmic bytes over the real (unpadded) sites
+    double sites = 0.0;
+    for (const auto &p : e.parts) sites += p.n_sites;
+    const double b = e.opt.precision == NRX_FP64 ? 8.0 : 4.0;
+    const double row = sites * e.C * e.S * b;  // one CLV over all sites
+    const double tip_b = e.S == 4 ? 1.0 : 4.0;
+    double fwd = 0.0;
+    for (const DevOp &op : e.ops_all) {
+      for (int k = 0; k < op.nk; ++k)
+        fwd += op.kid[k] >= 0 ? row + 4.0 * sites : tip_b * sites;
+      fwd += row + 4.0 * sites;
+    }
+    fwd += e.root_slots.size() * (row + 4.0 * sites + 8.0 * sites);
+    fwd += static_cast<double>(e.P) * e.E * e.C * e.S * e.S * b;
+    double adj = 0.0;
+    for (const DevChild &ch : e.children) {
+      adj += ch.child >= 0 ? row + 4.0 * sites : tip_b * sites;  // own CLV
+      for (int u = ch.use_begin; u < ch.use_end; ++u) {
+        const DevUse &us = e.uses[u];
+        adj += row + 4.0 * sites;  // parent adjoint + scaler
+        if (us.sib != NO_CHILD) adj += us.sib >= 0 ? row + 4.0 * sites : tip_b * sites;
+      }
+      if (ch.child >= 0) adj += row;  // adjoint written
+    }
+    out->fwd_bytes = fwd;
+    out->grad_bytes = fwd + adj;
+    out->clv_bytes = static_cast<double>(e.d_clv.bytes + e.d_scl.bytes + e.d_adj.bytes);
+    return NRX_OK;
+  });
+}
+
+int nrx_last_timing(nrx_engine *eng, double *total_ms, double *dominant_ms,
+                    int32_t *dominant_launches) {
+  return guarded([&]() -> int {
+    if (!eng) throw std::invalid_argument("null argument");
+    Engine &e = *static_cast<Engine *>(eng);
+    if (total_ms) *total_ms = e.last_total_ms;
+    if (dominant_ms) *dominant_ms = e.last_dom_ms;
+    if (dominant_launches) *dominant_launches = e.last_dom_launches;
+    return NRX_OK;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,122 @@
+// Internal structures of the MI355X network-likelihood engine.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/netrax_gpu.h"
+
+namespace nrx {
+
+// Sites of every partition are padded to a multiple of this tile; one tile
+// is the unit of work of every site-parallel kernel.
+constexpr int TILE = 64;
+constexpr int MAX_CATS = 8;
+
+// ---------------------------------------------------------------- model
+struct EigenModel {
+  int S = 0;
+  std::vector<double> eval;   // [S]
+  std::vector<double> V;      // [S][S]  P = V diag(exp(eval r t)) Vinv
+  std::vector<double> Vinv;   // [S][S]
+  std::vector<double> freqs;  // [S]
+};
+
+std::vector<double> gamma_mean_rates(double alpha, int ncat);
+void build_model(int S, const double *subst, const double *freqs, EigenModel &m);
+
+// ---------------------------------------------------------------- op DAG
+// Child code: >= 0 is a variant slot, < 0 is tip (-code - 1).
+constexpr int32_t NO_CHILD = INT32_MIN;
+inline int32_t tip_code(int tip) { return -tip - 1; }
+inline bool is_tip_code(int32_t c) { return c < 0 && c != NO_CHILD; }
+inline int tip_of(int32_t c) { return -c - 1; }
+
+struct Variant {
+  int node = -1;
+  int nkids = 0;
+  int32_t kid[2] = {NO_CHILD, NO_CHILD};
+  int32_t edge[2] = {-1, -1};
+  int level = 0;
+};
+
+struct RootClass {
+  std::vector<uint32_t> configs;  // member configurations
+  int dtroot_slot = -1;           // variant at the displayed-tree root
+  std::vector<uint64_t> edges;    // alive edge set (bitset over edges)
+};
+
+struct Dag {
+  int n_retic = 0;
+  std::vector<Variant> var;
+  std::vector<RootClass> classes;
+};
+
+// Enumerates the 2^r displayed-tree configurations and deduplicates CLVs
+// by (node, alive subtree).  Throws std::invalid_argument on bad input.
+Dag compile_dag(const nrx_network_desc &net);
+
+// ---------------------------------------------------------------- device
+struct DevOp {  // forward CLV op
+  int32_t out;
+  int32_t nk;
+  int32_t kid[2];
+  int32_t edge[2];
+};
+
+struct DevChild {  // adjoint work item: one CLV (or tip) and its consumers
+  int32_t child;
+  int32_t use_begin;
+  int32_t use_end;
+  int32_t root_index;  // >= 0 when the CLV is a displayed-tree root CLV
+};
+
+struct DevUse {  // child used by parent op `parent` across `edge`
+  int32_t parent;
+  int32_t edge;
+  int32_t sib;       // sibling child code or NO_CHILD for a unary op
+  int32_t sib_edge;
+  int32_t grad_slot; // row of the per-use gradient partials
+  int32_t pad;
+};
+
+struct DevClass {
+  double logprob;
+  int32_t root_index;
+  int32_t use;
+};
+
+// Everything a kernel needs about the site axis and the models.
+struct DevLayout {
+  int64_t n_sites_pad;   // sites over all partitions, tile padded
+  int32_t n_tiles;
+  int32_t n_tiles_grid;  // n_tiles rounded up to a multiple of 8 (XCDs)
+  int32_t n_parts;
+  int32_t n_edges;
+  int32_t S;
+  int32_t C;
+};
+
+struct Engine;
+
+// Kernel launchers (kernels.hip)
+void launch_pmatrices(Engine &e);
+void launch_forward_level(Engine &e, const DevOp *ops, int n_ops);
+void launch_root_partials(Engine &e);
+void launch_mixture(Engine &e, int variant);
+void launch_adjoint_level(Engine &e, const int32_t *children, int n_children);
+void launch_grad_reduce(Engine &e);
+void launch_forward_persistent(Engine &e, int n_ops_total);
+void launch_adjoint_persistent(Engine &e);
+void launch_sumtable(Engine &e, int edge);
+void launch_deriv_eval(Engine &e, double brlen);
+
+struct HipCheck {
+  static void check(hipError_t err, const char *what);
+};
+#define NRX_HIP(x) ::nrx::HipCheck::check((x), #x)
+
+}  // namespace nrx

@@ -1,0 +1,117 @@
+// Engine state shared by the host orchestration (engine.cpp) and the
+// kernel launchers (kernels.hip).
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "engine.hpp"
+
+struct nrx_engine {};
+
+namespace nrx {
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { release(); }
+  void alloc(size_t n) {
+    if (n <= bytes && p) return;
+    release();
+    if (n == 0) n = 16;
+    NRX_HIP(hipMalloc(&p, n));
+    bytes = n;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <typename T>
+  T *as() const {
+    return static_cast<T *>(p);
+  }
+};
+
+struct PartInfo {
+  int n_sites = 0;        // patterns
+  int64_t site_off = 0;   // first padded site
+  int tile_begin = 0, tile_end = 0;
+  bool model_set = false;
+  EigenModel model;
+  std::vector<double> rates;  // [C]
+};
+
+// Result block offsets (doubles) inside the device/host result buffers.
+struct ResultLayout {
+  int total, per_part, root_logl, tree_logl, alpha, dbrlen, part_dbrlen, dprob, size;
+};
+
+struct Engine : nrx_engine {
+  nrx_options opt{};
+  int N = 0, E = 0, R = 0, T = 0, root = 0;
+  std::vector<int32_t> edge_parent, edge_child, retic_first, retic_second;
+  int P = 0, S = 0, C = 0;
+  std::vector<PartInfo> parts;
+  DevLayout L{};
+  Dag dag;
+
+  // host parameters
+  std::vector<double> brlen;  // clamped
+  std::vector<double> probs;  // first-parent probabilities
+  std::vector<double> class_logprob, class_dlogprob;  // [nc], [nc][R]
+  std::vector<char> class_use;
+
+  // schedule (rebuilt when the set of used classes changes)
+  std::vector<char> needed;                 // [n_var]
+  std::vector<std::vector<int>> level_ops;  // variant slots per level (needed only)
+  std::vector<int> root_slots;              // distinct displayed-tree-root CLVs
+  std::vector<int> class_root_index;        // [nc]
+  std::vector<DevOp> ops_all;               // levels concatenated
+  std::vector<int> level_begin;             // [n_levels + 1] into ops_all
+  std::vector<DevChild> children;           // adjoint work items
+  std::vector<int32_t> child_lists;         // reverse level order
+  std::vector<int> child_level_begin;       // [n_levels + 1], level 0 = tips
+  std::vector<DevUse> uses;
+  std::vector<int32_t> edge_use_begin, edge_use_list;
+  int n_uses = 0;
+
+  // dirty tracking
+  bool model_dirty = true;
+  std::vector<char> edge_dirty;  // P needs recompute
+  std::vector<char> clv_dirty;   // [n_var]
+  bool probs_dirty = true;
+  bool cache_valid = false;
+  int cache_variant = -1;
+  double cached_total = 0.0;
+  std::vector<double> cached_per_part;
+
+  // device memory
+  hipStream_t stream = nullptr;
+  DevBuf d_tips, d_weights, d_tile_part, d_part_tile_begin;
+  DevBuf d_V, d_Vinv, d_eval, d_rates, d_freqs, d_brlen;
+  DevBuf d_P, d_dP, d_d2P, d_Pf, d_dPf;
+  DevBuf d_clv, d_scl, d_adj;
+  DevBuf d_ops, d_ops_dirty;
+  DevBuf d_children, d_child_lists, d_uses, d_edge_use_begin, d_edge_use_list;
+  DevBuf d_classes, d_class_dlogprob, d_root_slots;
+  DevBuf d_root_partials, d_beta, d_grad_partials;
+  DevBuf d_result;
+  double *h_result = nullptr;  // pinned
+  ResultLayout rl{};
+  size_t clv_elems_per_slot = 0;  // n_sites_pad * C * S
+
+  // timing
+  hipEvent_t ev_begin = nullptr, ev_end = nullptr;
+  std::vector<hipEvent_t> ev_dom;  // pairs around dominant-kernel launches
+  int n_dom = 0;
+  double last_total_ms = 0.0, last_dom_ms = 0.0;
+  int last_dom_launches = 0;
+
+  ~Engine();
+};
+
+}  // namespace nrx

@@ -1,0 +1,548 @@
+// HIP kernels of the network-likelihood engine (gfx950).
+//
+// Data layout in HBM (one engine):
+//   CLV arena   real [n_variants][n_sites_pad][C][S]  site-major, state innermost
+//   scalers     int32 [n_variants][n_sites_pad]
+//   adjoints    real  [n_variants][n_sites_pad][C][S]
+//   tips        uint8 (S <= 8) or uint32 (S = 20) state masks [n_tips][n_sites_pad]
+//   P, P', P''  real  [n_parts][n_edges][C][S][S]
+// Every site-parallel kernel works on tiles of TILE = 64 sites; a workgroup
+// holds TILE * C threads, thread = (site, rate category), so the C lanes of
+// a site are adjacent and a CLV row of a workgroup is one contiguous
+// TILE*C*S*sizeof(real) span.  Workgroup b of a per-level launch handles
+// tile b % n_tiles_grid of op b / n_tiles_grid; n_tiles_grid is a multiple
+// of 8, so a tile is always dispatched to the same XCD (round-robin over
+// workgroup ids) and its children's CLVs stay in that XCD's L2.
+#include <cfloat>
+#include <cmath>
+
+#include "engine_state.hpp"
+
+namespace nrx {
+
+void HipCheck::check(hipError_t err, const char *what) {
+  if (err != hipSuccess)
+    throw std::runtime_error(std::string("HIP error ") + hipGetErrorString(err) + " in " + what);
+}
+
+namespace {
+
+// libpll's per-site scaling constants (PLL_SCALE_THRESHOLD = 2^-256,
+// PLL_SCALE_FACTOR = 2^256): a site is rescaled when all its entries are
+// below the threshold; the scaler counts rescalings.
+constexpr double SCALE_THRESHOLD = 0x1p-256;
+constexpr double SCALE_FACTOR = 0x1p256;
+constexpr double LOG_SCALE_THRESHOLD = -256.0 * 0.69314718055994530941723212145818;
+
+template <int S>
+struct TipType {
+  using type = uint8_t;
+};
+template <>
+struct TipType<20> {
+  using type = uint32_t;
+};
+
+template <int C>
+__device__ __forceinline__ double cat_max(double m) {
+#pragma unroll
+  for (int off = 1; off < C; off <<= 1) m = fmax(m, __shfl_xor(m, off));
+  return m;
+}
+template <int C>
+__device__ __forceinline__ double cat_sum(double m) {
+#pragma unroll
+  for (int off = 1; off < C; off <<= 1) m += __shfl_xor(m, off);
+  return m;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+  return v;
+}
+
+// Deterministic workgroup sum; result valid in thread 0.
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double *sh) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < NT / 64; ++i) r += sh[i];
+  }
+  __syncthreads();
+  return r;
+}
+
+template <int S>
+__device__ __forceinline__ void load_clv(const double *__restrict__ p, double (&x)[S]) {
+  if constexpr (S == 4) {
+    const double2 a = reinterpret_cast<const double2 *>(p)[0];
+    const double2 b = reinterpret_cast<const double2 *>(p)[1];
+    x[0] = a.x;
+    x[1] = a.y;
+    x[2] = b.x;
+    x[3] = b.y;
+  } else {
+#pragma unroll
+    for (int i = 0; i < S; ++i) x[i] = p[i];
+  }
+}
+
+template <int S>
+__device__ __forceinline__ void store_clv(double *__restrict__ p, const double (&x)[S]) {
+  if constexpr (S == 4) {
+    reinterpret_cast<double2 *>(p)[0] = make_double2(x[0], x[1]);
+    reinterpret_cast<double2 *>(p)[1] = make_double2(x[2], x[3]);
+  } else {
+#pragma unroll
+    for (int i = 0; i < S; ++i) p[i] = x[i];
+  }
+}
+
+template <int S>
+__device__ __forceinline__ void tip_vec(uint32_t mask, double (&x)[S]) {
+#pragma unroll
+  for (int i = 0; i < S; ++i) x[i] = ((mask >> i) & 1u) ? 1.0 : 0.0;
+}
+
+// y = M x for one S x S row-major matrix
+template <int S>
+__device__ __forceinline__ void matvec(const double *__restrict__ M, const double (&x)[S],
+                                       double (&y)[S]) {
+#pragma unroll
+  for (int i = 0; i < S; ++i) {
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < S; ++j) s = fma(M[i * S + j], x[j], s);
+    y[i] = s;
+  }
+}
+
+// y = M^T x
+template <int S>
+__device__ __forceinline__ void matvec_t(const double *__restrict__ M, const double (&x)[S],
+                                         double (&y)[S]) {
+#pragma unroll
+  for (int j = 0; j < S; ++j) {
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < S; ++i) s = fma(M[i * S + j], x[i], s);
+    y[j] = s;
+  }
+}
+
+// ------------------------------------------------------------- P matrices
+// P_{p,e,c} = V diag(exp(lambda r_c t_e)) V^-1, with first and second
+// derivatives in t (libpll's pll_update_prob_matrices computes the same
+// product from the same eigendecomposition).
+__global__ void k_pmatrices(DevLayout L, const double *__restrict__ V,
+                            const double *__restrict__ Vinv, const double *__restrict__ ev,
+                            const double *__restrict__ rates, const double *__restrict__ brlen,
+                            double *__restrict__ P, double *__restrict__ dP,
+                            double *__restrict__ d2P, float *__restrict__ Pf,
+                            float *__restrict__ dPf) {
+  const int S = L.S, C = L.C;
+  const int64_t total = static_cast<int64_t>(L.n_parts) * L.n_edges * C * S * S;
+  const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (idx >= total) return;
+  const int j = idx % S;
+  const int i = (idx / S) % S;
+  const int c = (idx / (S * S)) % C;
+  const int e = (idx / (S * S * C)) % L.n_edges;
+  const int p = idx / (static_cast<int64_t>(S) * S * C * L.n_edges);
+  const double rt = rates[p * C + c];
+  const double t = brlen[e];
+  const double *Vp = V + static_cast<size_t>(p) * S * S;
+  const double *Wp = Vinv + static_cast<size_t>(p) * S * S;
+  const double *lp = ev + static_cast<size_t>(p) * S;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  for (int k = 0; k < S; ++k) {
+    const double lr = lp[k] * rt;
+    const double x = Vp[i * S + k] * exp(lr * t) * Wp[k * S + j];
+    a0 += x;
+    a1 += x * lr;
+    a2 += x * lr * lr;
+  }
+  P[idx] = a0;
+  dP[idx] = a1;
+  d2P[idx] = a2;
+  if (Pf) Pf[idx] = static_cast<float>(a0);
+  if (dPf) dPf[idx] = static_cast<float>(a1);
+}
+
+// ------------------------------------------------------- forward (FP64)
+template <int S, int C>
+__global__ void __launch_bounds__(TILE *C)
+    k_forward_level(const DevOp *__restrict__ ops, DevLayout L, double *__restrict__ clv,
+                    int32_t *__restrict__ scl,
+                    const typename TipType<S>::type *__restrict__ tips,
+                    const double *__restrict__ P, const int32_t *__restrict__ tile_part) {
+  const int tile = blockIdx.x % L.n_tiles_grid;
+  if (tile >= L.n_tiles) return;
+  const DevOp op = ops[blockIdx.x / L.n_tiles_grid];
+  const int c = threadIdx.x % C;
+  const int64_t site = static_cast<int64_t>(tile) * TILE + threadIdx.x / C;
+  const int part = tile_part[tile];
+  const int64_t n = L.n_sites_pad;
+  double v[S];
+#pragma unroll
+  for (int i = 0; i < S; ++i) v[i] = 1.0;
+  int sc = 0;
+  for (int k = 0; k < op.nk; ++k) {
+    double x[S], y[S];
+    const int32_t kid = op.kid[k];
+    if (kid >= 0) {
+      load_clv<S>(clv + (static_cast<size_t>(kid) * n + site) * (C * S) + c * S, x);
+      sc += scl[static_cast<size_t>(kid) * n + site];
+    } else {
+      tip_vec<S>(tips[static_cast<size_t>(-kid - 1) * n + site], x);
+    }
+    matvec<S>(P + ((static_cast<size_t>(part) * L.n_edges + op.edge[k]) * C + c) * (S * S), x, y);
+#pragma unroll
+    for (int i = 0; i < S; ++i) v[i] *= y[i];
+  }
+  double m = v[0];
+#pragma unroll
+  for (int i = 1; i < S; ++i) m = fmax(m, v[i]);
+  m = cat_max<C>(m);
+  if (m < SCALE_THRESHOLD) {
+#pragma unroll
+    for (int i = 0; i < S; ++i) v[i] *= SCALE_FACTOR;
+    sc += 1;
+  }
+  store_clv<S>(clv + (static_cast<size_t>(op.out) * n + site) * (C * S) + c * S, v);
+  if (c == 0) scl[static_cast<size_t>(op.out) * n + site] = sc;
+}
+
+// ---------------------------------------------------- root site sums (FP64)
+// partial[r][tile] = sum over the tile's sites of
+//   w_s * (log(sum_c (1/C) sum_i pi_i CLV[s,c,i]) + scaler_s * log(2^-256))
+// (pll_compute_root_loglikelihood semantics, ImprovedLoglikelihood.cpp:450).
+template <int S, int C>
+__global__ void __launch_bounds__(TILE *C)
+    k_root_partials(const int32_t *__restrict__ root_slots, DevLayout L,
+                    const double *__restrict__ clv, const int32_t *__restrict__ scl,
+                    const double *__restrict__ freqs, const double *__restrict__ weights,
+                    const int32_t *__restrict__ tile_part, double *__restrict__ partials) {
+  __shared__ double sh[TILE * C / 64];
+  const int tile = blockIdx.x % L.n_tiles_grid;
+  if (tile >= L.n_tiles) return;
+  const int r = blockIdx.x / L.n_tiles_grid;
+  const int slot = root_slots[r];
+  const int c = threadIdx.x % C;
+  const int64_t site = static_cast<int64_t>(tile) * TILE + threadIdx.x / C;
+  const int part = tile_part[tile];
+  const int64_t n = L.n_sites_pad;
+  double x[S];
+  load_clv<S>(clv + (static_cast<size_t>(slot) * n + site) * (C * S) + c * S, x);
+  double t = 0.0;
+#pragma unroll
+  for (int i = 0; i < S; ++i) t = fma(freqs[part * S + i], x[i], t);
+  t = cat_sum<C>(t) * (1.0 / C);
+  double val = 0.0;
+  if (c == 0) {
+    const double w = weights[site];
+    if (w != 0.0)
+      val = w * (log(t) + scl[static_cast<size_t>(slot) * n + site] * LOG_SCALE_THRESHOLD);
+  }
+  val = block_sum<TILE * C>(val, sh);
+  if (threadIdx.x == 0) partials[static_cast<size_t>(r) * L.n_tiles + tile] = val;
+}
+
+// ----------------------------------------------------------------- mixture
+// Per partition: lnL_p = log sum_t P(t) exp(lnL_{p,t}) (AVERAGE) or
+// max_t log P(t) + lnL_{p,t} (BEST) over root classes whose log-probability
+// passes min_interesting_tree_logprob (ImprovedLoglikelihood.cpp:521-604);
+// a max-shifted FP64 log-sum-exp stands in for the reference's MPFR sum.
+// Also produces the adjoint seeds beta[p][root CLV] = sum of the class
+// weights d lnL_p / d lnL_{p,t}, and d lnL / d p_r.
+__global__ void k_mixture(DevLayout L, int n_root, int nc, int R, int variant,
+                          const DevClass *__restrict__ classes,
+                          const double *__restrict__ class_dlogprob,
+                          const int32_t *__restrict__ part_tile_begin,
+                          const double *__restrict__ partials, double *__restrict__ beta,
+                          double *__restrict__ res, ResultLayout rl) {
+  const int P = L.n_parts;
+  for (int idx = threadIdx.x; idx < n_root * P; idx += blockDim.x) {
+    const int r = idx / P, p = idx % P;
+    double s = 0.0;
+    for (int t = part_tile_begin[p]; t < part_tile_begin[p + 1]; ++t)
+      s += partials[static_cast<size_t>(r) * L.n_tiles + t];
+    res[rl.root_logl + p * n_root + r] = s;
+  }
+  __syncthreads();
+  for (int p = threadIdx.x; p < P; p += blockDim.x) {
+    const double *rll = res + rl.root_logl + p * n_root;
+    double *alpha = res + rl.alpha + p * nc;
+    double best = -INFINITY;
+    int arg = -1;
+    for (int k = 0; k < nc; ++k) {
+      const double tl = classes[k].root_index >= 0 ? rll[classes[k].root_index] : 0.0;
+      res[rl.tree_logl + p * nc + k] = tl;
+      if (!classes[k].use) continue;
+      const double v = classes[k].logprob + tl;
+      if (v > best) {
+        best = v;
+        arg = k;
+      }
+    }
+    double lnl;
+    if (variant == 1) {
+      for (int k = 0; k < nc; ++k) alpha[k] = (k == arg) ? 1.0 : 0.0;
+      lnl = best;
+    } else {
+      double s = 0.0;
+      for (int k = 0; k < nc; ++k) {
+        const double a =
+            classes[k].use && best > -INFINITY
+                ? exp(classes[k].logprob + rll[classes[k].root_index] - best)
+                : 0.0;
+        alpha[k] = a;
+        s += a;
+      }
+      for (int k = 0; k < nc; ++k) alpha[k] = s > 0.0 ? alpha[k] / s : 0.0;
+      lnl = best + log(s);
+    }
+    res[rl.per_part + p] = lnl;
+    for (int r = 0; r < n_root; ++r) beta[p * n_root + r] = 0.0;
+    for (int k = 0; k < nc; ++k)
+      if (alpha[k] != 0.0) beta[p * n_root + classes[k].root_index] += alpha[k];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double tot = 0.0;
+    for (int p = 0; p < P; ++p) tot += res[rl.per_part + p];
+    res[rl.total] = tot;
+    for (int r = 0; r < R; ++r) {
+      double g = 0.0;
+      for (int p = 0; p < P; ++p)
+        for (int k = 0; k < nc; ++k)
+          g += res[rl.alpha + p * nc + k] * class_dlogprob[static_cast<size_t>(k) * R + r];
+      res[rl.dprob + r] = g;
+    }
+  }
+}
+
+// ------------------------------------------------------- adjoint (FP64)
+// Reverse pruning, pull form.  For a CLV o (or tip) with consumers q (parent
+// op across edge e, sibling term y = P_f sib or 1, rescale factor F of q):
+//   Abar_o[c,j] = seed_o[c,j] + sum_q sum_i Abar_q[c,i] F y[c,i] P_e[c,i,j]
+//   dJ/dt_e    += sum_{c,i} Abar_q[c,i] F y[c,i] (P'_e x_o)[c,i]
+// where J = sum_p sum_t beta_{p,t} lnL_{p,t} and Abar is the derivative of
+// J w.r.t. the *scaled* CLV values; the root seed is
+// beta * w_s * (1/C) pi_i / Ltilde_s.  Each CLV is written by exactly one
+// workgroup per tile and consumers are walked in a fixed order, so the
+// result is deterministic without atomics.
+template <int S, int C>
+__global__ void __launch_bounds__(TILE *C)
+    k_adjoint_level(const int32_t *__restrict__ child_list, const DevChild *__restrict__ children,
+                    const DevUse *__restrict__ uses, DevLayout L, const double *__restrict__ clv,
+                    const int32_t *__restrict__ scl, double *__restrict__ adj,
+                    const typename TipType<S>::type *__restrict__ tips,
+                    const double *__restrict__ P, const double *__restrict__ dP,
+                    const double *__restrict__ freqs, const double *__restrict__ weights,
+                    const int32_t *__restrict__ tile_part, const double *__restrict__ beta,
+                    int n_root, double *__restrict__ grad_partials) {
+  __shared__ double sh[TILE * C / 64];
+  const int tile = blockIdx.x % L.n_tiles_grid;
+  if (tile >= L.n_tiles) return;
+  const DevChild ch = children[child_list[blockIdx.x / L.n_tiles_grid]];
+  const int c = threadIdx.x % C;
+  const int64_t site = static_cast<int64_t>(tile) * TILE + threadIdx.x / C;
+  const int part = tile_part[tile];
+  const int64_t n = L.n_sites_pad;
+  const size_t row = static_cast<size_t>(C * S);
+
+  double xo[S];
+  int ko = 0;
+  if (ch.child >= 0) {
+    load_clv<S>(clv + (static_cast<size_t>(ch.child) * n + site) * row + c * S, xo);
+    ko = scl[static_cast<size_t>(ch.child) * n + site];
+  } else {
+    tip_vec<S>(tips[static_cast<size_t>(-ch.child - 1) * n + site], xo);
+  }
+  double a[S];
+#pragma unroll
+  for (int i = 0; i < S; ++i) a[i] = 0.0;
+  if (ch.root_index >= 0) {
+    double t = 0.0;
+#pragma unroll
+    for (int i = 0; i < S; ++i) t = fma(freqs[part * S + i], xo[i], t);
+    t = cat_sum<C>(t) * (1.0 / C);
+    const double b = beta[part * n_root + ch.root_index] * weights[site];
+    if (b != 0.0) {
+      const double f = b * (1.0 / C) / t;
+#pragma unroll
+      for (int i = 0; i < S; ++i) a[i] = f * freqs[part * S + i];
+    }
+  }
+  for (int u = ch.use_begin; u < ch.use_end; ++u) {
+    const DevUse us = uses[u];
+    double aq[S], y[S], z[S];
+    load_clv<S>(adj + (static_cast<size_t>(us.parent) * n + site) * row + c * S, aq);
+    const int kq = scl[static_cast<size_t>(us.parent) * n + site];
+    int ks = 0;
+    if (us.sib == NO_CHILD) {
+#pragma unroll
+      for (int i = 0; i < S; ++i) y[i] = 1.0;
+    } else {
+      double xs[S];
+      if (us.sib >= 0) {
+        load_clv<S>(clv + (static_cast<size_t>(us.sib) * n + site) * row + c * S, xs);
+        ks = scl[static_cast<size_t>(us.sib) * n + site];
+      } else {
+        tip_vec<S>(tips[static_cast<size_t>(-us.sib - 1) * n + site], xs);
+      }
+      matvec<S>(P + ((static_cast<size_t>(part) * L.n_edges + us.sib_edge) * C + c) * (S * S), xs,
+                y);
+    }
+    const double F = (kq - ko - ks) > 0 ? SCALE_FACTOR : 1.0;
+#pragma unroll
+    for (int i = 0; i < S; ++i) z[i] = aq[i] * y[i] * F;
+    const size_t poff = ((static_cast<size_t>(part) * L.n_edges + us.edge) * C + c) * (S * S);
+    double back[S], dx[S];
+    matvec_t<S>(P + poff, z, back);
+#pragma unroll
+    for (int i = 0; i < S; ++i) a[i] += back[i];
+    matvec<S>(dP + poff, xo, dx);
+    double g = 0.0;
+#pragma unroll
+    for (int i = 0; i < S; ++i) g = fma(z[i], dx[i], g);
+    g = block_sum<TILE * C>(g, sh);
+    if (threadIdx.x == 0) grad_partials[static_cast<size_t>(us.grad_slot) * L.n_tiles + tile] = g;
+  }
+  if (ch.child >= 0)
+    store_clv<S>(adj + (static_cast<size_t>(ch.child) * n + site) * row + c * S, a);
+}
+
+// ------------------------------------------------------- gradient reduce
+// part_dbrlen[p][e] = sum over the uses of edge e and the tiles of partition p
+// (fixed assignment of (use, tile) pairs to threads + tree reduction).
+__global__ void __launch_bounds__(256)
+    k_grad_reduce(DevLayout L, const int32_t *__restrict__ edge_use_begin,
+                  const int32_t *__restrict__ edge_use_list,
+                  const int32_t *__restrict__ part_tile_begin,
+                  const double *__restrict__ grad_partials, double *__restrict__ res,
+                  ResultLayout rl) {
+  __shared__ double sh[4];
+  const int P = L.n_parts;
+  const int e = blockIdx.x / P, p = blockIdx.x % P;
+  const int u0 = edge_use_begin[e], nu = edge_use_begin[e + 1] - u0;
+  const int t0 = part_tile_begin[p], nt = part_tile_begin[p + 1] - t0;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nu * nt; i += 256) {
+    const int u = edge_use_list[u0 + i / nt];
+    s += grad_partials[static_cast<size_t>(u) * L.n_tiles + t0 + i % nt];
+  }
+  s = block_sum<256>(s, sh);
+  if (threadIdx.x == 0) res[rl.part_dbrlen + p * L.n_edges + e] = s;
+}
+
+__global__ void k_grad_total(DevLayout L, double *__restrict__ res, ResultLayout rl) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= L.n_edges) return;
+  double s = 0.0;
+  for (int p = 0; p < L.n_parts; ++p) s += res[rl.part_dbrlen + p * L.n_edges + e];
+  res[rl.dbrlen + e] = s;
+}
+
+template <typename F>
+void dispatch_sc(int S, int C, F &&f) {
+  // instantiated shapes: DNA (4 states) and protein (20 states) with
+  // 1, 2, 4 or 8 rate categories
+  auto withC = [&](auto s_tag) {
+    constexpr int SS = decltype(s_tag)::value;
+    switch (C) {
+      case 1: f(std::integral_constant<int, SS>{}, std::integral_constant<int, 1>{}); break;
+      case 2: f(std::integral_constant<int, SS>{}, std::integral_constant<int, 2>{}); break;
+      case 4: f(std::integral_constant<int, SS>{}, std::integral_constant<int, 4>{}); break;
+      case 8: f(std::integral_constant<int, SS>{}, std::integral_constant<int, 8>{}); break;
+      default: throw std::invalid_argument("unsupported number of rate categories");
+    }
+  };
+  if (S == 4)
+    withC(std::integral_constant<int, 4>{});
+  else if (S == 20)
+    withC(std::integral_constant<int, 20>{});
+  else
+    throw std::invalid_argument("unsupported number of states");
+}
+
+}  // namespace
+
+void launch_pmatrices(Engine &e) {
+  const int64_t total = static_cast<int64_t>(e.P) * e.E * e.C * e.S * e.S;
+  const int nb = static_cast<int>((total + 255) / 256);
+  hipLaunchKernelGGL(k_pmatrices, dim3(nb), dim3(256), 0, e.stream, e.L, e.d_V.as<double>(),
+                     e.d_Vinv.as<double>(), e.d_eval.as<double>(), e.d_rates.as<double>(),
+                     e.d_brlen.as<double>(), e.d_P.as<double>(), e.d_dP.as<double>(),
+                     e.d_d2P.as<double>(), e.d_Pf.as<float>(), e.d_dPf.as<float>());
+  NRX_HIP(hipGetLastError());
+}
+
+void launch_forward_level(Engine &e, const DevOp *ops, int n_ops) {
+  if (n_ops == 0) return;
+  dispatch_sc(e.S, e.C, [&](auto s, auto c) {
+    constexpr int S = decltype(s)::value, C = decltype(c)::value;
+    using TT = typename TipType<S>::type;
+    hipLaunchKernelGGL((k_forward_level<S, C>), dim3(n_ops * e.L.n_tiles_grid), dim3(TILE * C), 0,
+                       e.stream, ops, e.L, e.d_clv.as<double>(), e.d_scl.as<int32_t>(),
+                       e.d_tips.as<TT>(), e.d_P.as<double>(), e.d_tile_part.as<int32_t>());
+  });
+  NRX_HIP(hipGetLastError());
+}
+
+void launch_root_partials(Engine &e) {
+  const int nr = static_cast<int>(e.root_slots.size());
+  dispatch_sc(e.S, e.C, [&](auto s, auto c) {
+    constexpr int S = decltype(s)::value, C = decltype(c)::value;
+    hipLaunchKernelGGL((k_root_partials<S, C>), dim3(nr * e.L.n_tiles_grid), dim3(TILE * C), 0,
+                       e.stream, e.d_root_slots.as<int32_t>(), e.L, e.d_clv.as<double>(),
+                       e.d_scl.as<int32_t>(), e.d_freqs.as<double>(), e.d_weights.as<double>(),
+                       e.d_tile_part.as<int32_t>(), e.d_root_partials.as<double>());
+  });
+  NRX_HIP(hipGetLastError());
+}
+
+void launch_mixture(Engine &e, int variant) {
+  hipLaunchKernelGGL(k_mixture, dim3(1), dim3(256), 0, e.stream, e.L,
+                     static_cast<int>(e.root_slots.size()), static_cast<int>(e.dag.classes.size()),
+                     e.R, variant, e.d_classes.as<DevClass>(), e.d_class_dlogprob.as<double>(),
+                     e.d_part_tile_begin.as<int32_t>(), e.d_root_partials.as<double>(),
+                     e.d_beta.as<double>(), e.d_result.as<double>(), e.rl);
+  NRX_HIP(hipGetLastError());
+}
+
+void launch_adjoint_level(Engine &e, const int32_t *children, int n_children) {
+  if (n_children == 0) return;
+  dispatch_sc(e.S, e.C, [&](auto s, auto c) {
+    constexpr int S = decltype(s)::value, C = decltype(c)::value;
+    using TT = typename TipType<S>::type;
+    hipLaunchKernelGGL((k_adjoint_level<S, C>), dim3(n_children * e.L.n_tiles_grid),
+                       dim3(TILE * C), 0, e.stream, children, e.d_children.as<DevChild>(),
+                       e.d_uses.as<DevUse>(), e.L, e.d_clv.as<double>(), e.d_scl.as<int32_t>(),
+                       e.d_adj.as<double>(), e.d_tips.as<TT>(), e.d_P.as<double>(),
+                       e.d_dP.as<double>(), e.d_freqs.as<double>(), e.d_weights.as<double>(),
+                       e.d_tile_part.as<int32_t>(), e.d_beta.as<double>(),
+                       static_cast<int>(e.root_slots.size()), e.d_grad_partials.as<double>());
+  });
+  NRX_HIP(hipGetLastError());
+}
+
+void launch_grad_reduce(Engine &e) {
+  hipLaunchKernelGGL(k_grad_reduce, dim3(e.E * e.P), dim3(256), 0, e.stream, e.L,
+                     e.d_edge_use_begin.as<int32_t>(), e.d_edge_use_list.as<int32_t>(),
+                     e.d_part_tile_begin.as<int32_t>(), e.d_grad_partials.as<double>(),
+                     e.d_result.as<double>(), e.rl);
+  NRX_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_grad_total, dim3((e.E + 255) / 256), dim3(256), 0, e.stream, e.L,
+                     e.d_result.as<double>(), e.rl);
+  NRX_HIP(hipGetLastError());
+}
+
+}  // namespace nrx

@@ -1,0 +1,208 @@
+"""GPU engine vs the oracle / golden vectors (FP64 plain form).
+
+Mirrors the reference's likelihood tests (test/src/LikelihoodTest.cpp):
+repeated evaluation is bit-identical, incremental == full recomputation,
+and every lnL/gradient is compared with the CPU oracle at 1e-9 relative
+(the north-star bar for the plain FP64 form).
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from fixtures import SAMPLE_PAIRS, dead_retic_sample, fixed_model, load_golden, load_sample
+from netrax_amd import AnnotatedNetwork, LikelihoodVariant, NetraxOptions, computeLoglikelihood
+from netrax_amd.simulate import make_workload, workload_small_1k, workload_30taxa_10k
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-9  # plain FP64 form vs the FP64 oracle
+GOLD = load_golden()
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def engine(net, parts, variant=0, **kw):
+    opts = NetraxOptions(likelihood_variant=LikelihoodVariant(variant), **kw)
+    return AnnotatedNetwork(net, parts, opts)
+
+
+def _golden_model():
+    m = GOLD["model"]
+    from netrax_amd.simulate import Model
+    return Model(4, np.array(m["subst_rates"]), np.array(m["freqs"]), m["alpha"], m["ncat"])
+
+
+@pytest.mark.parametrize("name", sorted(GOLD["cases"]))
+def test_golden_vectors(name):
+    say("golden", name)
+    case = GOLD["cases"][name]
+    model = _golden_model()
+    net, parts = dead_retic_sample(model) if name == "dead_retic" else load_sample(name, model)
+    for var, key in ((0, "average"), (1, "best")):
+        ann = engine(net, parts, var)
+        got = computeLoglikelihood(ann)
+        want = case[f"lnl_{key}"]
+        assert abs(got - want) <= RTOL * abs(want), (name, key, got, want)
+        if key == "average" and "grad_average" in case:
+            _, g, _ = ann.gradient()
+            assert np.allclose(g, case["grad_average"], rtol=1e-8, atol=1e-9)
+        ann.close()
+
+
+@pytest.mark.parametrize("name", sorted(SAMPLE_PAIRS))
+def test_samples_match_oracle(name):
+    say("sample", name)
+    net, parts = load_sample(name, fixed_model())
+    for var in (0, 1):
+        ann = engine(net, parts, var)
+        got = computeLoglikelihood(ann, incremental=0)
+        want = O.loglikelihood(net, parts, var)[0]
+        assert abs(got - want) <= RTOL * abs(want), (name, var, got, want)
+        lnl, g, gp = ann.gradient()
+        _, og, ogp = O.gradient(net, parts, var)
+        assert np.allclose(g, og, rtol=1e-7, atol=1e-8 * max(1.0, np.abs(og).max())), (name, var)
+        assert np.allclose(gp, ogp, rtol=1e-7, atol=1e-8), (name, var)
+        ann.close()
+
+
+def test_celine_zero_branch_lengths_match_oracle():
+    say("celine with clamped zero lengths")
+    net, parts = load_sample("celine", fixed_model(), with_lengths=False)
+    ann = engine(net, parts)
+    got = computeLoglikelihood(ann)
+    want = O.loglikelihood(net, parts)[0]
+    assert abs(got - want) <= RTOL * abs(want)
+
+
+def test_tree_logls_and_classes_match_oracle():
+    net, parts = load_sample("three_reticulations", fixed_model())
+    ann = engine(net, parts)
+    computeLoglikelihood(ann)
+    tl, lp = ann.tree_logls()
+    _, _, otl, olp = O.loglikelihood(net, parts, tree_logls=True)
+    assert tl.shape == otl.shape
+    assert np.allclose(lp, olp, rtol=0, atol=1e-14)
+    assert np.allclose(tl, otl, rtol=1e-11)
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_config_8taxa_1retic_1k_sites(seed):
+    say("8 taxa / 1 reticulation / 1k sites, seed", seed)
+    w = workload_small_1k(seed)
+    for var in (0, 1):
+        ann = engine(w.network, w.partitions, var)
+        got = computeLoglikelihood(ann)
+        want = O.loglikelihood(w.network, w.partitions, var)[0]
+        assert abs(got - want) <= RTOL * abs(want)
+        lnl, g, gp = ann.gradient()
+        _, og, ogp = O.gradient(w.network, w.partitions, var)
+        assert np.allclose(g, og, rtol=1e-7, atol=1e-7)
+        assert np.allclose(gp, ogp, rtol=1e-7, atol=1e-7)
+
+
+def test_config_30taxa_3retic_four_partitions():
+    say("30 taxa / 3 reticulations / 1k+2k+3k+4k sites")
+    w = workload_30taxa_10k()
+    for var in (0, 1):
+        ann = engine(w.network, w.partitions, var)
+        got = computeLoglikelihood(ann)
+        per = np.array(ann.partition_loglh)
+        want, wper = O.loglikelihood(w.network, w.partitions, var)
+        assert abs(got - want) <= RTOL * abs(want)
+        assert np.allclose(per, wper, rtol=RTOL)
+        lnl, g, gp, pg = ann.gradient(per_partition=True)
+        _, og, ogp, opg = O.gradient(w.network, w.partitions, var, per_partition=True)
+        scale = max(1.0, np.abs(og).max())
+        assert np.allclose(g, og, rtol=1e-7, atol=1e-9 * scale)
+        assert np.allclose(pg, opg, rtol=1e-7, atol=1e-9 * scale)
+        assert np.allclose(gp, ogp, rtol=1e-7, atol=1e-9 * scale)
+
+
+def test_repeated_evaluation_is_bit_identical():
+    """LikelihoodTest.cpp:237-239: repeated computeLoglikelihood(.,0,1)."""
+    w = workload_30taxa_10k()
+    ann = engine(w.network, w.partitions)
+    a = computeLoglikelihood(ann, 0, 1)
+    b = computeLoglikelihood(ann, 0, 1)
+    c = computeLoglikelihood(ann, 1, 1)
+    assert a == b == c
+    g1 = ann.gradient()[1]
+    g2 = ann.gradient()[1]
+    assert np.array_equal(g1, g2)
+
+
+def test_incremental_equals_full_recompute():
+    """LikelihoodTest.cpp:257-279 incrementalTest."""
+    w = workload_30taxa_10k()
+    ann = engine(w.network, w.partitions)
+    computeLoglikelihood(ann)
+    rng = np.random.default_rng(3)
+    for e in rng.choice(w.network.n_edges, 5, replace=False):
+        ann.set_branch_length(int(e), float(w.network.brlen[e]) * 1.7 + 0.01)
+        inc = computeLoglikelihood(ann, incremental=1)
+        full = computeLoglikelihood(ann, incremental=0)
+        assert inc == full
+        want = O.loglikelihood(w.network, w.partitions)[0]
+        assert abs(inc - want) <= RTOL * abs(want)
+
+
+def test_reticulation_probability_update_and_threshold():
+    """updateReticulationProb (LikelihoodTest.cpp:362-377) and the
+    min_interesting_tree_logprob cut (NetraxOptions.hpp:108)."""
+    net, parts = load_sample("two_reticulations", fixed_model())
+    ann = engine(net, parts)
+    computeLoglikelihood(ann)
+    for probs in ([0.3, 0.8], [1e-4, 0.5], [0.5, 1e-7]):
+        ann.set_reticulation_probs(np.array(probs))
+        got = computeLoglikelihood(ann)
+        want = O.loglikelihood(net, parts, probs=np.array(probs))[0]
+        assert abs(got - want) <= RTOL * abs(want), (probs, got, want)
+
+
+def test_gradient_matches_central_differences_50taxa():
+    """Full-size headline shape (50 taxa, 4 reticulations, 50k sites):
+    gradient vs central differences (h = 1e-6) on a few branches and on
+    every reticulation probability."""
+    say("50 taxa / 4 reticulations / 50k sites FD")
+    w = make_workload("50taxa_4retic_50k", 50, 4, [20000, 30000], seed=3)
+    ann = engine(w.network, w.partitions)
+    lnl, g, gp = ann.gradient()
+    h = 1e-6
+    rng = np.random.default_rng(0)
+    for e in rng.choice(w.network.n_edges, 6, replace=False):
+        b0 = w.network.brlen.copy()
+        b = b0.copy()
+        b[e] += h
+        ann.set_branch_lengths(b)
+        up = computeLoglikelihood(ann)
+        b[e] -= 2 * h
+        ann.set_branch_lengths(b)
+        dn = computeLoglikelihood(ann)
+        ann.set_branch_lengths(b0)
+        fd = (up - dn) / (2 * h)
+        assert abs(fd - g[e]) < 2e-4 * max(1.0, abs(g[e])), (e, fd, g[e])
+    p0 = w.network.retic_prob.copy()
+    for r in range(w.network.n_retic):
+        p = p0.copy()
+        p[r] += h
+        ann.set_reticulation_probs(p)
+        up = computeLoglikelihood(ann)
+        p[r] -= 2 * h
+        ann.set_reticulation_probs(p)
+        dn = computeLoglikelihood(ann)
+        ann.set_reticulation_probs(p0)
+        fd = (up - dn) / (2 * h)
+        assert abs(fd - gp[r]) < 2e-4 * max(1.0, abs(gp[r])), (r, fd, gp[r])
+
+
+def test_average_bounds_best():
+    w = workload_30taxa_10k()
+    a = computeLoglikelihood(engine(w.network, w.partitions, 0))
+    b = computeLoglikelihood(engine(w.network, w.partitions, 1))
+    assert a >= b
