@@ -109,10 +109,25 @@ typedef struct {
   int32_t n_root_clvs;     /* distinct CLVs the root kernel reads */
   int32_t n_uses;          /* (parent op, child) pairs walked by the adjoint */
   int64_t n_sites_padded;  /* sites over all partitions after tile padding */
-  double fwd_bytes;        /* algorithmic HBM bytes of one forward pass */
-  double grad_bytes;       /* algorithmic HBM bytes of forward + adjoint */
+  /* algorithmic HBM bytes over the real (unpadded) sites: each CLV, scaler
+   * and tip row counted once per read or write an op makes */
+  double fwd_op_bytes;     /* pruning ops of one full forward pass */
+  double root_bytes;       /* displayed-tree root site sums */
+  double adj_bytes;        /* adjoint ops of one reverse pass */
+  double fwd_bytes;        /* P matrices + pruning + root */
+  double grad_bytes;       /* fwd_bytes + adj_bytes */
   double clv_bytes;        /* device bytes held by CLVs, scalers and adjoints */
 } nrx_stats;
+
+/* Device time of the last nrx_loglikelihood / nrx_gradient call (HIP
+ * events on the engine's stream; requires nrx_options.timing). */
+typedef struct {
+  double total_ms;
+  double fwd_ms;           /* pruning kernel(s), summed over launches */
+  int32_t fwd_launches;
+  int32_t adj_launches;
+  double adj_ms;           /* adjoint kernel(s), summed over launches */
+} nrx_timing;
 
 void nrx_default_options(nrx_options *opt);
 const char *nrx_last_error(void);
@@ -159,10 +174,8 @@ int nrx_tree_logls(nrx_engine *eng, double *tree_logl, double *class_logprob);
 
 int nrx_get_stats(nrx_engine *eng, nrx_stats *out);
 
-/* Milliseconds of the device work of the last nrx_loglikelihood /
- * nrx_gradient call, and of its dominant kernel summed over launches. */
-int nrx_last_timing(nrx_engine *eng, double *total_ms, double *dominant_ms,
-                    int32_t *dominant_launches);
+/* Device timing of the last evaluation (zeros unless timing is enabled). */
+int nrx_last_timing(nrx_engine *eng, nrx_timing *out);
 
 #ifdef __cplusplus
 }

@@ -49,8 +49,14 @@ class Options(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("n_variants", C.c_int32), ("n_ops_needed", C.c_int32), ("n_levels", C.c_int32),
                 ("n_classes", C.c_int32), ("n_root_clvs", C.c_int32), ("n_uses", C.c_int32),
-                ("n_sites_padded", C.c_int64), ("fwd_bytes", C.c_double),
+                ("n_sites_padded", C.c_int64), ("fwd_op_bytes", C.c_double),
+                ("root_bytes", C.c_double), ("adj_bytes", C.c_double), ("fwd_bytes", C.c_double),
                 ("grad_bytes", C.c_double), ("clv_bytes", C.c_double)]
+
+
+class Timing(C.Structure):
+    _fields_ = [("total_ms", C.c_double), ("fwd_ms", C.c_double), ("fwd_launches", C.c_int32),
+                ("adj_launches", C.c_int32), ("adj_ms", C.c_double)]
 
 
 _lib = None
@@ -84,7 +90,7 @@ def load(build_if_missing: bool = True):
                                           pd, pd, pd, pd]
     lib.nrx_tree_logls.argtypes = [C.c_void_p, pd, pd]
     lib.nrx_get_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
-    lib.nrx_last_timing.argtypes = [C.c_void_p, pd, pd, C.POINTER(C.c_int32)]
+    lib.nrx_last_timing.argtypes = [C.c_void_p, C.POINTER(Timing)]
     _lib = lib
     return lib
 

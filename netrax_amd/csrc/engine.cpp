@@ -59,7 +59,8 @@ Engine::~Engine() {
   if (h_result) (void)hipHostFree(h_result);
   if (ev_begin) (void)hipEventDestroy(ev_begin);
   if (ev_end) (void)hipEventDestroy(ev_end);
-  for (auto ev : ev_dom) (void)hipEventDestroy(ev);
+  for (auto &v : ev_k)
+    for (auto ev : v) (void)hipEventDestroy(ev);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -309,22 +310,23 @@ static void update_pmatrices(Engine &e, bool do_update) {
   std::fill(e.edge_dirty.begin(), e.edge_dirty.end(), 0);
 }
 
-static void record_dom_begin(Engine &e) {
+static void record_kernel_begin(Engine &e, int kind) {
   if (!e.opt.timing) return;
-  if (static_cast<int>(e.ev_dom.size()) < 2 * (e.n_dom + 1)) {
+  auto &evs = e.ev_k[kind];
+  if (static_cast<int>(evs.size()) < 2 * (e.n_k[kind] + 1)) {
     for (int i = 0; i < 2; ++i) {
       hipEvent_t ev;
       NRX_HIP(hipEventCreate(&ev));
-      e.ev_dom.push_back(ev);
+      evs.push_back(ev);
     }
   }
-  NRX_HIP(hipEventRecord(e.ev_dom[2 * e.n_dom], e.stream));
+  NRX_HIP(hipEventRecord(evs[2 * e.n_k[kind]], e.stream));
 }
 
-static void record_dom_end(Engine &e) {
+static void record_kernel_end(Engine &e, int kind) {
   if (!e.opt.timing) return;
-  NRX_HIP(hipEventRecord(e.ev_dom[2 * e.n_dom + 1], e.stream));
-  e.n_dom++;
+  NRX_HIP(hipEventRecord(e.ev_k[kind][2 * e.n_k[kind] + 1], e.stream));
+  e.n_k[kind]++;
 }
 
 static void forward(Engine &e, bool incremental) {
@@ -339,9 +341,9 @@ static void forward(Engine &e, bool incremental) {
   if (all) {
     for (int l = 1; l < n_levels; ++l) {
       const int b = e.level_begin[l], n = e.level_begin[l + 1] - b;
-      record_dom_begin(e);
+      record_kernel_begin(e, 0);
       launch_forward_level(e, e.d_ops.as<DevOp>() + b, n);
-      record_dom_end(e);
+      record_kernel_end(e, 0);
     }
   } else {
     std::vector<DevOp> dirty;
@@ -357,9 +359,9 @@ static void forward(Engine &e, bool incremental) {
       for (int l = 1; l < n_levels; ++l) {
         const int b = begin[l], n = begin[l + 1] - b;
         if (n == 0) continue;
-        record_dom_begin(e);
+        record_kernel_begin(e, 0);
         launch_forward_level(e, e.d_ops_dirty.as<DevOp>() + b, n);
-        record_dom_end(e);
+        record_kernel_end(e, 0);
       }
     }
   }
@@ -367,7 +369,7 @@ static void forward(Engine &e, bool incremental) {
 }
 
 static void begin_timing(Engine &e) {
-  e.n_dom = 0;
+  e.n_k[0] = e.n_k[1] = 0;
   if (e.opt.timing) NRX_HIP(hipEventRecord(e.ev_begin, e.stream));
 }
 
@@ -377,14 +379,19 @@ static void end_timing(Engine &e) {
   NRX_HIP(hipEventSynchronize(e.ev_end));
   float ms = 0.f;
   NRX_HIP(hipEventElapsedTime(&ms, e.ev_begin, e.ev_end));
-  e.last_total_ms = ms;
-  double dom = 0.0;
-  for (int i = 0; i < e.n_dom; ++i) {
-    NRX_HIP(hipEventElapsedTime(&ms, e.ev_dom[2 * i], e.ev_dom[2 * i + 1]));
-    dom += ms;
-  }
-  e.last_dom_ms = dom;
-  e.last_dom_launches = e.n_dom;
+  nrx_timing t{};
+  t.total_ms = ms;
+  double acc[2] = {0.0, 0.0};
+  for (int k = 0; k < 2; ++k)
+    for (int i = 0; i < e.n_k[k]; ++i) {
+      NRX_HIP(hipEventElapsedTime(&ms, e.ev_k[k][2 * i], e.ev_k[k][2 * i + 1]));
+      acc[k] += ms;
+    }
+  t.fwd_ms = acc[0];
+  t.fwd_launches = e.n_k[0];
+  t.adj_ms = acc[1];
+  t.adj_launches = e.n_k[1];
+  e.last_timing = t;
 }
 
 static void fetch_results(Engine &e) {
@@ -413,7 +420,10 @@ static int evaluate(Engine &e, int variant, bool incremental, bool update_pm, bo
     const int nl = static_cast<int>(e.child_level_begin.size()) - 1;
     for (int i = 0; i < nl; ++i) {
       const int b = e.child_level_begin[i], n = e.child_level_begin[i + 1] - b;
+      if (n == 0) continue;
+      record_kernel_begin(e, 1);
       launch_adjoint_level(e, e.d_child_lists.as<int32_t>() + b, n);
+      record_kernel_end(e, 1);
     }
     launch_grad_reduce(e);
   }
@@ -743,40 +753,40 @@ int nrx_get_stats(nrx_engine *eng, nrx_stats *out) {
     for (const auto &p : e.parts) sites += p.n_sites;
     const double b = e.opt.precision == NRX_FP64 ? 8.0 : 4.0;
     const double row = sites * e.C * e.S * b;  // one CLV over all sites
-    const double tip_b = e.S == 4 ? 1.0 : 4.0;
-    double fwd = 0.0;
+    const double scl = 4.0 * sites;            // one scaler row
+    const double tip_b = (e.S == 4 ? 1.0 : 4.0) * sites;
+    double fwd_ops = 0.0;
     for (const DevOp &op : e.ops_all) {
-      for (int k = 0; k < op.nk; ++k)
-        fwd += op.kid[k] >= 0 ? row + 4.0 * sites : tip_b * sites;
-      fwd += row + 4.0 * sites;
+      for (int k = 0; k < op.nk; ++k) fwd_ops += op.kid[k] >= 0 ? row + scl : tip_b;
+      fwd_ops += row + scl;  // parent CLV and scaler written
     }
-    fwd += e.root_slots.size() * (row + 4.0 * sites + 8.0 * sites);
-    fwd += static_cast<double>(e.P) * e.E * e.C * e.S * e.S * b;
+    const double root = e.root_slots.size() * (row + scl + 8.0 * sites);
+    const double pm = static_cast<double>(e.P) * e.E * e.C * e.S * e.S * b;
     double adj = 0.0;
     for (const DevChild &ch : e.children) {
-      adj += ch.child >= 0 ? row + 4.0 * sites : tip_b * sites;  // own CLV
+      adj += ch.child >= 0 ? row + scl : tip_b;  // own CLV
       for (int u = ch.use_begin; u < ch.use_end; ++u) {
         const DevUse &us = e.uses[u];
-        adj += row + 4.0 * sites;  // parent adjoint + scaler
-        if (us.sib != NO_CHILD) adj += us.sib >= 0 ? row + 4.0 * sites : tip_b * sites;
+        adj += row + scl;  // parent adjoint + parent scaler
+        if (us.sib != NO_CHILD) adj += us.sib >= 0 ? row + scl : tip_b;
       }
       if (ch.child >= 0) adj += row;  // adjoint written
     }
-    out->fwd_bytes = fwd;
-    out->grad_bytes = fwd + adj;
+    adj += 2.0 * pm;  // P and P'
+    out->fwd_op_bytes = fwd_ops;
+    out->root_bytes = root;
+    out->adj_bytes = adj;
+    out->fwd_bytes = pm + fwd_ops + root;
+    out->grad_bytes = out->fwd_bytes + adj;
     out->clv_bytes = static_cast<double>(e.d_clv.bytes + e.d_scl.bytes + e.d_adj.bytes);
     return NRX_OK;
   });
 }
 
-int nrx_last_timing(nrx_engine *eng, double *total_ms, double *dominant_ms,
-                    int32_t *dominant_launches) {
+int nrx_last_timing(nrx_engine *eng, nrx_timing *out) {
   return guarded([&]() -> int {
-    if (!eng) throw std::invalid_argument("null argument");
-    Engine &e = *static_cast<Engine *>(eng);
-    if (total_ms) *total_ms = e.last_total_ms;
-    if (dominant_ms) *dominant_ms = e.last_dom_ms;
-    if (dominant_launches) *dominant_launches = e.last_dom_launches;
+    if (!eng || !out) throw std::invalid_argument("null argument");
+    *out = static_cast<Engine *>(eng)->last_timing;
     return NRX_OK;
   });
 }

@@ -106,10 +106,10 @@ struct Engine : nrx_engine {
 
   // timing
   hipEvent_t ev_begin = nullptr, ev_end = nullptr;
-  std::vector<hipEvent_t> ev_dom;  // pairs around dominant-kernel launches
-  int n_dom = 0;
-  double last_total_ms = 0.0, last_dom_ms = 0.0;
-  int last_dom_launches = 0;
+  // event pairs around pruning (kind 0) and adjoint (kind 1) launches
+  std::vector<hipEvent_t> ev_k[2];
+  int n_k[2] = {0, 0};
+  nrx_timing last_timing{};
 
   ~Engine();
 };

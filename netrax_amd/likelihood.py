@@ -210,11 +210,10 @@ class AnnotatedNetwork:
         _lib.check(self.lib.nrx_get_stats(self.handle, C.byref(s)))
         return s
 
-    def last_timing(self):
-        a, b = C.c_double(), C.c_double()
-        n = C.c_int32()
-        _lib.check(self.lib.nrx_last_timing(self.handle, C.byref(a), C.byref(b), C.byref(n)))
-        return a.value, b.value, n.value
+    def last_timing(self) -> _lib.Timing:
+        t = _lib.Timing()
+        _lib.check(self.lib.nrx_last_timing(self.handle, C.byref(t)))
+        return t
 
     def close(self) -> None:
         if getattr(self, "handle", None):
