@@ -152,8 +152,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--form", choices=["plain"], default="plain",
-                    help="plain: FP64 CLVs, one launch per pruning level")
+    ap.add_argument("--form", choices=["batched", "plain"], default="batched",
+                    help="headline form: batched = FP32 CLVs, persistent site-tile kernels; "
+                         "plain = FP64 CLVs, one launch per pruning level")
+    ap.add_argument("--only", choices=["batched", "plain"], default=None,
+                    help="time only this form (profiling runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-oracle-check", action="store_true")
     ap.add_argument("--cpu-sample-patterns", type=int, default=4000)
@@ -178,10 +181,21 @@ def main():
     n_pat = sum(p.n_sites for p in w.partitions)
     log(f"workload ready in {time.perf_counter() - t0:.1f}s: {n_pat} patterns")
 
-    plain = run_form(w, 0, args.steps, args.warmup)
-    log(f"plain FP64: {plain['evals_per_s']:.1f} evals/s, device {plain['device_ms_per_eval']:.3f} ms/eval")
-    head = plain
+    forms = {}
+    for name, prec in (("plain", 0), ("batched", 1)):
+        if args.only and args.only != name:
+            continue
+        forms[name] = run_form(w, prec, args.steps, args.warmup)
+        r = forms[name]
+        log(f"{name}: {r['evals_per_s']:.1f} evals/s, device {r['device_ms_per_eval']:.3f} ms/eval "
+            f"(fwd {r['fwd_ms_per_eval']:.3f} ms, adj {r['adj_ms_per_eval']:.3f} ms)")
+    head_name = args.only or args.form
+    head = forms[head_name]
     st = head["stats"]
+    desc = {"plain": "plain FP64 (FP64 CLVs, one launch per pruning level, FP64 adjoint)",
+            "batched": "batched FP32 (FP32 CLVs + per-site power-of-two exponents, one persistent "
+                       "site-tile launch for all ops and trees, FP32 adjoint with per-site "
+                       "exponents, FP64 site sums / mixture / gradient reduction)"}
     out = {
         "metric": METRIC,
         "value": round(head["evals_per_s"], 3),
@@ -193,24 +207,35 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32" if head_name == "batched" else "f64",
         "data": "synthetic: seeded Yule network + arc insertions, GTR+G4 alignment simulated on displayed trees",
         "config": {
             "workload": "50 taxa / 4 reticulations (16 displayed trees) / 50,000 DNA sites in 20k+30k "
                         "partitions, GTR+G4, lnL + full gradient (all branch lengths + reticulation probs)",
-            "form": "plain FP64 (one launch per pruning level, FP64 adjoint)",
+            "form": desc[head_name],
             "patterns": n_pat, "sites": 50000, "edges": int(w.network.n_edges),
             "clv_ops": st.n_ops_needed, "levels": st.n_levels, "displayed_trees": st.n_classes,
             "adjoint_uses": st.n_uses, "parallelism": "single GPU",
         },
         "roofline": roofline(head),
         "device_ms_per_eval": round(head["device_ms_per_eval"], 4),
-        "plain_fp64": {"evals_per_s": round(plain["evals_per_s"], 3),
-                       "device_ms_per_eval": round(plain["device_ms_per_eval"], 4),
-                       "fwd_ms": round(plain["fwd_ms_per_eval"], 4),
-                       "adj_ms": round(plain["adj_ms_per_eval"], 4),
-                       "grad_bytes_per_eval": st.grad_bytes},
     }
+    for name, r in forms.items():
+        out[name] = {"evals_per_s": round(r["evals_per_s"], 3),
+                     "ms_per_eval": round(r["ms_per_eval"], 4),
+                     "device_ms_per_eval": round(r["device_ms_per_eval"], 4),
+                     "fwd_ms": round(r["fwd_ms_per_eval"], 4),
+                     "adj_ms": round(r["adj_ms_per_eval"], 4),
+                     "grad_bytes_per_eval": r["stats"].grad_bytes,
+                     "roofline": roofline(r)}
+    if "plain" in forms and "batched" in forms:
+        p64, b32 = forms["plain"], forms["batched"]
+        out["batched_vs_plain"] = {
+            "speedup": round(b32["evals_per_s"] / p64["evals_per_s"], 3),
+            "rel_lnl_diff": abs(b32["lnl"] - p64["lnl"]) / abs(p64["lnl"]),
+            "max_grad_diff_rel_to_norm": float(np.abs(b32["grad"] - p64["grad"]).max()
+                                               / max(1.0, np.abs(p64["grad"]).max())),
+        }
     if not args.no_oracle_check:
         out["accuracy"] = oracle_check(w, head["lnl"], head["grad"], head["dprob"], threads)
     if not args.no_cpu_baseline:

@@ -135,7 +135,7 @@ static void build_schedule(Engine &e) {
   }
   e.level_begin[max_level + 1] = static_cast<int>(e.ops_all.size());
 
-  // root CLVs
+  // root CLVs (computed below, before the DFS order that refers to them)
   e.root_slots.clear();
   e.class_root_index.assign(nc, -1);
   for (int k = 0; k < nc; ++k) {
@@ -147,6 +147,49 @@ static void build_schedule(Engine &e) {
       it = e.root_slots.end() - 1;
     }
     e.class_root_index[k] = static_cast<int>(it - e.root_slots.begin());
+  }
+
+  std::vector<int> root_of_slot(nv, -1);
+  for (size_t r = 0; r < e.root_slots.size(); ++r) root_of_slot[e.root_slots[r]] = static_cast<int>(r);
+  for (DevOp &op : e.ops_all) op.root = root_of_slot[op.out];
+  // depth-first post-order from the root CLVs: a child is consumed soon
+  // after it is produced
+  std::vector<int> post;
+  {
+    std::vector<char> seen(nv, 0);
+    std::vector<std::pair<int, int>> stack;
+    for (int rs : e.root_slots) {
+      if (seen[rs]) continue;
+      stack.push_back({rs, 0});
+      seen[rs] = 1;
+      while (!stack.empty()) {
+        auto &top = stack.back();
+        const Variant &v = e.dag.var[top.first];
+        if (top.second < v.nkids) {
+          const int32_t kid = v.kid[top.second++];
+          if (kid >= 0 && !seen[kid]) {
+            seen[kid] = 1;
+            stack.push_back({kid, 0});
+          }
+        } else {
+          post.push_back(top.first);
+          stack.pop_back();
+        }
+      }
+    }
+  }
+  e.ops_dfs.clear();
+  for (int s : post) {
+    const Variant &v = e.dag.var[s];
+    DevOp op{};
+    op.out = s;
+    op.nk = v.nkids;
+    for (int k = 0; k < 2; ++k) {
+      op.kid[k] = v.kid[k];
+      op.edge[k] = v.edge[k] < 0 ? 0 : v.edge[k];
+    }
+    op.root = root_of_slot[s];
+    e.ops_dfs.push_back(op);
   }
 
   // adjoint: one work item per needed CLV and per tip, consumers in order
@@ -197,6 +240,16 @@ static void build_schedule(Engine &e) {
     for (int c : child_by_level[l]) e.child_lists.push_back(c);
   }
   e.child_level_begin[max_level + 1] = static_cast<int>(e.child_lists.size());
+  // FP32 form: items in reverse post-order (every consumer first), tips last
+  {
+    std::vector<int> item_of_slot(nv, -1);
+    for (size_t i = 0; i < e.children.size(); ++i)
+      if (e.children[i].child >= 0) item_of_slot[e.children[i].child] = static_cast<int>(i);
+    e.child_order_dfs.clear();
+    for (auto it = post.rbegin(); it != post.rend(); ++it) e.child_order_dfs.push_back(item_of_slot[*it]);
+    for (size_t i = 0; i < e.children.size(); ++i)
+      if (e.children[i].child < 0) e.child_order_dfs.push_back(static_cast<int>(i));
+  }
   // uses grouped by edge
   std::vector<std::vector<int>> by_edge(e.E);
   for (int u = 0; u < e.n_uses; ++u) by_edge[e.uses[u].edge].push_back(u);
@@ -215,6 +268,8 @@ static void build_schedule(Engine &e) {
     dc[k].use = e.class_use[k];
   }
   upload(e.d_ops, e.ops_all, e.stream);
+  upload(e.d_ops_dfs, e.ops_dfs, e.stream);
+  upload(e.d_child_order_dfs, e.child_order_dfs, e.stream);
   upload(e.d_children, e.children, e.stream);
   upload(e.d_child_lists, e.child_lists, e.stream);
   upload(e.d_uses, e.uses, e.stream);
@@ -224,9 +279,10 @@ static void build_schedule(Engine &e) {
   upload(e.d_class_dlogprob, e.class_dlogprob, e.stream);
   std::vector<int32_t> rs(e.root_slots.begin(), e.root_slots.end());
   upload(e.d_root_slots, rs, e.stream);
-  e.d_root_partials.alloc(sizeof(double) * std::max<size_t>(1, rs.size()) * e.L.n_tiles);
+  const size_t n_wt = static_cast<size_t>(e.L.n_tiles) * e.pw;
+  e.d_root_partials.alloc(sizeof(double) * std::max<size_t>(1, rs.size()) * n_wt);
   e.d_beta.alloc(sizeof(double) * std::max<size_t>(1, rs.size()) * e.P);
-  e.d_grad_partials.alloc(sizeof(double) * std::max(1, e.n_uses) * static_cast<size_t>(e.L.n_tiles));
+  e.d_grad_partials.alloc(sizeof(double) * std::max(1, e.n_uses) * n_wt);
   NRX_HIP(hipStreamSynchronize(e.stream));
 }
 
@@ -288,6 +344,22 @@ static void upload_models(Engine &e) {
     std::copy(pi.model.eval.begin(), pi.model.eval.end(), ev.begin() + p * S);
     std::copy(pi.rates.begin(), pi.rates.end(), rates.begin() + p * C);
     std::copy(pi.model.freqs.begin(), pi.model.freqs.end(), fr.begin() + p * S);
+  }
+  if (e.opt.precision == NRX_FP32) {
+    // r_c Q per partition and category (dP/dt = Q P), FP32
+    std::vector<float> Qf(static_cast<size_t>(P) * C * S * S);
+    for (int p = 0; p < P; ++p) {
+      const EigenModel &m = e.parts[p].model;
+      for (int c = 0; c < C; ++c)
+        for (int i = 0; i < S; ++i)
+          for (int j = 0; j < S; ++j) {
+            double q = 0.0;
+            for (int k = 0; k < S; ++k) q += m.V[i * S + k] * m.eval[k] * m.Vinv[k * S + j];
+            Qf[((static_cast<size_t>(p) * C + c) * S + i) * S + j] =
+                static_cast<float>(q * e.parts[p].rates[c]);
+          }
+    }
+    upload(e.d_Qf, Qf, e.stream);
   }
   upload(e.d_V, V, e.stream);
   upload(e.d_Vinv, Vi, e.stream);
@@ -412,20 +484,36 @@ static int evaluate(Engine &e, int variant, bool incremental, bool update_pm, bo
     end_timing(e);
     return NRX_OK;  // nothing changed since the last evaluation
   }
-  forward(e, incremental);
-  launch_root_partials(e);
-  launch_mixture(e, variant);
-  if (gradient) {
-    e.d_adj.alloc(sizeof(double) * e.clv_elems_per_slot * e.dag.var.size());
-    const int nl = static_cast<int>(e.child_level_begin.size()) - 1;
-    for (int i = 0; i < nl; ++i) {
-      const int b = e.child_level_begin[i], n = e.child_level_begin[i + 1] - b;
-      if (n == 0) continue;
+  if (e.opt.precision == NRX_FP32) {
+    // batched form: the whole op list in one persistent launch, root site
+    // sums fused into it; every CLV is recomputed
+    record_kernel_begin(e, 0);
+    launch_forward_persistent(e);
+    record_kernel_end(e, 0);
+    std::fill(e.clv_dirty.begin(), e.clv_dirty.end(), 0);
+    launch_mixture(e, variant, e.pw);
+    if (gradient) {
       record_kernel_begin(e, 1);
-      launch_adjoint_level(e, e.d_child_lists.as<int32_t>() + b, n);
+      launch_adjoint_persistent(e);
       record_kernel_end(e, 1);
+      launch_grad_reduce(e, e.pw);
     }
-    launch_grad_reduce(e);
+  } else {
+    forward(e, incremental);
+    launch_root_partials(e);
+    launch_mixture(e, variant, 1);
+    if (gradient) {
+      e.d_adj.alloc(sizeof(double) * e.clv_elems_per_slot * e.dag.var.size());
+      const int nl = static_cast<int>(e.child_level_begin.size()) - 1;
+      for (int i = 0; i < nl; ++i) {
+        const int b = e.child_level_begin[i], n = e.child_level_begin[i + 1] - b;
+        if (n == 0) continue;
+        record_kernel_begin(e, 1);
+        launch_adjoint_level(e, e.d_child_lists.as<int32_t>() + b, n);
+        record_kernel_end(e, 1);
+      }
+      launch_grad_reduce(e, 1);
+    }
   }
   end_timing(e);
   fetch_results(e);
@@ -473,8 +561,6 @@ int nrx_create(const nrx_network_desc *net, const nrx_partition_desc *parts, int
       nrx_default_options(&e->opt);
     if (e->opt.precision != NRX_FP64 && e->opt.precision != NRX_FP32)
       throw std::invalid_argument("unknown precision");
-    if (e->opt.precision == NRX_FP32)
-      return fail(NRX_ERR_UNSUPPORTED, "FP32 form not available in this build");
     e->N = net->n_nodes;
     e->E = net->n_edges;
     e->R = net->n_retic;
@@ -499,6 +585,14 @@ int nrx_create(const nrx_network_desc *net, const nrx_partition_desc *parts, int
     if (e->S != 4 && e->S != 20) throw std::invalid_argument("states must be 4 or 20");
     if (e->C != 1 && e->C != 2 && e->C != 4 && e->C != 8)
       throw std::invalid_argument("rate categories must be 1, 2, 4 or 8");
+    if (e->opt.precision == NRX_FP32) {
+      if (e->S != 4)
+        return fail(NRX_ERR_UNSUPPORTED, "the FP32 form is built for 4-state data");
+      // P (and Q) of one partition are staged in LDS (64 KiB per workgroup)
+      if (sizeof(float) * (static_cast<size_t>(e->E) + 1) * e->C * 16 > 65536)
+        return fail(NRX_ERR_UNSUPPORTED, "network too large for the FP32 form's LDS tables");
+      e->pw = TILE * e->C / 64;
+    }
     e->parts.resize(n_parts);
     int64_t off = 0;
     int tile = 0;
@@ -577,8 +671,14 @@ int nrx_create(const nrx_network_desc *net, const nrx_partition_desc *parts, int
     e->d_d2P.alloc(sizeof(double) * pm);
     e->clv_elems_per_slot = static_cast<size_t>(n) * e->C * e->S;
     const size_t nv = e->dag.var.size();
-    e->d_clv.alloc(sizeof(double) * e->clv_elems_per_slot * std::max<size_t>(nv, 1));
+    const size_t real_b = e->opt.precision == NRX_FP32 ? sizeof(float) : sizeof(double);
+    e->d_clv.alloc(real_b * e->clv_elems_per_slot * std::max<size_t>(nv, 1));
     e->d_scl.alloc(sizeof(int32_t) * static_cast<size_t>(n) * std::max<size_t>(nv, 1));
+    if (e->opt.precision == NRX_FP32) {
+      e->d_Pf.alloc(sizeof(float) * pm);
+      e->d_adj.alloc(sizeof(float) * e->clv_elems_per_slot * std::max<size_t>(nv, 1));
+      e->d_adj_exp.alloc(sizeof(int32_t) * static_cast<size_t>(n) * std::max<size_t>(nv, 1));
+    }
 
     const int nc = static_cast<int>(e->dag.classes.size());
     ResultLayout &rl = e->rl;
@@ -772,7 +872,14 @@ int nrx_get_stats(nrx_engine *eng, nrx_stats *out) {
       }
       if (ch.child >= 0) adj += row;  // adjoint written
     }
-    adj += 2.0 * pm;  // P and P'
+    if (e.opt.precision == NRX_FP32) {
+      // per-site adjoint exponents: read per use, written per CLV
+      for (const DevChild &ch : e.children)
+        adj += (ch.use_end - ch.use_begin) * scl + (ch.child >= 0 ? scl : 0.0);
+      adj += pm;  // P (Q is per partition)
+    } else {
+      adj += 2.0 * pm;  // P and P'
+    }
     out->fwd_op_bytes = fwd_ops;
     out->root_bytes = root;
     out->adj_bytes = adj;

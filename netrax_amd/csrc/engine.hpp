@@ -65,6 +65,8 @@ struct DevOp {  // forward CLV op
   int32_t nk;
   int32_t kid[2];
   int32_t edge[2];
+  int32_t root;  // index of the root-CLV site sum this op feeds, or -1
+  int32_t pad;
 };
 
 struct DevChild {  // adjoint work item: one CLV (or tip) and its consumers
@@ -106,13 +108,11 @@ struct Engine;
 void launch_pmatrices(Engine &e);
 void launch_forward_level(Engine &e, const DevOp *ops, int n_ops);
 void launch_root_partials(Engine &e);
-void launch_mixture(Engine &e, int variant);
+void launch_mixture(Engine &e, int variant, int partials_per_tile);
 void launch_adjoint_level(Engine &e, const int32_t *children, int n_children);
-void launch_grad_reduce(Engine &e);
-void launch_forward_persistent(Engine &e, int n_ops_total);
+void launch_grad_reduce(Engine &e, int partials_per_tile);
+void launch_forward_persistent(Engine &e);
 void launch_adjoint_persistent(Engine &e);
-void launch_sumtable(Engine &e, int edge);
-void launch_deriv_eval(Engine &e, double brlen);
 
 struct HipCheck {
   static void check(hipError_t err, const char *what);

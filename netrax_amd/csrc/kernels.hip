@@ -261,18 +261,19 @@ __global__ void __launch_bounds__(TILE *C)
 // a max-shifted FP64 log-sum-exp stands in for the reference's MPFR sum.
 // Also produces the adjoint seeds beta[p][root CLV] = sum of the class
 // weights d lnL_p / d lnL_{p,t}, and d lnL / d p_r.
-__global__ void k_mixture(DevLayout L, int n_root, int nc, int R, int variant,
+__global__ void k_mixture(DevLayout L, int pw, int n_root, int nc, int R, int variant,
                           const DevClass *__restrict__ classes,
                           const double *__restrict__ class_dlogprob,
                           const int32_t *__restrict__ part_tile_begin,
                           const double *__restrict__ partials, double *__restrict__ beta,
                           double *__restrict__ res, ResultLayout rl) {
   const int P = L.n_parts;
+  const size_t n_wt = static_cast<size_t>(L.n_tiles) * pw;
   for (int idx = threadIdx.x; idx < n_root * P; idx += blockDim.x) {
     const int r = idx / P, p = idx % P;
     double s = 0.0;
-    for (int t = part_tile_begin[p]; t < part_tile_begin[p + 1]; ++t)
-      s += partials[static_cast<size_t>(r) * L.n_tiles + t];
+    for (int t = part_tile_begin[p] * pw; t < part_tile_begin[p + 1] * pw; ++t)
+      s += partials[static_cast<size_t>(r) * n_wt + t];
     res[rl.root_logl + p * n_root + r] = s;
   }
   __syncthreads();
@@ -424,20 +425,21 @@ __global__ void __launch_bounds__(TILE *C)
 // part_dbrlen[p][e] = sum over the uses of edge e and the tiles of partition p
 // (fixed assignment of (use, tile) pairs to threads + tree reduction).
 __global__ void __launch_bounds__(256)
-    k_grad_reduce(DevLayout L, const int32_t *__restrict__ edge_use_begin,
+    k_grad_reduce(DevLayout L, int pw, const int32_t *__restrict__ edge_use_begin,
                   const int32_t *__restrict__ edge_use_list,
                   const int32_t *__restrict__ part_tile_begin,
                   const double *__restrict__ grad_partials, double *__restrict__ res,
                   ResultLayout rl) {
   __shared__ double sh[4];
   const int P = L.n_parts;
+  const size_t n_wt = static_cast<size_t>(L.n_tiles) * pw;
   const int e = blockIdx.x / P, p = blockIdx.x % P;
   const int u0 = edge_use_begin[e], nu = edge_use_begin[e + 1] - u0;
-  const int t0 = part_tile_begin[p], nt = part_tile_begin[p + 1] - t0;
+  const int t0 = part_tile_begin[p] * pw, nt = (part_tile_begin[p + 1] - part_tile_begin[p]) * pw;
   double s = 0.0;
   for (int i = threadIdx.x; i < nu * nt; i += 256) {
     const int u = edge_use_list[u0 + i / nt];
-    s += grad_partials[static_cast<size_t>(u) * L.n_tiles + t0 + i % nt];
+    s += grad_partials[static_cast<size_t>(u) * n_wt + t0 + i % nt];
   }
   s = block_sum<256>(s, sh);
   if (threadIdx.x == 0) res[rl.part_dbrlen + p * L.n_edges + e] = s;
@@ -449,6 +451,236 @@ __global__ void k_grad_total(DevLayout L, double *__restrict__ res, ResultLayout
   double s = 0.0;
   for (int p = 0; p < L.n_parts; ++p) s += res[rl.part_dbrlen + p * L.n_edges + e];
   res[rl.dbrlen + e] = s;
+}
+
+
+// ==================================================== FP32 batched form
+// One workgroup per 64-site tile walks the whole op list (children before
+// parents, depth-first post-order so a child is consumed soon after it is
+// written).  Thread (site, category) owns its CLV row in every op, so every
+// value a thread reads was written by the same thread: no barrier is needed
+// after the P tables are staged in LDS.  CLVs are FP32, renormalised at
+// every op by an exact power of two so the site maximum lies in [0.5, 1);
+// the per-site exponent k (true = stored * 2^-k) is carried as an int32.
+// Site sums, the mixture and the gradient reduction are FP64.
+constexpr double LN2 = 0.69314718055994530941723212145818;
+constexpr int EXP_EMPTY = -(1 << 28);  // exponent of an all-zero adjoint
+
+template <int C>
+__device__ __forceinline__ float cat_maxf(float m) {
+#pragma unroll
+  for (int off = 1; off < C; off <<= 1) m = fmaxf(m, __shfl_xor(m, off));
+  return m;
+}
+
+__device__ __forceinline__ void tip_vec4(uint32_t mask, float (&x)[4]) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i) x[i] = ((mask >> i) & 1u) ? 1.0f : 0.0f;
+}
+
+__device__ __forceinline__ void load4(const float *__restrict__ p, float (&x)[4]) {
+  const float4 v = *reinterpret_cast<const float4 *>(p);
+  x[0] = v.x;
+  x[1] = v.y;
+  x[2] = v.z;
+  x[3] = v.w;
+}
+
+__device__ __forceinline__ void store4(float *__restrict__ p, const float (&x)[4]) {
+  *reinterpret_cast<float4 *>(p) = make_float4(x[0], x[1], x[2], x[3]);
+}
+
+__device__ __forceinline__ void mv4(const float *__restrict__ M, const float (&x)[4], float (&y)[4]) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    y[i] = fmaf(M[4 * i + 3], x[3], fmaf(M[4 * i + 2], x[2], fmaf(M[4 * i + 1], x[1], M[4 * i] * x[0])));
+}
+
+__device__ __forceinline__ void mv4t(const float *__restrict__ M, const float (&x)[4], float (&y)[4]) {
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+    y[j] = fmaf(M[12 + j], x[3], fmaf(M[8 + j], x[2], fmaf(M[4 + j], x[1], M[j] * x[0])));
+}
+
+template <int C>
+__global__ void __launch_bounds__(TILE *C)
+    k_forward_persistent(const DevOp *__restrict__ ops, int n_ops, DevLayout L,
+                         float *__restrict__ clv, int32_t *__restrict__ scl,
+                         const uint8_t *__restrict__ tips, const float *__restrict__ Pf,
+                         const int32_t *__restrict__ tile_part, const double *__restrict__ freqs,
+                         const double *__restrict__ weights, double *__restrict__ root_partials) {
+  extern __shared__ float sP[];  // P of this tile's partition: [edge][cat][4][4]
+  constexpr int PW = TILE * C / 64;
+  const int tile = blockIdx.x;
+  const int part = tile_part[tile];
+  const int np = L.n_edges * C * 16;
+  const float *gP = Pf + static_cast<size_t>(part) * np;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) sP[i] = gP[i];
+  __syncthreads();
+  const int c = threadIdx.x % C;
+  const int64_t site = static_cast<int64_t>(tile) * TILE + threadIdx.x / C;
+  const int64_t n = L.n_sites_pad;
+  const size_t n_wt = static_cast<size_t>(L.n_tiles) * PW;
+  const int wt = tile * PW + threadIdx.x / 64;
+  const double w = weights[site];
+  double pi[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) pi[i] = freqs[part * 4 + i];
+  for (int it = 0; it < n_ops; ++it) {
+    const DevOp op = ops[it];
+    float v[4] = {1.f, 1.f, 1.f, 1.f};
+    int k = 0;
+    for (int kk = 0; kk < op.nk; ++kk) {
+      float x[4], y[4];
+      const int32_t kid = op.kid[kk];
+      if (kid >= 0) {
+        load4(clv + (static_cast<size_t>(kid) * n + site) * (C * 4) + c * 4, x);
+        k += scl[static_cast<size_t>(kid) * n + site];
+      } else {
+        tip_vec4(tips[static_cast<size_t>(-kid - 1) * n + site], x);
+      }
+      mv4(sP + (op.edge[kk] * C + c) * 16, x, y);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) v[i] *= y[i];
+    }
+    const float m = cat_maxf<C>(fmaxf(fmaxf(v[0], v[1]), fmaxf(v[2], v[3])));
+    int e = 0;
+    if (m > 0.f) (void)frexpf(m, &e);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) v[i] = ldexpf(v[i], -e);
+    k -= e;
+    store4(clv + (static_cast<size_t>(op.out) * n + site) * (C * 4) + c * 4, v);
+    if (c == 0) scl[static_cast<size_t>(op.out) * n + site] = k;
+    if (op.root >= 0) {
+      double t = 0.0;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) t = fma(pi[i], static_cast<double>(v[i]), t);
+      t = cat_sum<C>(t) * (1.0 / C);
+      double val = (c == 0 && w != 0.0) ? w * (log(t) - k * LN2) : 0.0;
+      val = wave_sum(val);
+      if ((threadIdx.x & 63) == 0) root_partials[static_cast<size_t>(op.root) * n_wt + wt] = val;
+    }
+  }
+}
+
+// Reverse sweep (pull form, as k_adjoint_level) with the adjoint of every
+// CLV stored as FP32 mantissas plus a per-site exponent g:
+// dJ/d(stored CLV) = stored adjoint * 2^g.  Contributions of the consumers
+// are aligned to the largest exponent before they are added, so no
+// intermediate can overflow.  dP/dt = Q P supplies P' x = r_c Q (P x).
+template <int C>
+__global__ void __launch_bounds__(TILE *C)
+    k_adjoint_persistent(const int32_t *__restrict__ order, int n_items,
+                         const DevChild *__restrict__ children, const DevUse *__restrict__ uses,
+                         DevLayout L, const float *__restrict__ clv, const int32_t *__restrict__ scl,
+                         float *__restrict__ adj, int32_t *__restrict__ adj_exp,
+                         const uint8_t *__restrict__ tips, const float *__restrict__ Pf,
+                         const float *__restrict__ Qf, const int32_t *__restrict__ tile_part,
+                         const double *__restrict__ freqs, const double *__restrict__ weights,
+                         const double *__restrict__ beta, int n_root,
+                         double *__restrict__ grad_partials) {
+  extern __shared__ float sP[];  // [edge][cat][16] then Q: [cat][16]
+  constexpr int PW = TILE * C / 64;
+  const int tile = blockIdx.x;
+  const int part = tile_part[tile];
+  const int np = L.n_edges * C * 16;
+  const float *gP = Pf + static_cast<size_t>(part) * np;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) sP[i] = gP[i];
+  for (int i = threadIdx.x; i < C * 16; i += blockDim.x) sP[np + i] = Qf[part * C * 16 + i];
+  __syncthreads();
+  const float *sQ = sP + np;
+  const int c = threadIdx.x % C;
+  const int64_t site = static_cast<int64_t>(tile) * TILE + threadIdx.x / C;
+  const int64_t n = L.n_sites_pad;
+  const size_t n_wt = static_cast<size_t>(L.n_tiles) * PW;
+  const int wt = tile * PW + threadIdx.x / 64;
+  const double w = weights[site];
+  double pi[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) pi[i] = freqs[part * 4 + i];
+  const size_t row = C * 4;
+  for (int it = 0; it < n_items; ++it) {
+    const DevChild ch = children[order[it]];
+    float xo[4];
+    int ko = 0;
+    if (ch.child >= 0) {
+      load4(clv + (static_cast<size_t>(ch.child) * n + site) * row + c * 4, xo);
+      ko = scl[static_cast<size_t>(ch.child) * n + site];
+    } else {
+      tip_vec4(tips[static_cast<size_t>(-ch.child - 1) * n + site], xo);
+    }
+    float a[4] = {0.f, 0.f, 0.f, 0.f};
+    int ea = EXP_EMPTY;
+    if (ch.root_index >= 0) {
+      double t = 0.0;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) t = fma(pi[i], static_cast<double>(xo[i]), t);
+      t = cat_sum<C>(t) * (1.0 / C);
+      const double b = beta[part * n_root + ch.root_index] * w;
+      if (b != 0.0) {
+        int ef;
+        const double mf = frexp(b / (C * t), &ef);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a[i] = static_cast<float>(mf * pi[i]);
+        ea = ef;
+      }
+    }
+    for (int u = ch.use_begin; u < ch.use_end; ++u) {
+      const DevUse us = uses[u];
+      float aq[4], y[4], z[4];
+      load4(adj + (static_cast<size_t>(us.parent) * n + site) * row + c * 4, aq);
+      const int gq = adj_exp[static_cast<size_t>(us.parent) * n + site];
+      const int kq = scl[static_cast<size_t>(us.parent) * n + site];
+      int ks = 0;
+      if (us.sib == NO_CHILD) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) y[i] = 1.f;
+      } else {
+        float xs[4];
+        if (us.sib >= 0) {
+          load4(clv + (static_cast<size_t>(us.sib) * n + site) * row + c * 4, xs);
+          ks = scl[static_cast<size_t>(us.sib) * n + site];
+        } else {
+          tip_vec4(tips[static_cast<size_t>(-us.sib - 1) * n + site], xs);
+        }
+        mv4(sP + (us.sib_edge * C + c) * 16, xs, y);
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) z[i] = aq[i] * y[i];
+      const int ez = gq + kq - ko - ks;
+      const float *Pe = sP + (us.edge * C + c) * 16;
+      float back[4], px[4], qx[4];
+      mv4t(Pe, z, back);
+      if (ez > ea) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a[i] = ldexpf(a[i], ea - ez) + back[i];
+        ea = ez;
+      } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a[i] += ldexpf(back[i], ez - ea);
+      }
+      mv4(Pe, xo, px);
+      mv4(sQ + c * 16, px, qx);
+      const float gd = fmaf(z[3], qx[3], fmaf(z[2], qx[2], fmaf(z[1], qx[1], z[0] * qx[0])));
+      double gv = ldexp(static_cast<double>(gd), ez);
+      gv = wave_sum(gv);
+      if ((threadIdx.x & 63) == 0) grad_partials[static_cast<size_t>(us.grad_slot) * n_wt + wt] = gv;
+    }
+    if (ch.child >= 0) {
+      const float m = cat_maxf<C>(fmaxf(fmaxf(a[0], a[1]), fmaxf(a[2], a[3])));
+      int e = 0;
+      if (m > 0.f) {
+        (void)frexpf(m, &e);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a[i] = ldexpf(a[i], -e);
+        ea += e;
+      } else {
+        ea = EXP_EMPTY;
+      }
+      store4(adj + (static_cast<size_t>(ch.child) * n + site) * row + c * 4, a);
+      if (c == 0) adj_exp[static_cast<size_t>(ch.child) * n + site] = ea;
+    }
+  }
 }
 
 template <typename F>
@@ -509,8 +741,8 @@ void launch_root_partials(Engine &e) {
   NRX_HIP(hipGetLastError());
 }
 
-void launch_mixture(Engine &e, int variant) {
-  hipLaunchKernelGGL(k_mixture, dim3(1), dim3(256), 0, e.stream, e.L,
+void launch_mixture(Engine &e, int variant, int pw) {
+  hipLaunchKernelGGL(k_mixture, dim3(1), dim3(256), 0, e.stream, e.L, pw,
                      static_cast<int>(e.root_slots.size()), static_cast<int>(e.dag.classes.size()),
                      e.R, variant, e.d_classes.as<DevClass>(), e.d_class_dlogprob.as<double>(),
                      e.d_part_tile_begin.as<int32_t>(), e.d_root_partials.as<double>(),
@@ -534,14 +766,55 @@ void launch_adjoint_level(Engine &e, const int32_t *children, int n_children) {
   NRX_HIP(hipGetLastError());
 }
 
-void launch_grad_reduce(Engine &e) {
-  hipLaunchKernelGGL(k_grad_reduce, dim3(e.E * e.P), dim3(256), 0, e.stream, e.L,
+void launch_grad_reduce(Engine &e, int pw) {
+  hipLaunchKernelGGL(k_grad_reduce, dim3(e.E * e.P), dim3(256), 0, e.stream, e.L, pw,
                      e.d_edge_use_begin.as<int32_t>(), e.d_edge_use_list.as<int32_t>(),
                      e.d_part_tile_begin.as<int32_t>(), e.d_grad_partials.as<double>(),
                      e.d_result.as<double>(), e.rl);
   NRX_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_grad_total, dim3((e.E + 255) / 256), dim3(256), 0, e.stream, e.L,
                      e.d_result.as<double>(), e.rl);
+  NRX_HIP(hipGetLastError());
+}
+
+
+template <typename F>
+void dispatch_c(int C, F &&f) {
+  switch (C) {
+    case 1: f(std::integral_constant<int, 1>{}); break;
+    case 2: f(std::integral_constant<int, 2>{}); break;
+    case 4: f(std::integral_constant<int, 4>{}); break;
+    case 8: f(std::integral_constant<int, 8>{}); break;
+    default: throw std::invalid_argument("unsupported number of rate categories");
+  }
+}
+
+void launch_forward_persistent(Engine &e) {
+  const size_t lds = sizeof(float) * e.E * e.C * 16;
+  dispatch_c(e.C, [&](auto c) {
+    constexpr int C = decltype(c)::value;
+    hipLaunchKernelGGL((k_forward_persistent<C>), dim3(e.L.n_tiles), dim3(TILE * C), lds,
+                       e.stream, e.d_ops_dfs.as<DevOp>(), static_cast<int>(e.ops_dfs.size()), e.L,
+                       e.d_clv.as<float>(), e.d_scl.as<int32_t>(), e.d_tips.as<uint8_t>(),
+                       e.d_Pf.as<float>(), e.d_tile_part.as<int32_t>(), e.d_freqs.as<double>(),
+                       e.d_weights.as<double>(), e.d_root_partials.as<double>());
+  });
+  NRX_HIP(hipGetLastError());
+}
+
+void launch_adjoint_persistent(Engine &e) {
+  const size_t lds = sizeof(float) * (e.E * e.C * 16 + e.C * 16);
+  dispatch_c(e.C, [&](auto c) {
+    constexpr int C = decltype(c)::value;
+    hipLaunchKernelGGL((k_adjoint_persistent<C>), dim3(e.L.n_tiles), dim3(TILE * C), lds,
+                       e.stream, e.d_child_order_dfs.as<int32_t>(),
+                       static_cast<int>(e.child_order_dfs.size()), e.d_children.as<DevChild>(),
+                       e.d_uses.as<DevUse>(), e.L, e.d_clv.as<float>(), e.d_scl.as<int32_t>(),
+                       e.d_adj.as<float>(), e.d_adj_exp.as<int32_t>(), e.d_tips.as<uint8_t>(),
+                       e.d_Pf.as<float>(), e.d_Qf.as<float>(), e.d_tile_part.as<int32_t>(),
+                       e.d_freqs.as<double>(), e.d_weights.as<double>(), e.d_beta.as<double>(),
+                       static_cast<int>(e.root_slots.size()), e.d_grad_partials.as<double>());
+  });
   NRX_HIP(hipGetLastError());
 }
 

@@ -206,3 +206,66 @@ def test_average_bounds_best():
     a = computeLoglikelihood(engine(w.network, w.partitions, 0))
     b = computeLoglikelihood(engine(w.network, w.partitions, 1))
     assert a >= b
+
+
+# ---------------------------------------------------------------- FP32 form
+# Batched form: FP32 CLVs with power-of-two per-site exponents, FP64 sums.
+FP32_RTOL = 1e-6  # |dlnL| <= 1e-6 relative to the FP64 result
+
+
+def engine32(net, parts, variant=0):
+    from netrax_amd import Precision
+    return engine(net, parts, variant, precision=Precision.FP32)
+
+
+@pytest.mark.parametrize("name", sorted(SAMPLE_PAIRS))
+def test_fp32_samples_match_oracle(name):
+    say("fp32 sample", name)
+    net, parts = load_sample(name, fixed_model())
+    for var in (0, 1):
+        ann = engine32(net, parts, var)
+        got = computeLoglikelihood(ann)
+        want = O.loglikelihood(net, parts, var)[0]
+        assert abs(got - want) <= FP32_RTOL * abs(want), (name, var, got, want)
+        lnl, g, gp = ann.gradient()
+        _, og, ogp = O.gradient(net, parts, var)
+        scale = max(1.0, np.abs(og).max())
+        assert np.allclose(g, og, rtol=1e-4, atol=1e-4 * scale), (name, var, np.abs(g - og).max())
+        assert np.allclose(gp, ogp, rtol=1e-4, atol=1e-4 * max(1.0, np.abs(ogp).max()))
+        ann.close()
+
+
+@pytest.mark.parametrize("wl", ["8taxa", "30taxa", "50taxa"])
+def test_fp32_configs_match_fp64(wl):
+    say("fp32 vs fp64", wl)
+    from netrax_amd.simulate import workload_50taxa_50k
+    w = {"8taxa": workload_small_1k, "30taxa": workload_30taxa_10k,
+         "50taxa": workload_50taxa_50k}[wl]()
+    for var in (0, 1):
+        a64 = engine(w.network, w.partitions, var)
+        a32 = engine32(w.network, w.partitions, var)
+        l64, g64, p64 = a64.gradient()
+        l32, g32, p32 = a32.gradient()
+        assert abs(l32 - l64) <= FP32_RTOL * abs(l64), (var, l32, l64)
+        scale = max(1.0, np.abs(g64).max())
+        assert np.abs(g32 - g64).max() <= 1e-4 * scale, (var, np.abs(g32 - g64).max(), scale)
+        assert np.abs(p32 - p64).max() <= 1e-4 * max(1.0, np.abs(p64).max())
+        a64.close()
+        a32.close()
+
+
+def test_fp32_deep_underflow_matches_closed_form():
+    """Saturated branches on 200 tips: site likelihoods ~1e-120 need the
+    per-site exponents; lnL = sum of log stationary frequencies."""
+    from netrax_amd.network import random_network
+    from netrax_amd.simulate import Model, Partition
+    rng = np.random.default_rng(11)
+    net = random_network(200, 0, rng)
+    net.brlen[:] = 100.0
+    m = Model(4, np.ones(6), np.array([0.1, 0.2, 0.3, 0.4]), 1.0, 1)
+    states = rng.integers(4, size=(200, 100))
+    part = Partition(m, (1 << states).astype(np.uint32), np.ones(100, np.uint32))
+    want = float(np.log(m.freqs)[states].sum())
+    for eng in (engine, engine32):
+        got = computeLoglikelihood(eng(net, [part]))
+        assert abs(got - want) <= 1e-6 * abs(want), (got, want)
