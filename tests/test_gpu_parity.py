@@ -231,7 +231,7 @@ def test_fp32_samples_match_oracle(name):
         _, og, ogp = O.gradient(net, parts, var)
         scale = max(1.0, np.abs(og).max())
         assert np.allclose(g, og, rtol=1e-4, atol=1e-4 * scale), (name, var, np.abs(g - og).max())
-        assert np.allclose(gp, ogp, rtol=1e-4, atol=1e-4 * max(1.0, np.abs(ogp).max()))
+        assert np.allclose(gp, ogp, rtol=1e-4, atol=1e-4 * max([1.0, *np.abs(ogp)]))
         ann.close()
 
 
@@ -249,7 +249,7 @@ def test_fp32_configs_match_fp64(wl):
         assert abs(l32 - l64) <= FP32_RTOL * abs(l64), (var, l32, l64)
         scale = max(1.0, np.abs(g64).max())
         assert np.abs(g32 - g64).max() <= 1e-4 * scale, (var, np.abs(g32 - g64).max(), scale)
-        assert np.abs(p32 - p64).max() <= 1e-4 * max(1.0, np.abs(p64).max())
+        assert np.abs(p32 - p64).max(initial=0.0) <= 1e-4 * max([1.0, *np.abs(p64)])
         a64.close()
         a32.close()
 
