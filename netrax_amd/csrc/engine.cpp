@@ -178,18 +178,52 @@ static void build_schedule(Engine &e) {
       }
     }
   }
-  e.ops_dfs.clear();
-  for (int s : post) {
-    const Variant &v = e.dag.var[s];
-    DevOp op{};
-    op.out = s;
-    op.nk = v.nkids;
-    for (int k = 0; k < 2; ++k) {
-      op.kid[k] = v.kid[k];
-      op.edge[k] = v.edge[k] < 0 ? 0 : v.edge[k];
+  // Groups of up to GROUP mutually independent ops for the persistent
+  // kernel: greedy list schedule over ready ops, lowest post-order index
+  // first, so locality of the depth-first order is kept while every group
+  // exposes several CLV loads at once.
+  constexpr int GROUP = 4;
+  {
+    std::vector<int> pos(nv, -1), deps(nv, 0);
+    std::vector<std::vector<int>> parents(nv);
+    for (size_t i = 0; i < post.size(); ++i) pos[post[i]] = static_cast<int>(i);
+    for (int s : post)
+      for (int k = 0; k < e.dag.var[s].nkids; ++k)
+        if (e.dag.var[s].kid[k] >= 0) {
+          deps[s]++;
+          parents[e.dag.var[s].kid[k]].push_back(s);
+        }
+    std::vector<int> ready;  // kept sorted by post-order index
+    for (int s : post)
+      if (deps[s] == 0) ready.push_back(s);
+    auto by_pos = [&](int a, int b) { return pos[a] < pos[b]; };
+    std::sort(ready.begin(), ready.end(), by_pos);
+    e.ops_dfs.clear();
+    e.group_begin.assign(1, 0);
+    while (!ready.empty()) {
+      const int take = std::min<int>(GROUP, static_cast<int>(ready.size()));
+      std::vector<int> grp(ready.begin(), ready.begin() + take);
+      ready.erase(ready.begin(), ready.begin() + take);
+      for (int s : grp) {
+        const Variant &v = e.dag.var[s];
+        DevOp op{};
+        op.out = s;
+        op.nk = v.nkids;
+        for (int k = 0; k < 2; ++k) {
+          op.kid[k] = v.kid[k];
+          op.edge[k] = v.edge[k] < 0 ? 0 : v.edge[k];
+        }
+        op.root = root_of_slot[s];
+        e.ops_dfs.push_back(op);
+        for (int par : parents[s])
+          if (--deps[par] == 0) ready.insert(std::upper_bound(ready.begin(), ready.end(), par, by_pos), par);
+      }
+      e.group_begin.push_back(static_cast<int32_t>(e.ops_dfs.size()));
     }
-    op.root = root_of_slot[s];
-    e.ops_dfs.push_back(op);
+    if (e.ops_dfs.size() != post.size()) throw std::logic_error("op grouping lost ops");
+    // the adjoint walks the grouped order backwards
+    post.clear();
+    for (const DevOp &op : e.ops_dfs) post.push_back(op.out);
   }
 
   // adjoint: one work item per needed CLV and per tip, consumers in order
@@ -269,6 +303,7 @@ static void build_schedule(Engine &e) {
   }
   upload(e.d_ops, e.ops_all, e.stream);
   upload(e.d_ops_dfs, e.ops_dfs, e.stream);
+  upload(e.d_group_begin, e.group_begin, e.stream);
   upload(e.d_child_order_dfs, e.child_order_dfs, e.stream);
   upload(e.d_children, e.children, e.stream);
   upload(e.d_child_lists, e.child_lists, e.stream);
@@ -283,6 +318,8 @@ static void build_schedule(Engine &e) {
   e.d_root_partials.alloc(sizeof(double) * std::max<size_t>(1, rs.size()) * n_wt);
   e.d_beta.alloc(sizeof(double) * std::max<size_t>(1, rs.size()) * e.P);
   e.d_grad_partials.alloc(sizeof(double) * std::max(1, e.n_uses) * n_wt);
+  e.d_root_sums.alloc(sizeof(double) * std::max<size_t>(1, rs.size()) * e.P);
+  e.d_use_sums.alloc(sizeof(double) * std::max(1, e.n_uses) * static_cast<size_t>(e.P));
   NRX_HIP(hipStreamSynchronize(e.stream));
 }
 
@@ -501,7 +538,7 @@ static int evaluate(Engine &e, int variant, bool incremental, bool update_pm, bo
   } else {
     forward(e, incremental);
     launch_root_partials(e);
-    launch_mixture(e, variant, 1);
+    launch_mixture(e, variant, e.pw);
     if (gradient) {
       e.d_adj.alloc(sizeof(double) * e.clv_elems_per_slot * e.dag.var.size());
       const int nl = static_cast<int>(e.child_level_begin.size()) - 1;
@@ -512,7 +549,7 @@ static int evaluate(Engine &e, int variant, bool incremental, bool update_pm, bo
         launch_adjoint_level(e, e.d_child_lists.as<int32_t>() + b, n);
         record_kernel_end(e, 1);
       }
-      launch_grad_reduce(e, 1);
+      launch_grad_reduce(e, e.pw);
     }
   }
   end_timing(e);
@@ -591,8 +628,8 @@ int nrx_create(const nrx_network_desc *net, const nrx_partition_desc *parts, int
       // P (and Q) of one partition are staged in LDS (64 KiB per workgroup)
       if (sizeof(float) * (static_cast<size_t>(e->E) + 1) * e->C * 16 > 65536)
         return fail(NRX_ERR_UNSUPPORTED, "network too large for the FP32 form's LDS tables");
-      e->pw = TILE * e->C / 64;
     }
+    e->pw = TILE * e->C / 64;  // site-sum partials per tile: one per wave
     e->parts.resize(n_parts);
     int64_t off = 0;
     int tile = 0;

@@ -79,7 +79,8 @@ struct Engine : nrx_engine {
   std::vector<int32_t> edge_use_begin, edge_use_list;
   int n_uses = 0;
   // FP32 form: depth-first post-order of the ops, reverse order of adjoint items
-  std::vector<DevOp> ops_dfs;
+  std::vector<DevOp> ops_dfs;             // grouped order (see group_begin)
+  std::vector<int32_t> group_begin;       // [n_groups + 1] into ops_dfs
   std::vector<int32_t> child_order_dfs;
   int pw = 1;  // site-sum partials per tile (1 per workgroup, or 1 per wave)
 
@@ -99,11 +100,11 @@ struct Engine : nrx_engine {
   DevBuf d_V, d_Vinv, d_eval, d_rates, d_freqs, d_brlen;
   DevBuf d_P, d_dP, d_d2P, d_Pf, d_dPf;
   DevBuf d_clv, d_scl, d_adj, d_adj_exp, d_Qf;
-  DevBuf d_ops_dfs, d_child_order_dfs;
+  DevBuf d_ops_dfs, d_group_begin, d_child_order_dfs;
   DevBuf d_ops, d_ops_dirty;
   DevBuf d_children, d_child_lists, d_uses, d_edge_use_begin, d_edge_use_list;
   DevBuf d_classes, d_class_dlogprob, d_root_slots;
-  DevBuf d_root_partials, d_beta, d_grad_partials;
+  DevBuf d_root_partials, d_beta, d_grad_partials, d_root_sums, d_use_sums;
   DevBuf d_result;
   double *h_result = nullptr;  // pinned
   ResultLayout rl{};

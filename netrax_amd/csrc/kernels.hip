@@ -176,82 +176,113 @@ __global__ void k_pmatrices(DevLayout L, const double *__restrict__ V,
 }
 
 // ------------------------------------------------------- forward (FP64)
-template <int S, int C>
+// Plain form: one launch per pruning level.  Workgroup b handles tile b and
+// walks every op of the level for that tile, G mutually independent ops at
+// a time: all children of the G ops are loaded before any is combined, so
+// each wave keeps 2G CLV rows in flight.
+template <int S, int C, int G>
 __global__ void __launch_bounds__(TILE *C)
-    k_forward_level(const DevOp *__restrict__ ops, DevLayout L, double *__restrict__ clv,
-                    int32_t *__restrict__ scl,
+    k_forward_level(const DevOp *__restrict__ ops, int n_ops, DevLayout L,
+                    double *__restrict__ clv, int32_t *__restrict__ scl,
                     const typename TipType<S>::type *__restrict__ tips,
                     const double *__restrict__ P, const int32_t *__restrict__ tile_part) {
-  const int tile = blockIdx.x % L.n_tiles_grid;
+  const int tile = blockIdx.x;
   if (tile >= L.n_tiles) return;
-  const DevOp op = ops[blockIdx.x / L.n_tiles_grid];
   const int c = threadIdx.x % C;
   const int64_t site = static_cast<int64_t>(tile) * TILE + threadIdx.x / C;
   const int part = tile_part[tile];
   const int64_t n = L.n_sites_pad;
-  double v[S];
+  for (int g0 = 0; g0 < n_ops; g0 += G) {
+    const int nb = min(G, n_ops - g0);
+    double x[G][2][S];
+    int sc[G];
 #pragma unroll
-  for (int i = 0; i < S; ++i) v[i] = 1.0;
-  int sc = 0;
-  for (int k = 0; k < op.nk; ++k) {
-    double x[S], y[S];
-    const int32_t kid = op.kid[k];
-    if (kid >= 0) {
-      load_clv<S>(clv + (static_cast<size_t>(kid) * n + site) * (C * S) + c * S, x);
-      sc += scl[static_cast<size_t>(kid) * n + site];
-    } else {
-      tip_vec<S>(tips[static_cast<size_t>(-kid - 1) * n + site], x);
+    for (int j = 0; j < G; ++j) {
+      sc[j] = 0;
+      if (j < nb) {
+        const DevOp op = ops[g0 + j];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+          if (k < op.nk) {
+            const int32_t kid = op.kid[k];
+            if (kid >= 0) {
+              load_clv<S>(clv + (static_cast<size_t>(kid) * n + site) * (C * S) + c * S, x[j][k]);
+              sc[j] += scl[static_cast<size_t>(kid) * n + site];
+            } else {
+              tip_vec<S>(tips[static_cast<size_t>(-kid - 1) * n + site], x[j][k]);
+            }
+          }
+        }
+      }
     }
-    matvec<S>(P + ((static_cast<size_t>(part) * L.n_edges + op.edge[k]) * C + c) * (S * S), x, y);
 #pragma unroll
-    for (int i = 0; i < S; ++i) v[i] *= y[i];
+    for (int j = 0; j < G; ++j) {
+      if (j < nb) {
+        const DevOp op = ops[g0 + j];
+        double v[S];
+#pragma unroll
+        for (int i = 0; i < S; ++i) v[i] = 1.0;
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+          if (k < op.nk) {
+            double y[S];
+            matvec<S>(P + ((static_cast<size_t>(part) * L.n_edges + op.edge[k]) * C + c) * (S * S),
+                      x[j][k], y);
+#pragma unroll
+            for (int i = 0; i < S; ++i) v[i] *= y[i];
+          }
+        }
+        double m = v[0];
+#pragma unroll
+        for (int i = 1; i < S; ++i) m = fmax(m, v[i]);
+        m = cat_max<C>(m);
+        int scj = sc[j];
+        if (m < SCALE_THRESHOLD) {
+#pragma unroll
+          for (int i = 0; i < S; ++i) v[i] *= SCALE_FACTOR;
+          scj += 1;
+        }
+        store_clv<S>(clv + (static_cast<size_t>(op.out) * n + site) * (C * S) + c * S, v);
+        if (c == 0) scl[static_cast<size_t>(op.out) * n + site] = scj;
+      }
+    }
   }
-  double m = v[0];
-#pragma unroll
-  for (int i = 1; i < S; ++i) m = fmax(m, v[i]);
-  m = cat_max<C>(m);
-  if (m < SCALE_THRESHOLD) {
-#pragma unroll
-    for (int i = 0; i < S; ++i) v[i] *= SCALE_FACTOR;
-    sc += 1;
-  }
-  store_clv<S>(clv + (static_cast<size_t>(op.out) * n + site) * (C * S) + c * S, v);
-  if (c == 0) scl[static_cast<size_t>(op.out) * n + site] = sc;
 }
 
 // ---------------------------------------------------- root site sums (FP64)
-// partial[r][tile] = sum over the tile's sites of
+// partial[r][wave tile] = sum over the wave's sites of
 //   w_s * (log(sum_c (1/C) sum_i pi_i CLV[s,c,i]) + scaler_s * log(2^-256))
 // (pll_compute_root_loglikelihood semantics, ImprovedLoglikelihood.cpp:450).
 template <int S, int C>
 __global__ void __launch_bounds__(TILE *C)
-    k_root_partials(const int32_t *__restrict__ root_slots, DevLayout L,
+    k_root_partials(const int32_t *__restrict__ root_slots, int n_root, DevLayout L,
                     const double *__restrict__ clv, const int32_t *__restrict__ scl,
                     const double *__restrict__ freqs, const double *__restrict__ weights,
                     const int32_t *__restrict__ tile_part, double *__restrict__ partials) {
-  __shared__ double sh[TILE * C / 64];
-  const int tile = blockIdx.x % L.n_tiles_grid;
+  constexpr int PW = TILE * C / 64;
+  const int tile = blockIdx.x;
   if (tile >= L.n_tiles) return;
-  const int r = blockIdx.x / L.n_tiles_grid;
-  const int slot = root_slots[r];
   const int c = threadIdx.x % C;
   const int64_t site = static_cast<int64_t>(tile) * TILE + threadIdx.x / C;
   const int part = tile_part[tile];
   const int64_t n = L.n_sites_pad;
-  double x[S];
-  load_clv<S>(clv + (static_cast<size_t>(slot) * n + site) * (C * S) + c * S, x);
-  double t = 0.0;
+  const size_t n_wt = static_cast<size_t>(L.n_tiles) * PW;
+  const int wt = tile * PW + threadIdx.x / 64;
+  const double w = weights[site];
+  for (int r = 0; r < n_root; ++r) {
+    const int slot = root_slots[r];
+    double x[S];
+    load_clv<S>(clv + (static_cast<size_t>(slot) * n + site) * (C * S) + c * S, x);
+    double t = 0.0;
 #pragma unroll
-  for (int i = 0; i < S; ++i) t = fma(freqs[part * S + i], x[i], t);
-  t = cat_sum<C>(t) * (1.0 / C);
-  double val = 0.0;
-  if (c == 0) {
-    const double w = weights[site];
-    if (w != 0.0)
+    for (int i = 0; i < S; ++i) t = fma(freqs[part * S + i], x[i], t);
+    t = cat_sum<C>(t) * (1.0 / C);
+    double val = 0.0;
+    if (c == 0 && w != 0.0)
       val = w * (log(t) + scl[static_cast<size_t>(slot) * n + site] * LOG_SCALE_THRESHOLD);
+    val = wave_sum(val);
+    if ((threadIdx.x & 63) == 0) partials[static_cast<size_t>(r) * n_wt + wt] = val;
   }
-  val = block_sum<TILE * C>(val, sh);
-  if (threadIdx.x == 0) partials[static_cast<size_t>(r) * L.n_tiles + tile] = val;
 }
 
 // ----------------------------------------------------------------- mixture
@@ -261,20 +292,15 @@ __global__ void __launch_bounds__(TILE *C)
 // a max-shifted FP64 log-sum-exp stands in for the reference's MPFR sum.
 // Also produces the adjoint seeds beta[p][root CLV] = sum of the class
 // weights d lnL_p / d lnL_{p,t}, and d lnL / d p_r.
-__global__ void k_mixture(DevLayout L, int pw, int n_root, int nc, int R, int variant,
+__global__ void k_mixture(DevLayout L, int n_root, int nc, int R, int variant,
                           const DevClass *__restrict__ classes,
                           const double *__restrict__ class_dlogprob,
-                          const int32_t *__restrict__ part_tile_begin,
-                          const double *__restrict__ partials, double *__restrict__ beta,
+                          const double *__restrict__ root_sums, double *__restrict__ beta,
                           double *__restrict__ res, ResultLayout rl) {
   const int P = L.n_parts;
-  const size_t n_wt = static_cast<size_t>(L.n_tiles) * pw;
   for (int idx = threadIdx.x; idx < n_root * P; idx += blockDim.x) {
     const int r = idx / P, p = idx % P;
-    double s = 0.0;
-    for (int t = part_tile_begin[p] * pw; t < part_tile_begin[p + 1] * pw; ++t)
-      s += partials[static_cast<size_t>(r) * n_wt + t];
-    res[rl.root_logl + p * n_root + r] = s;
+    res[rl.root_logl + p * n_root + r] = root_sums[idx];
   }
   __syncthreads();
   for (int p = threadIdx.x; p < P; p += blockDim.x) {
@@ -341,118 +367,129 @@ __global__ void k_mixture(DevLayout L, int pw, int n_root, int nc, int R, int va
 // result is deterministic without atomics.
 template <int S, int C>
 __global__ void __launch_bounds__(TILE *C)
-    k_adjoint_level(const int32_t *__restrict__ child_list, const DevChild *__restrict__ children,
-                    const DevUse *__restrict__ uses, DevLayout L, const double *__restrict__ clv,
-                    const int32_t *__restrict__ scl, double *__restrict__ adj,
-                    const typename TipType<S>::type *__restrict__ tips,
+    k_adjoint_level(const int32_t *__restrict__ child_list, int n_items,
+                    const DevChild *__restrict__ children, const DevUse *__restrict__ uses,
+                    DevLayout L, const double *__restrict__ clv, const int32_t *__restrict__ scl,
+                    double *__restrict__ adj, const typename TipType<S>::type *__restrict__ tips,
                     const double *__restrict__ P, const double *__restrict__ dP,
                     const double *__restrict__ freqs, const double *__restrict__ weights,
                     const int32_t *__restrict__ tile_part, const double *__restrict__ beta,
                     int n_root, double *__restrict__ grad_partials) {
-  __shared__ double sh[TILE * C / 64];
-  const int tile = blockIdx.x % L.n_tiles_grid;
+  constexpr int PW = TILE * C / 64;
+  const int tile = blockIdx.x;
   if (tile >= L.n_tiles) return;
-  const DevChild ch = children[child_list[blockIdx.x / L.n_tiles_grid]];
   const int c = threadIdx.x % C;
   const int64_t site = static_cast<int64_t>(tile) * TILE + threadIdx.x / C;
   const int part = tile_part[tile];
   const int64_t n = L.n_sites_pad;
   const size_t row = static_cast<size_t>(C * S);
-
-  double xo[S];
-  int ko = 0;
-  if (ch.child >= 0) {
-    load_clv<S>(clv + (static_cast<size_t>(ch.child) * n + site) * row + c * S, xo);
-    ko = scl[static_cast<size_t>(ch.child) * n + site];
-  } else {
-    tip_vec<S>(tips[static_cast<size_t>(-ch.child - 1) * n + site], xo);
-  }
-  double a[S];
-#pragma unroll
-  for (int i = 0; i < S; ++i) a[i] = 0.0;
-  if (ch.root_index >= 0) {
-    double t = 0.0;
-#pragma unroll
-    for (int i = 0; i < S; ++i) t = fma(freqs[part * S + i], xo[i], t);
-    t = cat_sum<C>(t) * (1.0 / C);
-    const double b = beta[part * n_root + ch.root_index] * weights[site];
-    if (b != 0.0) {
-      const double f = b * (1.0 / C) / t;
-#pragma unroll
-      for (int i = 0; i < S; ++i) a[i] = f * freqs[part * S + i];
-    }
-  }
-  for (int u = ch.use_begin; u < ch.use_end; ++u) {
-    const DevUse us = uses[u];
-    double aq[S], y[S], z[S];
-    load_clv<S>(adj + (static_cast<size_t>(us.parent) * n + site) * row + c * S, aq);
-    const int kq = scl[static_cast<size_t>(us.parent) * n + site];
-    int ks = 0;
-    if (us.sib == NO_CHILD) {
-#pragma unroll
-      for (int i = 0; i < S; ++i) y[i] = 1.0;
+  const size_t n_wt = static_cast<size_t>(L.n_tiles) * PW;
+  const int wt = tile * PW + threadIdx.x / 64;
+  const double w = weights[site];
+  for (int it = 0; it < n_items; ++it) {
+    const DevChild ch = children[child_list[it]];
+    double xo[S];
+    int ko = 0;
+    if (ch.child >= 0) {
+      load_clv<S>(clv + (static_cast<size_t>(ch.child) * n + site) * row + c * S, xo);
+      ko = scl[static_cast<size_t>(ch.child) * n + site];
     } else {
-      double xs[S];
-      if (us.sib >= 0) {
-        load_clv<S>(clv + (static_cast<size_t>(us.sib) * n + site) * row + c * S, xs);
-        ks = scl[static_cast<size_t>(us.sib) * n + site];
-      } else {
-        tip_vec<S>(tips[static_cast<size_t>(-us.sib - 1) * n + site], xs);
-      }
-      matvec<S>(P + ((static_cast<size_t>(part) * L.n_edges + us.sib_edge) * C + c) * (S * S), xs,
-                y);
+      tip_vec<S>(tips[static_cast<size_t>(-ch.child - 1) * n + site], xo);
     }
-    const double F = (kq - ko - ks) > 0 ? SCALE_FACTOR : 1.0;
+    double a[S];
 #pragma unroll
-    for (int i = 0; i < S; ++i) z[i] = aq[i] * y[i] * F;
-    const size_t poff = ((static_cast<size_t>(part) * L.n_edges + us.edge) * C + c) * (S * S);
-    double back[S], dx[S];
-    matvec_t<S>(P + poff, z, back);
+    for (int i = 0; i < S; ++i) a[i] = 0.0;
+    if (ch.root_index >= 0) {
+      double t = 0.0;
 #pragma unroll
-    for (int i = 0; i < S; ++i) a[i] += back[i];
-    matvec<S>(dP + poff, xo, dx);
-    double g = 0.0;
+      for (int i = 0; i < S; ++i) t = fma(freqs[part * S + i], xo[i], t);
+      t = cat_sum<C>(t) * (1.0 / C);
+      const double b = beta[part * n_root + ch.root_index] * w;
+      if (b != 0.0) {
+        const double f = b * (1.0 / C) / t;
 #pragma unroll
-    for (int i = 0; i < S; ++i) g = fma(z[i], dx[i], g);
-    g = block_sum<TILE * C>(g, sh);
-    if (threadIdx.x == 0) grad_partials[static_cast<size_t>(us.grad_slot) * L.n_tiles + tile] = g;
+        for (int i = 0; i < S; ++i) a[i] = f * freqs[part * S + i];
+      }
+    }
+    for (int u = ch.use_begin; u < ch.use_end; ++u) {
+      const DevUse us = uses[u];
+      double aq[S], y[S], z[S];
+      load_clv<S>(adj + (static_cast<size_t>(us.parent) * n + site) * row + c * S, aq);
+      const int kq = scl[static_cast<size_t>(us.parent) * n + site];
+      int ks = 0;
+      if (us.sib == NO_CHILD) {
+#pragma unroll
+        for (int i = 0; i < S; ++i) y[i] = 1.0;
+      } else {
+        double xs[S];
+        if (us.sib >= 0) {
+          load_clv<S>(clv + (static_cast<size_t>(us.sib) * n + site) * row + c * S, xs);
+          ks = scl[static_cast<size_t>(us.sib) * n + site];
+        } else {
+          tip_vec<S>(tips[static_cast<size_t>(-us.sib - 1) * n + site], xs);
+        }
+        matvec<S>(P + ((static_cast<size_t>(part) * L.n_edges + us.sib_edge) * C + c) * (S * S),
+                  xs, y);
+      }
+      const double F = (kq - ko - ks) > 0 ? SCALE_FACTOR : 1.0;
+#pragma unroll
+      for (int i = 0; i < S; ++i) z[i] = aq[i] * y[i] * F;
+      const size_t poff = ((static_cast<size_t>(part) * L.n_edges + us.edge) * C + c) * (S * S);
+      double back[S], dx[S];
+      matvec_t<S>(P + poff, z, back);
+#pragma unroll
+      for (int i = 0; i < S; ++i) a[i] += back[i];
+      matvec<S>(dP + poff, xo, dx);
+      double g = 0.0;
+#pragma unroll
+      for (int i = 0; i < S; ++i) g = fma(z[i], dx[i], g);
+      g = wave_sum(g);
+      if ((threadIdx.x & 63) == 0)
+        grad_partials[static_cast<size_t>(us.grad_slot) * n_wt + wt] = g;
+    }
+    if (ch.child >= 0)
+      store_clv<S>(adj + (static_cast<size_t>(ch.child) * n + site) * row + c * S, a);
   }
-  if (ch.child >= 0)
-    store_clv<S>(adj + (static_cast<size_t>(ch.child) * n + site) * row + c * S, a);
 }
 
-// ------------------------------------------------------- gradient reduce
-// part_dbrlen[p][e] = sum over the uses of edge e and the tiles of partition p
-// (fixed assignment of (use, tile) pairs to threads + tree reduction).
+// ------------------------------------------------------- reductions
+// out[row][p] = sum of in[row][t] over the partials t of partition p
+// (partials_per_tile per tile, tiles of a partition are contiguous).  One
+// workgroup per (row, partition): coalesced strided loads, fixed tree
+// order, so the sums are deterministic.
 __global__ void __launch_bounds__(256)
-    k_grad_reduce(DevLayout L, int pw, const int32_t *__restrict__ edge_use_begin,
-                  const int32_t *__restrict__ edge_use_list,
-                  const int32_t *__restrict__ part_tile_begin,
-                  const double *__restrict__ grad_partials, double *__restrict__ res,
-                  ResultLayout rl) {
+    k_reduce_segments(const double *__restrict__ in, size_t row_stride, int pw,
+                      const int32_t *__restrict__ part_tile_begin, int P,
+                      double *__restrict__ out) {
   __shared__ double sh[4];
-  const int P = L.n_parts;
-  const size_t n_wt = static_cast<size_t>(L.n_tiles) * pw;
-  const int e = blockIdx.x / P, p = blockIdx.x % P;
-  const int u0 = edge_use_begin[e], nu = edge_use_begin[e + 1] - u0;
-  const int t0 = part_tile_begin[p] * pw, nt = (part_tile_begin[p + 1] - part_tile_begin[p]) * pw;
+  const int row = blockIdx.x / P, p = blockIdx.x % P;
+  const double *r = in + static_cast<size_t>(row) * row_stride;
+  const int t0 = part_tile_begin[p] * pw, t1 = part_tile_begin[p + 1] * pw;
   double s = 0.0;
-  for (int i = threadIdx.x; i < nu * nt; i += 256) {
-    const int u = edge_use_list[u0 + i / nt];
-    s += grad_partials[static_cast<size_t>(u) * n_wt + t0 + i % nt];
-  }
+  for (int t = t0 + threadIdx.x; t < t1; t += 256) s += r[t];
   s = block_sum<256>(s, sh);
-  if (threadIdx.x == 0) res[rl.part_dbrlen + p * L.n_edges + e] = s;
+  if (threadIdx.x == 0) out[static_cast<size_t>(row) * P + p] = s;
 }
 
-__global__ void k_grad_total(DevLayout L, double *__restrict__ res, ResultLayout rl) {
+// part_dbrlen[p][e] = sum over the uses of edge e of use_sums[u][p];
+// dbrlen[e] = sum over partitions.
+__global__ void k_grad_total(DevLayout L, const int32_t *__restrict__ edge_use_begin,
+                             const int32_t *__restrict__ edge_use_list,
+                             const double *__restrict__ use_sums, double *__restrict__ res,
+                             ResultLayout rl) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= L.n_edges) return;
-  double s = 0.0;
-  for (int p = 0; p < L.n_parts; ++p) s += res[rl.part_dbrlen + p * L.n_edges + e];
-  res[rl.dbrlen + e] = s;
+  const int P = L.n_parts;
+  double tot = 0.0;
+  for (int p = 0; p < P; ++p) {
+    double s = 0.0;
+    for (int i = edge_use_begin[e]; i < edge_use_begin[e + 1]; ++i)
+      s += use_sums[static_cast<size_t>(edge_use_list[i]) * P + p];
+    res[rl.part_dbrlen + p * L.n_edges + e] = s;
+    tot += s;
+  }
+  res[rl.dbrlen + e] = tot;
 }
-
 
 // ==================================================== FP32 batched form
 // One workgroup per 64-site tile walks the whole op list (children before
@@ -502,13 +539,14 @@ __device__ __forceinline__ void mv4t(const float *__restrict__ M, const float (&
     y[j] = fmaf(M[12 + j], x[3], fmaf(M[8 + j], x[2], fmaf(M[4 + j], x[1], M[j] * x[0])));
 }
 
-template <int C>
+template <int C, int G>
 __global__ void __launch_bounds__(TILE *C)
-    k_forward_persistent(const DevOp *__restrict__ ops, int n_ops, DevLayout L,
-                         float *__restrict__ clv, int32_t *__restrict__ scl,
-                         const uint8_t *__restrict__ tips, const float *__restrict__ Pf,
-                         const int32_t *__restrict__ tile_part, const double *__restrict__ freqs,
-                         const double *__restrict__ weights, double *__restrict__ root_partials) {
+    k_forward_persistent(const DevOp *__restrict__ ops, const int32_t *__restrict__ group_begin,
+                         int n_groups, DevLayout L, float *__restrict__ clv,
+                         int32_t *__restrict__ scl, const uint8_t *__restrict__ tips,
+                         const float *__restrict__ Pf, const int32_t *__restrict__ tile_part,
+                         const double *__restrict__ freqs, const double *__restrict__ weights,
+                         double *__restrict__ root_partials) {
   extern __shared__ float sP[];  // P of this tile's partition: [edge][cat][4][4]
   constexpr int PW = TILE * C / 64;
   const int tile = blockIdx.x;
@@ -526,39 +564,63 @@ __global__ void __launch_bounds__(TILE *C)
   double pi[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) pi[i] = freqs[part * 4 + i];
-  for (int it = 0; it < n_ops; ++it) {
-    const DevOp op = ops[it];
-    float v[4] = {1.f, 1.f, 1.f, 1.f};
-    int k = 0;
-    for (int kk = 0; kk < op.nk; ++kk) {
-      float x[4], y[4];
-      const int32_t kid = op.kid[kk];
-      if (kid >= 0) {
-        load4(clv + (static_cast<size_t>(kid) * n + site) * (C * 4) + c * 4, x);
-        k += scl[static_cast<size_t>(kid) * n + site];
-      } else {
-        tip_vec4(tips[static_cast<size_t>(-kid - 1) * n + site], x);
+  for (int gi = 0; gi < n_groups; ++gi) {
+    const int g0 = group_begin[gi];
+    const int nb = group_begin[gi + 1] - g0;
+    float x[G][2][4];
+    int ks[G];
+    // load every child of the group first
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+      ks[j] = 0;
+      if (j < nb) {
+        const DevOp op = ops[g0 + j];
+#pragma unroll
+        for (int kk = 0; kk < 2; ++kk) {
+          if (kk < op.nk) {
+            const int32_t kid = op.kid[kk];
+            if (kid >= 0) {
+              load4(clv + (static_cast<size_t>(kid) * n + site) * (C * 4) + c * 4, x[j][kk]);
+              ks[j] += scl[static_cast<size_t>(kid) * n + site];
+            } else {
+              tip_vec4(tips[static_cast<size_t>(-kid - 1) * n + site], x[j][kk]);
+            }
+          }
+        }
       }
-      mv4(sP + (op.edge[kk] * C + c) * 16, x, y);
-#pragma unroll
-      for (int i = 0; i < 4; ++i) v[i] *= y[i];
     }
-    const float m = cat_maxf<C>(fmaxf(fmaxf(v[0], v[1]), fmaxf(v[2], v[3])));
-    int e = 0;
-    if (m > 0.f) (void)frexpf(m, &e);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) v[i] = ldexpf(v[i], -e);
-    k -= e;
-    store4(clv + (static_cast<size_t>(op.out) * n + site) * (C * 4) + c * 4, v);
-    if (c == 0) scl[static_cast<size_t>(op.out) * n + site] = k;
-    if (op.root >= 0) {
-      double t = 0.0;
+    for (int j = 0; j < G; ++j) {
+      if (j < nb) {
+        const DevOp op = ops[g0 + j];
+        float v[4] = {1.f, 1.f, 1.f, 1.f};
 #pragma unroll
-      for (int i = 0; i < 4; ++i) t = fma(pi[i], static_cast<double>(v[i]), t);
-      t = cat_sum<C>(t) * (1.0 / C);
-      double val = (c == 0 && w != 0.0) ? w * (log(t) - k * LN2) : 0.0;
-      val = wave_sum(val);
-      if ((threadIdx.x & 63) == 0) root_partials[static_cast<size_t>(op.root) * n_wt + wt] = val;
+        for (int kk = 0; kk < 2; ++kk) {
+          if (kk < op.nk) {
+            float y[4];
+            mv4(sP + (op.edge[kk] * C + c) * 16, x[j][kk], y);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) v[i] *= y[i];
+          }
+        }
+        const float m = cat_maxf<C>(fmaxf(fmaxf(v[0], v[1]), fmaxf(v[2], v[3])));
+        int e = 0;
+        if (m > 0.f) (void)frexpf(m, &e);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v[i] = ldexpf(v[i], -e);
+        const int k = ks[j] - e;
+        store4(clv + (static_cast<size_t>(op.out) * n + site) * (C * 4) + c * 4, v);
+        if (c == 0) scl[static_cast<size_t>(op.out) * n + site] = k;
+        if (op.root >= 0) {
+          double t = 0.0;
+#pragma unroll
+          for (int i = 0; i < 4; ++i) t = fma(pi[i], static_cast<double>(v[i]), t);
+          t = cat_sum<C>(t) * (1.0 / C);
+          double val = (c == 0 && w != 0.0) ? w * (log(t) - k * LN2) : 0.0;
+          val = wave_sum(val);
+          if ((threadIdx.x & 63) == 0) root_partials[static_cast<size_t>(op.root) * n_wt + wt] = val;
+        }
+      }
     }
   }
 }
@@ -721,9 +783,10 @@ void launch_forward_level(Engine &e, const DevOp *ops, int n_ops) {
   if (n_ops == 0) return;
   dispatch_sc(e.S, e.C, [&](auto s, auto c) {
     constexpr int S = decltype(s)::value, C = decltype(c)::value;
+    constexpr int G = S == 4 ? 4 : 1;
     using TT = typename TipType<S>::type;
-    hipLaunchKernelGGL((k_forward_level<S, C>), dim3(n_ops * e.L.n_tiles_grid), dim3(TILE * C), 0,
-                       e.stream, ops, e.L, e.d_clv.as<double>(), e.d_scl.as<int32_t>(),
+    hipLaunchKernelGGL((k_forward_level<S, C, G>), dim3(e.L.n_tiles), dim3(TILE * C), 0,
+                       e.stream, ops, n_ops, e.L, e.d_clv.as<double>(), e.d_scl.as<int32_t>(),
                        e.d_tips.as<TT>(), e.d_P.as<double>(), e.d_tile_part.as<int32_t>());
   });
   NRX_HIP(hipGetLastError());
@@ -733,8 +796,8 @@ void launch_root_partials(Engine &e) {
   const int nr = static_cast<int>(e.root_slots.size());
   dispatch_sc(e.S, e.C, [&](auto s, auto c) {
     constexpr int S = decltype(s)::value, C = decltype(c)::value;
-    hipLaunchKernelGGL((k_root_partials<S, C>), dim3(nr * e.L.n_tiles_grid), dim3(TILE * C), 0,
-                       e.stream, e.d_root_slots.as<int32_t>(), e.L, e.d_clv.as<double>(),
+    hipLaunchKernelGGL((k_root_partials<S, C>), dim3(e.L.n_tiles), dim3(TILE * C), 0, e.stream,
+                       e.d_root_slots.as<int32_t>(), nr, e.L, e.d_clv.as<double>(),
                        e.d_scl.as<int32_t>(), e.d_freqs.as<double>(), e.d_weights.as<double>(),
                        e.d_tile_part.as<int32_t>(), e.d_root_partials.as<double>());
   });
@@ -742,11 +805,16 @@ void launch_root_partials(Engine &e) {
 }
 
 void launch_mixture(Engine &e, int variant, int pw) {
-  hipLaunchKernelGGL(k_mixture, dim3(1), dim3(256), 0, e.stream, e.L, pw,
-                     static_cast<int>(e.root_slots.size()), static_cast<int>(e.dag.classes.size()),
-                     e.R, variant, e.d_classes.as<DevClass>(), e.d_class_dlogprob.as<double>(),
-                     e.d_part_tile_begin.as<int32_t>(), e.d_root_partials.as<double>(),
-                     e.d_beta.as<double>(), e.d_result.as<double>(), e.rl);
+  const int nr = static_cast<int>(e.root_slots.size());
+  hipLaunchKernelGGL(k_reduce_segments, dim3(nr * e.P), dim3(256), 0, e.stream,
+                     e.d_root_partials.as<double>(), static_cast<size_t>(e.L.n_tiles) * pw, pw,
+                     e.d_part_tile_begin.as<int32_t>(), e.P, e.d_root_sums.as<double>());
+  NRX_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_mixture, dim3(1), dim3(256), 0, e.stream, e.L, nr,
+                     static_cast<int>(e.dag.classes.size()), e.R, variant,
+                     e.d_classes.as<DevClass>(), e.d_class_dlogprob.as<double>(),
+                     e.d_root_sums.as<double>(), e.d_beta.as<double>(), e.d_result.as<double>(),
+                     e.rl);
   NRX_HIP(hipGetLastError());
 }
 
@@ -755,11 +823,11 @@ void launch_adjoint_level(Engine &e, const int32_t *children, int n_children) {
   dispatch_sc(e.S, e.C, [&](auto s, auto c) {
     constexpr int S = decltype(s)::value, C = decltype(c)::value;
     using TT = typename TipType<S>::type;
-    hipLaunchKernelGGL((k_adjoint_level<S, C>), dim3(n_children * e.L.n_tiles_grid),
-                       dim3(TILE * C), 0, e.stream, children, e.d_children.as<DevChild>(),
-                       e.d_uses.as<DevUse>(), e.L, e.d_clv.as<double>(), e.d_scl.as<int32_t>(),
-                       e.d_adj.as<double>(), e.d_tips.as<TT>(), e.d_P.as<double>(),
-                       e.d_dP.as<double>(), e.d_freqs.as<double>(), e.d_weights.as<double>(),
+    hipLaunchKernelGGL((k_adjoint_level<S, C>), dim3(e.L.n_tiles), dim3(TILE * C), 0, e.stream,
+                       children, n_children, e.d_children.as<DevChild>(), e.d_uses.as<DevUse>(),
+                       e.L, e.d_clv.as<double>(), e.d_scl.as<int32_t>(), e.d_adj.as<double>(),
+                       e.d_tips.as<TT>(), e.d_P.as<double>(), e.d_dP.as<double>(),
+                       e.d_freqs.as<double>(), e.d_weights.as<double>(),
                        e.d_tile_part.as<int32_t>(), e.d_beta.as<double>(),
                        static_cast<int>(e.root_slots.size()), e.d_grad_partials.as<double>());
   });
@@ -767,13 +835,13 @@ void launch_adjoint_level(Engine &e, const int32_t *children, int n_children) {
 }
 
 void launch_grad_reduce(Engine &e, int pw) {
-  hipLaunchKernelGGL(k_grad_reduce, dim3(e.E * e.P), dim3(256), 0, e.stream, e.L, pw,
-                     e.d_edge_use_begin.as<int32_t>(), e.d_edge_use_list.as<int32_t>(),
-                     e.d_part_tile_begin.as<int32_t>(), e.d_grad_partials.as<double>(),
-                     e.d_result.as<double>(), e.rl);
+  hipLaunchKernelGGL(k_reduce_segments, dim3(e.n_uses * e.P), dim3(256), 0, e.stream,
+                     e.d_grad_partials.as<double>(), static_cast<size_t>(e.L.n_tiles) * pw, pw,
+                     e.d_part_tile_begin.as<int32_t>(), e.P, e.d_use_sums.as<double>());
   NRX_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_grad_total, dim3((e.E + 255) / 256), dim3(256), 0, e.stream, e.L,
-                     e.d_result.as<double>(), e.rl);
+                     e.d_edge_use_begin.as<int32_t>(), e.d_edge_use_list.as<int32_t>(),
+                     e.d_use_sums.as<double>(), e.d_result.as<double>(), e.rl);
   NRX_HIP(hipGetLastError());
 }
 
@@ -793,8 +861,9 @@ void launch_forward_persistent(Engine &e) {
   const size_t lds = sizeof(float) * e.E * e.C * 16;
   dispatch_c(e.C, [&](auto c) {
     constexpr int C = decltype(c)::value;
-    hipLaunchKernelGGL((k_forward_persistent<C>), dim3(e.L.n_tiles), dim3(TILE * C), lds,
-                       e.stream, e.d_ops_dfs.as<DevOp>(), static_cast<int>(e.ops_dfs.size()), e.L,
+    hipLaunchKernelGGL((k_forward_persistent<C, 4>), dim3(e.L.n_tiles), dim3(TILE * C), lds,
+                       e.stream, e.d_ops_dfs.as<DevOp>(), e.d_group_begin.as<int32_t>(),
+                       static_cast<int>(e.group_begin.size()) - 1, e.L,
                        e.d_clv.as<float>(), e.d_scl.as<int32_t>(), e.d_tips.as<uint8_t>(),
                        e.d_Pf.as<float>(), e.d_tile_part.as<int32_t>(), e.d_freqs.as<double>(),
                        e.d_weights.as<double>(), e.d_root_partials.as<double>());
