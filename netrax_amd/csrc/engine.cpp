@@ -8,6 +8,7 @@
 // changed since the last call (reuseOldDisplayedTreesCheck,
 // src/graph/AnnotatedNetwork.cpp:359-394).
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <exception>
@@ -99,8 +100,28 @@ static void compute_class_probs(Engine &e) {
   }
 }
 
-// Builds the level schedule, root CLV set and adjoint use lists for the
-// classes that pass the log-probability cut, and uploads them.
+// Sets OP_HAS_CONSUMERS / OP_FIRST_KID* for a list of ops walked by the
+// adjoint in the order given by `adj_order` (indices into ops).
+static void set_adjoint_flags(std::vector<DevOp> &ops, const std::vector<int> &adj_order, int nv) {
+  std::vector<char> has_cons(nv, 0), delivered(nv, 0);
+  for (const DevOp &op : ops)
+    for (int k = 0; k < op.nk; ++k)
+      if (op.kid[k] >= 0) has_cons[op.kid[k]] = 1;
+  for (int i : adj_order) {
+    DevOp &op = ops[i];
+    op.flags = has_cons[op.out] ? OP_HAS_CONSUMERS : 0;
+    for (int k = 0; k < op.nk; ++k) {
+      const int32_t kid = op.kid[k];
+      if (kid >= 0 && !delivered[kid]) {
+        op.flags |= k == 0 ? OP_FIRST_KID0 : OP_FIRST_KID1;
+        delivered[kid] = 1;
+      }
+    }
+  }
+}
+
+// Builds the level schedule, the batched order, root CLV set and gradient
+// rows for the classes that pass the log-probability cut, and uploads them.
 static void build_schedule(Engine &e) {
   const int nv = static_cast<int>(e.dag.var.size());
   const int nc = static_cast<int>(e.dag.classes.size());
@@ -117,25 +138,8 @@ static void build_schedule(Engine &e) {
   e.level_ops.assign(max_level + 1, {});
   for (int s = 0; s < nv; ++s)
     if (e.needed[s]) e.level_ops[e.dag.var[s].level].push_back(s);
-  e.ops_all.clear();
-  e.level_begin.assign(max_level + 2, 0);
-  for (int l = 0; l <= max_level; ++l) {
-    e.level_begin[l] = static_cast<int>(e.ops_all.size());
-    for (int s : e.level_ops[l]) {
-      const Variant &v = e.dag.var[s];
-      DevOp op{};
-      op.out = s;
-      op.nk = v.nkids;
-      for (int k = 0; k < 2; ++k) {
-        op.kid[k] = v.kid[k];
-        op.edge[k] = v.edge[k] < 0 ? 0 : v.edge[k];
-      }
-      e.ops_all.push_back(op);
-    }
-  }
-  e.level_begin[max_level + 1] = static_cast<int>(e.ops_all.size());
 
-  // root CLVs (computed below, before the DFS order that refers to them)
+  // root CLVs
   e.root_slots.clear();
   e.class_root_index.assign(nc, -1);
   for (int k = 0; k < nc; ++k) {
@@ -148,12 +152,92 @@ static void build_schedule(Engine &e) {
     }
     e.class_root_index[k] = static_cast<int>(it - e.root_slots.begin());
   }
-
   std::vector<int> root_of_slot(nv, -1);
   for (size_t r = 0; r < e.root_slots.size(); ++r) root_of_slot[e.root_slots[r]] = static_cast<int>(r);
-  for (DevOp &op : e.ops_all) op.root = root_of_slot[op.out];
-  // depth-first post-order from the root CLVs: a child is consumed soon
-  // after it is produced
+
+  // gradient rows: one per (op, child)
+  std::vector<std::array<int, 2>> gslot(nv, {-1, -1});
+  std::vector<std::vector<int>> by_edge(e.E);
+  e.n_uses = 0;
+  for (int s = 0; s < nv; ++s) {
+    if (!e.needed[s]) continue;
+    for (int k = 0; k < e.dag.var[s].nkids; ++k) {
+      gslot[s][k] = e.n_uses;
+      by_edge[e.dag.var[s].edge[k]].push_back(e.n_uses);
+      e.n_uses++;
+    }
+  }
+  e.edge_use_begin.assign(e.E + 1, 0);
+  e.edge_use_list.clear();
+  for (int ed = 0; ed < e.E; ++ed) {
+    e.edge_use_begin[ed] = static_cast<int>(e.edge_use_list.size());
+    for (int u : by_edge[ed]) e.edge_use_list.push_back(u);
+  }
+  e.edge_use_begin[e.E] = static_cast<int>(e.edge_use_list.size());
+
+  auto make_op = [&](int s) {
+    const Variant &v = e.dag.var[s];
+    DevOp op{};
+    op.out = s;
+    op.nk = v.nkids;
+    for (int k = 0; k < 2; ++k) {
+      op.kid[k] = k < v.nkids ? v.kid[k] : NO_CHILD;
+      op.edge[k] = k < v.nkids ? v.edge[k] : 0;
+      op.ledge[k] = 0;
+      op.gslot[k] = k < v.nkids ? gslot[s][k] : 0;
+    }
+    op.root = root_of_slot[s];
+    op.flags = 0;
+    return op;
+  };
+
+  // plain form: level order, each level cut into launches whose distinct
+  // edges fit the LDS P table (4-state data; 20-state reads P from memory)
+  const int max_edges = e.S == 4 ? std::max(8, 49152 / (e.C * 16 * 8)) : 1 << 30;
+  e.ops_all.clear();
+  e.chunks.clear();
+  e.chunk_edges.clear();
+  for (int l = 1; l <= max_level; ++l) {
+    size_t i = 0;
+    const auto &lv = e.level_ops[l];
+    while (i < lv.size()) {
+      LevelChunk ch{};
+      ch.level = l;
+      ch.op_begin = static_cast<int>(e.ops_all.size());
+      ch.edge_begin = static_cast<int>(e.chunk_edges.size());
+      std::vector<int> local(e.E, -1);
+      int n_local = 0;
+      for (; i < lv.size(); ++i) {
+        const Variant &v = e.dag.var[lv[i]];
+        int extra = 0;
+        for (int k = 0; k < v.nkids; ++k) extra += local[v.edge[k]] < 0;
+        if (n_local + extra > max_edges && ch.op_begin != static_cast<int>(e.ops_all.size())) break;
+        DevOp op = make_op(lv[i]);
+        for (int k = 0; k < v.nkids; ++k) {
+          if (local[v.edge[k]] < 0) {
+            local[v.edge[k]] = n_local++;
+            e.chunk_edges.push_back(v.edge[k]);
+          }
+          op.ledge[k] = local[v.edge[k]];
+        }
+        e.ops_all.push_back(op);
+      }
+      ch.op_end = static_cast<int>(e.ops_all.size());
+      ch.edge_end = static_cast<int>(e.chunk_edges.size());
+      e.chunks.push_back(ch);
+    }
+  }
+  {
+    std::vector<int> order;  // adjoint: chunks backwards, ops in index order
+    for (auto it = e.chunks.rbegin(); it != e.chunks.rend(); ++it)
+      for (int i = it->op_begin; i < it->op_end; ++i) order.push_back(i);
+    set_adjoint_flags(e.ops_all, order, nv);
+  }
+
+  // batched form: depth-first post-order from the root CLVs, then groups of
+  // up to GROUP mutually independent ops (greedy list schedule over ready
+  // ops, lowest post-order index first) so locality is kept while each
+  // group exposes several CLV loads at once
   std::vector<int> post;
   {
     std::vector<char> seen(nv, 0);
@@ -178,10 +262,6 @@ static void build_schedule(Engine &e) {
       }
     }
   }
-  // Groups of up to GROUP mutually independent ops for the persistent
-  // kernel: greedy list schedule over ready ops, lowest post-order index
-  // first, so locality of the depth-first order is kept while every group
-  // exposes several CLV loads at once.
   constexpr int GROUP = 4;
   {
     std::vector<int> pos(nv, -1), deps(nv, 0);
@@ -205,95 +285,19 @@ static void build_schedule(Engine &e) {
       std::vector<int> grp(ready.begin(), ready.begin() + take);
       ready.erase(ready.begin(), ready.begin() + take);
       for (int s : grp) {
-        const Variant &v = e.dag.var[s];
-        DevOp op{};
-        op.out = s;
-        op.nk = v.nkids;
-        for (int k = 0; k < 2; ++k) {
-          op.kid[k] = v.kid[k];
-          op.edge[k] = v.edge[k] < 0 ? 0 : v.edge[k];
-        }
-        op.root = root_of_slot[s];
-        e.ops_dfs.push_back(op);
+        e.ops_dfs.push_back(make_op(s));
         for (int par : parents[s])
-          if (--deps[par] == 0) ready.insert(std::upper_bound(ready.begin(), ready.end(), par, by_pos), par);
+          if (--deps[par] == 0)
+            ready.insert(std::upper_bound(ready.begin(), ready.end(), par, by_pos), par);
       }
       e.group_begin.push_back(static_cast<int32_t>(e.ops_dfs.size()));
     }
     if (e.ops_dfs.size() != post.size()) throw std::logic_error("op grouping lost ops");
-    // the adjoint walks the grouped order backwards
-    post.clear();
-    for (const DevOp &op : e.ops_dfs) post.push_back(op.out);
+    std::vector<int> order;  // adjoint: groups backwards, ops in index order
+    for (int g = static_cast<int>(e.group_begin.size()) - 2; g >= 0; --g)
+      for (int i = e.group_begin[g]; i < e.group_begin[g + 1]; ++i) order.push_back(i);
+    set_adjoint_flags(e.ops_dfs, order, nv);
   }
-
-  // adjoint: one work item per needed CLV and per tip, consumers in order
-  std::vector<std::vector<DevUse>> uses_of_slot(nv), uses_of_tip(e.T);
-  for (int s = 0; s < nv; ++s) {
-    if (!e.needed[s]) continue;
-    const Variant &v = e.dag.var[s];
-    for (int k = 0; k < v.nkids; ++k) {
-      DevUse u{};
-      u.parent = s;
-      u.edge = v.edge[k];
-      u.sib = v.nkids == 2 ? v.kid[1 - k] : NO_CHILD;
-      u.sib_edge = v.nkids == 2 ? v.edge[1 - k] : 0;
-      if (v.kid[k] >= 0)
-        uses_of_slot[v.kid[k]].push_back(u);
-      else
-        uses_of_tip[tip_of(v.kid[k])].push_back(u);
-    }
-  }
-  e.children.clear();
-  e.uses.clear();
-  std::vector<std::vector<int>> child_by_level(max_level + 1);
-  std::vector<int> root_index_of_slot(nv, -1);
-  for (size_t r = 0; r < e.root_slots.size(); ++r) root_index_of_slot[e.root_slots[r]] = r;
-  auto add_child = [&](int32_t code, const std::vector<DevUse> &us, int level) {
-    DevChild ch{};
-    ch.child = code;
-    ch.use_begin = static_cast<int>(e.uses.size());
-    for (DevUse u : us) {
-      u.grad_slot = static_cast<int>(e.uses.size());
-      e.uses.push_back(u);
-    }
-    ch.use_end = static_cast<int>(e.uses.size());
-    ch.root_index = code >= 0 ? root_index_of_slot[code] : -1;
-    child_by_level[level].push_back(static_cast<int>(e.children.size()));
-    e.children.push_back(ch);
-  };
-  for (int s = 0; s < nv; ++s)
-    if (e.needed[s]) add_child(s, uses_of_slot[s], e.dag.var[s].level);
-  for (int t = 0; t < e.T; ++t)
-    if (!uses_of_tip[t].empty()) add_child(tip_code(t), uses_of_tip[t], 0);
-  e.n_uses = static_cast<int>(e.uses.size());
-  e.child_lists.clear();
-  e.child_level_begin.assign(max_level + 2, 0);
-  // reverse order: highest level first
-  for (int l = max_level, i = 0; l >= 0; --l, ++i) {
-    e.child_level_begin[i] = static_cast<int>(e.child_lists.size());
-    for (int c : child_by_level[l]) e.child_lists.push_back(c);
-  }
-  e.child_level_begin[max_level + 1] = static_cast<int>(e.child_lists.size());
-  // FP32 form: items in reverse post-order (every consumer first), tips last
-  {
-    std::vector<int> item_of_slot(nv, -1);
-    for (size_t i = 0; i < e.children.size(); ++i)
-      if (e.children[i].child >= 0) item_of_slot[e.children[i].child] = static_cast<int>(i);
-    e.child_order_dfs.clear();
-    for (auto it = post.rbegin(); it != post.rend(); ++it) e.child_order_dfs.push_back(item_of_slot[*it]);
-    for (size_t i = 0; i < e.children.size(); ++i)
-      if (e.children[i].child < 0) e.child_order_dfs.push_back(static_cast<int>(i));
-  }
-  // uses grouped by edge
-  std::vector<std::vector<int>> by_edge(e.E);
-  for (int u = 0; u < e.n_uses; ++u) by_edge[e.uses[u].edge].push_back(u);
-  e.edge_use_begin.assign(e.E + 1, 0);
-  e.edge_use_list.clear();
-  for (int ed = 0; ed < e.E; ++ed) {
-    e.edge_use_begin[ed] = static_cast<int>(e.edge_use_list.size());
-    for (int u : by_edge[ed]) e.edge_use_list.push_back(u);
-  }
-  e.edge_use_begin[e.E] = static_cast<int>(e.edge_use_list.size());
 
   std::vector<DevClass> dc(nc);
   for (int k = 0; k < nc; ++k) {
@@ -302,12 +306,9 @@ static void build_schedule(Engine &e) {
     dc[k].use = e.class_use[k];
   }
   upload(e.d_ops, e.ops_all, e.stream);
+  upload(e.d_chunk_edges, e.chunk_edges, e.stream);
   upload(e.d_ops_dfs, e.ops_dfs, e.stream);
   upload(e.d_group_begin, e.group_begin, e.stream);
-  upload(e.d_child_order_dfs, e.child_order_dfs, e.stream);
-  upload(e.d_children, e.children, e.stream);
-  upload(e.d_child_lists, e.child_lists, e.stream);
-  upload(e.d_uses, e.uses, e.stream);
   upload(e.d_edge_use_begin, e.edge_use_begin, e.stream);
   upload(e.d_edge_use_list, e.edge_use_list, e.stream);
   upload(e.d_classes, dc, e.stream);
@@ -382,9 +383,10 @@ static void upload_models(Engine &e) {
     std::copy(pi.rates.begin(), pi.rates.end(), rates.begin() + p * C);
     std::copy(pi.model.freqs.begin(), pi.model.freqs.end(), fr.begin() + p * S);
   }
-  if (e.opt.precision == NRX_FP32) {
-    // r_c Q per partition and category (dP/dt = Q P), FP32
-    std::vector<float> Qf(static_cast<size_t>(P) * C * S * S);
+  {
+    // r_c Q per partition and category: dP/dt = Q P gives P' x = r_c Q (P x)
+    std::vector<double> Qd(static_cast<size_t>(P) * C * S * S);
+    std::vector<float> Qf(Qd.size());
     for (int p = 0; p < P; ++p) {
       const EigenModel &m = e.parts[p].model;
       for (int c = 0; c < C; ++c)
@@ -392,11 +394,13 @@ static void upload_models(Engine &e) {
           for (int j = 0; j < S; ++j) {
             double q = 0.0;
             for (int k = 0; k < S; ++k) q += m.V[i * S + k] * m.eval[k] * m.Vinv[k * S + j];
-            Qf[((static_cast<size_t>(p) * C + c) * S + i) * S + j] =
-                static_cast<float>(q * e.parts[p].rates[c]);
+            const size_t idx = ((static_cast<size_t>(p) * C + c) * S + i) * S + j;
+            Qd[idx] = q * e.parts[p].rates[c];
+            Qf[idx] = static_cast<float>(Qd[idx]);
           }
     }
-    upload(e.d_Qf, Qf, e.stream);
+    upload(e.d_Qd, Qd, e.stream);
+    if (e.opt.precision == NRX_FP32) upload(e.d_Qf, Qf, e.stream);
   }
   upload(e.d_V, V, e.stream);
   upload(e.d_Vinv, Vi, e.stream);
@@ -439,42 +443,41 @@ static void record_kernel_end(Engine &e, int kind) {
 }
 
 static void forward(Engine &e, bool incremental) {
-  const int n_levels = static_cast<int>(e.level_ops.size());
   if (!incremental) std::fill(e.clv_dirty.begin(), e.clv_dirty.end(), 1);
   bool all = true;
-  for (size_t i = 0; i < e.ops_all.size(); ++i)
-    if (!e.clv_dirty[e.ops_all[i].out]) {
+  for (const DevOp &op : e.ops_all)
+    if (!e.clv_dirty[op.out]) {
       all = false;
       break;
     }
   if (all) {
-    for (int l = 1; l < n_levels; ++l) {
-      const int b = e.level_begin[l], n = e.level_begin[l + 1] - b;
+    for (const LevelChunk &ch : e.chunks) {
       record_kernel_begin(e, 0);
-      launch_forward_level(e, e.d_ops.as<DevOp>() + b, n);
+      launch_forward_level(e, e.d_ops.as<DevOp>() + ch.op_begin, ch.op_end - ch.op_begin, ch);
       record_kernel_end(e, 0);
     }
   } else {
+    // only the invalid CLVs of each launch, with the launch's edge table
     std::vector<DevOp> dirty;
-    std::vector<int> begin(n_levels + 1, 0);
-    for (int l = 0; l < n_levels; ++l) {
-      begin[l] = static_cast<int>(dirty.size());
-      for (int i = e.level_begin[l]; i < e.level_begin[l + 1]; ++i)
+    std::vector<int> begin(e.chunks.size() + 1, 0);
+    for (size_t c = 0; c < e.chunks.size(); ++c) {
+      begin[c] = static_cast<int>(dirty.size());
+      for (int i = e.chunks[c].op_begin; i < e.chunks[c].op_end; ++i)
         if (e.clv_dirty[e.ops_all[i].out]) dirty.push_back(e.ops_all[i]);
     }
-    begin[n_levels] = static_cast<int>(dirty.size());
+    begin[e.chunks.size()] = static_cast<int>(dirty.size());
     if (!dirty.empty()) {
       upload(e.d_ops_dirty, dirty, e.stream);
-      for (int l = 1; l < n_levels; ++l) {
-        const int b = begin[l], n = begin[l + 1] - b;
+      for (size_t c = 0; c < e.chunks.size(); ++c) {
+        const int n = begin[c + 1] - begin[c];
         if (n == 0) continue;
         record_kernel_begin(e, 0);
-        launch_forward_level(e, e.d_ops_dirty.as<DevOp>() + b, n);
+        launch_forward_level(e, e.d_ops_dirty.as<DevOp>() + begin[c], n, e.chunks[c]);
         record_kernel_end(e, 0);
       }
     }
   }
-  for (size_t i = 0; i < e.ops_all.size(); ++i) e.clv_dirty[e.ops_all[i].out] = 0;
+  for (const DevOp &op : e.ops_all) e.clv_dirty[op.out] = 0;
 }
 
 static void begin_timing(Engine &e) {
@@ -541,12 +544,9 @@ static int evaluate(Engine &e, int variant, bool incremental, bool update_pm, bo
     launch_mixture(e, variant, e.pw);
     if (gradient) {
       e.d_adj.alloc(sizeof(double) * e.clv_elems_per_slot * e.dag.var.size());
-      const int nl = static_cast<int>(e.child_level_begin.size()) - 1;
-      for (int i = 0; i < nl; ++i) {
-        const int b = e.child_level_begin[i], n = e.child_level_begin[i + 1] - b;
-        if (n == 0) continue;
+      for (auto it = e.chunks.rbegin(); it != e.chunks.rend(); ++it) {
         record_kernel_begin(e, 1);
-        launch_adjoint_level(e, e.d_child_lists.as<int32_t>() + b, n);
+        launch_adjoint_level(e, *it);
         record_kernel_end(e, 1);
       }
       launch_grad_reduce(e, e.pw);
@@ -899,24 +899,27 @@ int nrx_get_stats(nrx_engine *eng, nrx_stats *out) {
     }
     const double root = e.root_slots.size() * (row + scl + 8.0 * sites);
     const double pm = static_cast<double>(e.P) * e.E * e.C * e.S * e.S * b;
+    // push-form adjoint: per op read its adjoint (when accumulated), its
+    // scaler, its children (and, for root CLVs, itself for the seed); per
+    // inner child write the delivered adjoint, and read it back first when
+    // another consumer delivered before
+    const double aexp = e.opt.precision == NRX_FP32 ? scl : 0.0;  // adjoint exponents
     double adj = 0.0;
-    for (const DevChild &ch : e.children) {
-      adj += ch.child >= 0 ? row + scl : tip_b;  // own CLV
-      for (int u = ch.use_begin; u < ch.use_end; ++u) {
-        const DevUse &us = e.uses[u];
-        adj += row + scl;  // parent adjoint + parent scaler
-        if (us.sib != NO_CHILD) adj += us.sib >= 0 ? row + scl : tip_b;
+    for (const DevOp &op : e.ops_all) {
+      adj += scl;
+      if (op.flags & OP_HAS_CONSUMERS) adj += row + aexp;
+      if (op.root >= 0) adj += row;
+      for (int k = 0; k < op.nk; ++k) {
+        if (op.kid[k] < 0) {
+          adj += tip_b;
+          continue;
+        }
+        adj += row + scl;  // child CLV and scaler
+        adj += row + aexp;  // delivered adjoint
+        if (!(op.flags & (k == 0 ? OP_FIRST_KID0 : OP_FIRST_KID1))) adj += row + aexp;
       }
-      if (ch.child >= 0) adj += row;  // adjoint written
     }
-    if (e.opt.precision == NRX_FP32) {
-      // per-site adjoint exponents: read per use, written per CLV
-      for (const DevChild &ch : e.children)
-        adj += (ch.use_end - ch.use_begin) * scl + (ch.child >= 0 ? scl : 0.0);
-      adj += pm;  // P (Q is per partition)
-    } else {
-      adj += 2.0 * pm;  // P and P'
-    }
+    adj += pm;
     out->fwd_op_bytes = fwd_ops;
     out->root_bytes = root;
     out->adj_bytes = adj;

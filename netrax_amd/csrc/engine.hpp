@@ -60,29 +60,27 @@ struct Dag {
 Dag compile_dag(const nrx_network_desc &net);
 
 // ---------------------------------------------------------------- device
-struct DevOp {  // forward CLV op
+// One CLV op.  The forward pass computes CLV `out` from its children; the
+// adjoint pass (push form) reads the adjoint of `out` and delivers the
+// back-propagated vectors to the children.
+struct DevOp {
   int32_t out;
   int32_t nk;
-  int32_t kid[2];
-  int32_t edge[2];
-  int32_t root;  // index of the root-CLV site sum this op feeds, or -1
-  int32_t pad;
+  int32_t kid[2];    // child codes
+  int32_t edge[2];   // edge to each child
+  int32_t ledge[2];  // row of that edge in the launch's LDS P table
+  int32_t root;      // index of the root-CLV site sum this op feeds, or -1
+  int32_t flags;     // OP_HAS_CONSUMERS | OP_FIRST_KID0 | OP_FIRST_KID1
+  int32_t gslot[2];  // gradient-partial row of (this op, child k)
 };
+constexpr int32_t OP_HAS_CONSUMERS = 1;  // adjoint of `out` was accumulated by consumers
+constexpr int32_t OP_FIRST_KID0 = 2;     // first consumer of kid[0] in adjoint order
+constexpr int32_t OP_FIRST_KID1 = 4;
 
-struct DevChild {  // adjoint work item: one CLV (or tip) and its consumers
-  int32_t child;
-  int32_t use_begin;
-  int32_t use_end;
-  int32_t root_index;  // >= 0 when the CLV is a displayed-tree root CLV
-};
-
-struct DevUse {  // child used by parent op `parent` across `edge`
-  int32_t parent;
-  int32_t edge;
-  int32_t sib;       // sibling child code or NO_CHILD for a unary op
-  int32_t sib_edge;
-  int32_t grad_slot; // row of the per-use gradient partials
-  int32_t pad;
+// A launch of the plain form: ops [op_begin, op_end) of one pruning level
+// whose edges are rows [edge_begin, edge_end) of the LDS edge table.
+struct LevelChunk {
+  int level, op_begin, op_end, edge_begin, edge_end;
 };
 
 struct DevClass {
@@ -106,10 +104,10 @@ struct Engine;
 
 // Kernel launchers (kernels.hip)
 void launch_pmatrices(Engine &e);
-void launch_forward_level(Engine &e, const DevOp *ops, int n_ops);
+void launch_forward_level(Engine &e, const DevOp *ops, int n_ops, const LevelChunk &ch);
 void launch_root_partials(Engine &e);
 void launch_mixture(Engine &e, int variant, int partials_per_tile);
-void launch_adjoint_level(Engine &e, const int32_t *children, int n_children);
+void launch_adjoint_level(Engine &e, const LevelChunk &ch);
 void launch_grad_reduce(Engine &e, int partials_per_tile);
 void launch_forward_persistent(Engine &e);
 void launch_adjoint_persistent(Engine &e);

@@ -70,19 +70,16 @@ struct Engine : nrx_engine {
   std::vector<std::vector<int>> level_ops;  // variant slots per level (needed only)
   std::vector<int> root_slots;              // distinct displayed-tree-root CLVs
   std::vector<int> class_root_index;        // [nc]
-  std::vector<DevOp> ops_all;               // levels concatenated
-  std::vector<int> level_begin;             // [n_levels + 1] into ops_all
-  std::vector<DevChild> children;           // adjoint work items
-  std::vector<int32_t> child_lists;         // reverse level order
-  std::vector<int> child_level_begin;       // [n_levels + 1], level 0 = tips
-  std::vector<DevUse> uses;
-  std::vector<int32_t> edge_use_begin, edge_use_list;
+  // plain form: ops in level order, cut into launches with bounded LDS tables
+  std::vector<DevOp> ops_all;
+  std::vector<LevelChunk> chunks;
+  std::vector<int32_t> chunk_edges;
+  // batched form: groups of independent ops in a depth-first-biased order
+  std::vector<DevOp> ops_dfs;
+  std::vector<int32_t> group_begin;  // [n_groups + 1] into ops_dfs
+  std::vector<int32_t> edge_use_begin, edge_use_list;  // gradient rows per edge
   int n_uses = 0;
-  // FP32 form: depth-first post-order of the ops, reverse order of adjoint items
-  std::vector<DevOp> ops_dfs;             // grouped order (see group_begin)
-  std::vector<int32_t> group_begin;       // [n_groups + 1] into ops_dfs
-  std::vector<int32_t> child_order_dfs;
-  int pw = 1;  // site-sum partials per tile (1 per workgroup, or 1 per wave)
+  int pw = 1;  // site-sum partials per tile (one per wave)
 
   // dirty tracking
   bool model_dirty = true;
@@ -100,9 +97,8 @@ struct Engine : nrx_engine {
   DevBuf d_V, d_Vinv, d_eval, d_rates, d_freqs, d_brlen;
   DevBuf d_P, d_dP, d_d2P, d_Pf, d_dPf;
   DevBuf d_clv, d_scl, d_adj, d_adj_exp, d_Qf;
-  DevBuf d_ops_dfs, d_group_begin, d_child_order_dfs;
-  DevBuf d_ops, d_ops_dirty;
-  DevBuf d_children, d_child_lists, d_uses, d_edge_use_begin, d_edge_use_list;
+  DevBuf d_ops, d_ops_dirty, d_chunk_edges, d_Qd;
+  DevBuf d_ops_dfs, d_group_begin, d_edge_use_begin, d_edge_use_list;
   DevBuf d_classes, d_class_dlogprob, d_root_slots;
   DevBuf d_root_partials, d_beta, d_grad_partials, d_root_sums, d_use_sums;
   DevBuf d_result;

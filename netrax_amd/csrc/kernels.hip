@@ -43,6 +43,14 @@ struct TipType<20> {
   using type = uint32_t;
 };
 
+// Dynamic LDS is one untyped buffer per kernel (all extern __shared__
+// declarations of a translation unit must agree on the type).
+template <typename T>
+__device__ __forceinline__ T *dyn_lds() {
+  extern __shared__ double lds_dyn[];
+  return reinterpret_cast<T *>(lds_dyn);
+}
+
 template <int C>
 __device__ __forceinline__ double cat_max(double m) {
 #pragma unroll
@@ -176,21 +184,46 @@ __global__ void k_pmatrices(DevLayout L, const double *__restrict__ V,
 }
 
 // ------------------------------------------------------- forward (FP64)
-// Plain form: one launch per pruning level.  Workgroup b handles tile b and
-// walks every op of the level for that tile, G mutually independent ops at
-// a time: all children of the G ops are loaded before any is combined, so
-// each wave keeps 2G CLV rows in flight.
+// Plain form: one launch per pruning level (or per part of a level whose
+// edges fit the LDS table).  Workgroup b handles tile b and walks every op
+// of the launch for that tile, G mutually independent ops at a time: all
+// children of the G ops are loaded before any is combined, so each wave
+// keeps 2G CLV rows in flight.  For 4-state data the P matrices of the
+// launch's edges are staged once in LDS.
+template <int S, int C>
+__device__ __forceinline__ const double *p_row(const double *sP, const double *P, int part,
+                                               const DevLayout &L, int ledge, int edge, int c) {
+  if constexpr (S == 4)
+    return sP + (ledge * C + c) * (S * S);
+  else
+    return P + ((static_cast<size_t>(part) * L.n_edges + edge) * C + c) * (S * S);
+}
+
+template <int S, int C>
+__device__ __forceinline__ void stage_p(double *sP, const double *P, const int32_t *edges,
+                                        int n_le, int part, const DevLayout &L) {
+  if constexpr (S == 4) {
+    constexpr int ROW = C * S * S;
+    for (int i = threadIdx.x; i < n_le * ROW; i += blockDim.x) {
+      const int le = i / ROW, r = i % ROW;
+      sP[i] = P[(static_cast<size_t>(part) * L.n_edges + edges[le]) * ROW + r];
+    }
+    __syncthreads();
+  }
+}
+
 template <int S, int C, int G>
 __global__ void __launch_bounds__(TILE *C)
-    k_forward_level(const DevOp *__restrict__ ops, int n_ops, DevLayout L,
-                    double *__restrict__ clv, int32_t *__restrict__ scl,
+    k_forward_level(const DevOp *__restrict__ ops, int n_ops, const int32_t *__restrict__ edges,
+                    int n_le, DevLayout L, double *__restrict__ clv, int32_t *__restrict__ scl,
                     const typename TipType<S>::type *__restrict__ tips,
                     const double *__restrict__ P, const int32_t *__restrict__ tile_part) {
+  double *sP = dyn_lds<double>();
   const int tile = blockIdx.x;
-  if (tile >= L.n_tiles) return;
+  const int part = tile_part[tile];
+  stage_p<S, C>(sP, P, edges, n_le, part, L);
   const int c = threadIdx.x % C;
   const int64_t site = static_cast<int64_t>(tile) * TILE + threadIdx.x / C;
-  const int part = tile_part[tile];
   const int64_t n = L.n_sites_pad;
   for (int g0 = 0; g0 < n_ops; g0 += G) {
     const int nb = min(G, n_ops - g0);
@@ -226,8 +259,7 @@ __global__ void __launch_bounds__(TILE *C)
         for (int k = 0; k < 2; ++k) {
           if (k < op.nk) {
             double y[S];
-            matvec<S>(P + ((static_cast<size_t>(part) * L.n_edges + op.edge[k]) * C + c) * (S * S),
-                      x[j][k], y);
+            matvec<S>(p_row<S, C>(sP, P, part, L, op.ledge[k], op.edge[k], c), x[j][k], y);
 #pragma unroll
             for (int i = 0; i < S; ++i) v[i] *= y[i];
           }
@@ -356,99 +388,108 @@ __global__ void k_mixture(DevLayout L, int n_root, int nc, int R, int variant,
 }
 
 // ------------------------------------------------------- adjoint (FP64)
-// Reverse pruning, pull form.  For a CLV o (or tip) with consumers q (parent
-// op across edge e, sibling term y = P_f sib or 1, rescale factor F of q):
-//   Abar_o[c,j] = seed_o[c,j] + sum_q sum_i Abar_q[c,i] F y[c,i] P_e[c,i,j]
-//   dJ/dt_e    += sum_{c,i} Abar_q[c,i] F y[c,i] (P'_e x_o)[c,i]
+// Reverse pruning in push form.  For op q with children k (edge e_k, CLV
+// x_k, y_k = P_k x_k) and rescale factor F = 2^(256 (k_q - sum k_k)):
+//   z_k      = Abar_q * F * prod_{j != k} y_j
+//   Abar_k  += P_k^T z_k            (delivered to child k)
+//   dJ/dt_k += z_k . (P'_k x_k),    P'_k x_k = r_c Q y_k
 // where J = sum_p sum_t beta_{p,t} lnL_{p,t} and Abar is the derivative of
-// J w.r.t. the *scaled* CLV values; the root seed is
-// beta * w_s * (1/C) pi_i / Ltilde_s.  Each CLV is written by exactly one
-// workgroup per tile and consumers are walked in a fixed order, so the
-// result is deterministic without atomics.
+// J w.r.t. the *scaled* CLV values; a root CLV adds the seed
+// beta * w_s * (1/C) pi_i / Ltilde_s.  Ops are walked in reverse
+// topological order; every consumer of a CLV delivers before the CLV's
+// own op runs, the first one writes and later ones accumulate.  The same
+// thread owns (site, category) in every op, so accumulation needs neither
+// atomics nor barriers and is deterministic.
 template <int S, int C>
 __global__ void __launch_bounds__(TILE *C)
-    k_adjoint_level(const int32_t *__restrict__ child_list, int n_items,
-                    const DevChild *__restrict__ children, const DevUse *__restrict__ uses,
-                    DevLayout L, const double *__restrict__ clv, const int32_t *__restrict__ scl,
-                    double *__restrict__ adj, const typename TipType<S>::type *__restrict__ tips,
-                    const double *__restrict__ P, const double *__restrict__ dP,
+    k_adjoint_level(const DevOp *__restrict__ ops, int n_ops, const int32_t *__restrict__ edges,
+                    int n_le, DevLayout L, const double *__restrict__ clv,
+                    const int32_t *__restrict__ scl, double *__restrict__ adj,
+                    const typename TipType<S>::type *__restrict__ tips,
+                    const double *__restrict__ P, const double *__restrict__ Qd,
                     const double *__restrict__ freqs, const double *__restrict__ weights,
                     const int32_t *__restrict__ tile_part, const double *__restrict__ beta,
                     int n_root, double *__restrict__ grad_partials) {
+  double *sP = dyn_lds<double>();
   constexpr int PW = TILE * C / 64;
   const int tile = blockIdx.x;
-  if (tile >= L.n_tiles) return;
+  const int part = tile_part[tile];
+  stage_p<S, C>(sP, P, edges, n_le, part, L);
   const int c = threadIdx.x % C;
   const int64_t site = static_cast<int64_t>(tile) * TILE + threadIdx.x / C;
-  const int part = tile_part[tile];
   const int64_t n = L.n_sites_pad;
   const size_t row = static_cast<size_t>(C * S);
   const size_t n_wt = static_cast<size_t>(L.n_tiles) * PW;
   const int wt = tile * PW + threadIdx.x / 64;
   const double w = weights[site];
-  for (int it = 0; it < n_items; ++it) {
-    const DevChild ch = children[child_list[it]];
-    double xo[S];
-    int ko = 0;
-    if (ch.child >= 0) {
-      load_clv<S>(clv + (static_cast<size_t>(ch.child) * n + site) * row + c * S, xo);
-      ko = scl[static_cast<size_t>(ch.child) * n + site];
-    } else {
-      tip_vec<S>(tips[static_cast<size_t>(-ch.child - 1) * n + site], xo);
-    }
+  const double *Qc = Qd + (static_cast<size_t>(part) * C + c) * (S * S);
+  for (int j = 0; j < n_ops; ++j) {
+    const DevOp op = ops[j];
     double a[S];
+    const int kq = scl[static_cast<size_t>(op.out) * n + site];
+    if (op.flags & OP_HAS_CONSUMERS) {
+      load_clv<S>(adj + (static_cast<size_t>(op.out) * n + site) * row + c * S, a);
+    } else {
 #pragma unroll
-    for (int i = 0; i < S; ++i) a[i] = 0.0;
-    if (ch.root_index >= 0) {
+      for (int i = 0; i < S; ++i) a[i] = 0.0;
+    }
+    if (op.root >= 0) {
+      double xq[S];
+      load_clv<S>(clv + (static_cast<size_t>(op.out) * n + site) * row + c * S, xq);
       double t = 0.0;
 #pragma unroll
-      for (int i = 0; i < S; ++i) t = fma(freqs[part * S + i], xo[i], t);
+      for (int i = 0; i < S; ++i) t = fma(freqs[part * S + i], xq[i], t);
       t = cat_sum<C>(t) * (1.0 / C);
-      const double b = beta[part * n_root + ch.root_index] * w;
+      const double b = beta[part * n_root + op.root] * w;
       if (b != 0.0) {
         const double f = b * (1.0 / C) / t;
 #pragma unroll
-        for (int i = 0; i < S; ++i) a[i] = f * freqs[part * S + i];
+        for (int i = 0; i < S; ++i) a[i] = fma(f, freqs[part * S + i], a[i]);
       }
     }
-    for (int u = ch.use_begin; u < ch.use_end; ++u) {
-      const DevUse us = uses[u];
-      double aq[S], y[S], z[S];
-      load_clv<S>(adj + (static_cast<size_t>(us.parent) * n + site) * row + c * S, aq);
-      const int kq = scl[static_cast<size_t>(us.parent) * n + site];
-      int ks = 0;
-      if (us.sib == NO_CHILD) {
+    double x[2][S], y[2][S];
+    int ksum = 0;
 #pragma unroll
-        for (int i = 0; i < S; ++i) y[i] = 1.0;
-      } else {
-        double xs[S];
-        if (us.sib >= 0) {
-          load_clv<S>(clv + (static_cast<size_t>(us.sib) * n + site) * row + c * S, xs);
-          ks = scl[static_cast<size_t>(us.sib) * n + site];
+    for (int k = 0; k < 2; ++k) {
+      if (k < op.nk) {
+        const int32_t kid = op.kid[k];
+        if (kid >= 0) {
+          load_clv<S>(clv + (static_cast<size_t>(kid) * n + site) * row + c * S, x[k]);
+          ksum += scl[static_cast<size_t>(kid) * n + site];
         } else {
-          tip_vec<S>(tips[static_cast<size_t>(-us.sib - 1) * n + site], xs);
+          tip_vec<S>(tips[static_cast<size_t>(-kid - 1) * n + site], x[k]);
         }
-        matvec<S>(P + ((static_cast<size_t>(part) * L.n_edges + us.sib_edge) * C + c) * (S * S),
-                  xs, y);
+        matvec<S>(p_row<S, C>(sP, P, part, L, op.ledge[k], op.edge[k], c), x[k], y[k]);
       }
-      const double F = (kq - ko - ks) > 0 ? SCALE_FACTOR : 1.0;
-#pragma unroll
-      for (int i = 0; i < S; ++i) z[i] = aq[i] * y[i] * F;
-      const size_t poff = ((static_cast<size_t>(part) * L.n_edges + us.edge) * C + c) * (S * S);
-      double back[S], dx[S];
-      matvec_t<S>(P + poff, z, back);
-#pragma unroll
-      for (int i = 0; i < S; ++i) a[i] += back[i];
-      matvec<S>(dP + poff, xo, dx);
-      double g = 0.0;
-#pragma unroll
-      for (int i = 0; i < S; ++i) g = fma(z[i], dx[i], g);
-      g = wave_sum(g);
-      if ((threadIdx.x & 63) == 0)
-        grad_partials[static_cast<size_t>(us.grad_slot) * n_wt + wt] = g;
     }
-    if (ch.child >= 0)
-      store_clv<S>(adj + (static_cast<size_t>(ch.child) * n + site) * row + c * S, a);
+    const double F = (kq - ksum) > 0 ? SCALE_FACTOR : 1.0;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      if (k < op.nk) {
+        double z[S], back[S], dx[S];
+#pragma unroll
+        for (int i = 0; i < S; ++i) z[i] = a[i] * F * (op.nk == 2 ? y[1 - k][i] : 1.0);
+        matvec_t<S>(p_row<S, C>(sP, P, part, L, op.ledge[k], op.edge[k], c), z, back);
+        matvec<S>(Qc, y[k], dx);
+        double g = 0.0;
+#pragma unroll
+        for (int i = 0; i < S; ++i) g = fma(z[i], dx[i], g);
+        g = wave_sum(g);
+        if ((threadIdx.x & 63) == 0)
+          grad_partials[static_cast<size_t>(op.gslot[k]) * n_wt + wt] = g;
+        const int32_t kid = op.kid[k];
+        if (kid >= 0) {
+          double *dst = adj + (static_cast<size_t>(kid) * n + site) * row + c * S;
+          if (!(op.flags & (k == 0 ? OP_FIRST_KID0 : OP_FIRST_KID1))) {
+            double acc[S];
+            load_clv<S>(dst, acc);
+#pragma unroll
+            for (int i = 0; i < S; ++i) back[i] += acc[i];
+          }
+          store_clv<S>(dst, back);
+        }
+      }
+    }
   }
 }
 
@@ -547,7 +588,7 @@ __global__ void __launch_bounds__(TILE *C)
                          const float *__restrict__ Pf, const int32_t *__restrict__ tile_part,
                          const double *__restrict__ freqs, const double *__restrict__ weights,
                          double *__restrict__ root_partials) {
-  extern __shared__ float sP[];  // P of this tile's partition: [edge][cat][4][4]
+  float *sP = dyn_lds<float>();  // P of this tile's partition: [edge][cat][4][4]
   constexpr int PW = TILE * C / 64;
   const int tile = blockIdx.x;
   const int part = tile_part[tile];
@@ -625,23 +666,54 @@ __global__ void __launch_bounds__(TILE *C)
   }
 }
 
-// Reverse sweep (pull form, as k_adjoint_level) with the adjoint of every
-// CLV stored as FP32 mantissas plus a per-site exponent g:
-// dJ/d(stored CLV) = stored adjoint * 2^g.  Contributions of the consumers
-// are aligned to the largest exponent before they are added, so no
-// intermediate can overflow.  dP/dt = Q P supplies P' x = r_c Q (P x).
+// Push-form reverse sweep (as k_adjoint_level) over the groups of the
+// forward order, backwards.  The adjoint of every CLV is stored as FP32
+// mantissas with a per-site exponent g: dJ/d(stored CLV) = stored * 2^g,
+// renormalised at every write; contributions are aligned to the larger
+// exponent before they are added, so no intermediate can overflow.
 template <int C>
+__device__ __forceinline__ void deliver_adjoint(float *__restrict__ dst_m, int32_t *__restrict__ dst_e,
+                                                bool first, bool lane0, float (&b)[4], int eb) {
+  float mb = cat_maxf<C>(fmaxf(fmaxf(fabsf(b[0]), fabsf(b[1])), fmaxf(fabsf(b[2]), fabsf(b[3]))));
+  if (!(mb > 0.f)) eb = EXP_EMPTY;
+  if (!first) {
+    float acc[4];
+    load4(dst_m, acc);
+    const int ea = *dst_e;
+    if (ea > eb) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) b[i] = acc[i] + ldexpf(b[i], eb - ea);
+      eb = ea;
+    } else {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) b[i] += ldexpf(acc[i], ea - eb);
+    }
+    mb = cat_maxf<C>(fmaxf(fmaxf(fabsf(b[0]), fabsf(b[1])), fmaxf(fabsf(b[2]), fabsf(b[3]))));
+  }
+  if (mb > 0.f) {
+    int e;
+    (void)frexpf(mb, &e);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) b[i] = ldexpf(b[i], -e);
+    eb += e;
+  } else {
+    eb = EXP_EMPTY;
+  }
+  store4(dst_m, b);
+  if (lane0) *dst_e = eb;
+}
+
+template <int C, int G>
 __global__ void __launch_bounds__(TILE *C)
-    k_adjoint_persistent(const int32_t *__restrict__ order, int n_items,
-                         const DevChild *__restrict__ children, const DevUse *__restrict__ uses,
-                         DevLayout L, const float *__restrict__ clv, const int32_t *__restrict__ scl,
-                         float *__restrict__ adj, int32_t *__restrict__ adj_exp,
-                         const uint8_t *__restrict__ tips, const float *__restrict__ Pf,
-                         const float *__restrict__ Qf, const int32_t *__restrict__ tile_part,
-                         const double *__restrict__ freqs, const double *__restrict__ weights,
-                         const double *__restrict__ beta, int n_root,
-                         double *__restrict__ grad_partials) {
-  extern __shared__ float sP[];  // [edge][cat][16] then Q: [cat][16]
+    k_adjoint_persistent(const DevOp *__restrict__ ops, const int32_t *__restrict__ group_begin,
+                         int n_groups, DevLayout L, const float *__restrict__ clv,
+                         const int32_t *__restrict__ scl, float *__restrict__ adj,
+                         int32_t *__restrict__ adj_exp, const uint8_t *__restrict__ tips,
+                         const float *__restrict__ Pf, const float *__restrict__ Qf,
+                         const int32_t *__restrict__ tile_part, const double *__restrict__ freqs,
+                         const double *__restrict__ weights, const double *__restrict__ beta,
+                         int n_root, double *__restrict__ grad_partials) {
+  float *sP = dyn_lds<float>();  // [edge][cat][16] then Q: [cat][16]
   constexpr int PW = TILE * C / 64;
   const int tile = blockIdx.x;
   const int part = tile_part[tile];
@@ -650,97 +722,109 @@ __global__ void __launch_bounds__(TILE *C)
   for (int i = threadIdx.x; i < np; i += blockDim.x) sP[i] = gP[i];
   for (int i = threadIdx.x; i < C * 16; i += blockDim.x) sP[np + i] = Qf[part * C * 16 + i];
   __syncthreads();
-  const float *sQ = sP + np;
   const int c = threadIdx.x % C;
+  const float *sQ = sP + np + c * 16;
   const int64_t site = static_cast<int64_t>(tile) * TILE + threadIdx.x / C;
   const int64_t n = L.n_sites_pad;
   const size_t n_wt = static_cast<size_t>(L.n_tiles) * PW;
   const int wt = tile * PW + threadIdx.x / 64;
+  const bool lane0 = c == 0;
   const double w = weights[site];
   double pi[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) pi[i] = freqs[part * 4 + i];
   const size_t row = C * 4;
-  for (int it = 0; it < n_items; ++it) {
-    const DevChild ch = children[order[it]];
-    float xo[4];
-    int ko = 0;
-    if (ch.child >= 0) {
-      load4(clv + (static_cast<size_t>(ch.child) * n + site) * row + c * 4, xo);
-      ko = scl[static_cast<size_t>(ch.child) * n + site];
-    } else {
-      tip_vec4(tips[static_cast<size_t>(-ch.child - 1) * n + site], xo);
-    }
-    float a[4] = {0.f, 0.f, 0.f, 0.f};
-    int ea = EXP_EMPTY;
-    if (ch.root_index >= 0) {
-      double t = 0.0;
+  for (int gi = n_groups - 1; gi >= 0; --gi) {
+    const int g0 = group_begin[gi];
+    const int nb = group_begin[gi + 1] - g0;
+    float aq[G][4], x[G][2][4];
+    int gq[G], kq[G], ks[G];
+    // load the adjoints and children of the whole group first
 #pragma unroll
-      for (int i = 0; i < 4; ++i) t = fma(pi[i], static_cast<double>(xo[i]), t);
-      t = cat_sum<C>(t) * (1.0 / C);
-      const double b = beta[part * n_root + ch.root_index] * w;
-      if (b != 0.0) {
-        int ef;
-        const double mf = frexp(b / (C * t), &ef);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) a[i] = static_cast<float>(mf * pi[i]);
-        ea = ef;
-      }
-    }
-    for (int u = ch.use_begin; u < ch.use_end; ++u) {
-      const DevUse us = uses[u];
-      float aq[4], y[4], z[4];
-      load4(adj + (static_cast<size_t>(us.parent) * n + site) * row + c * 4, aq);
-      const int gq = adj_exp[static_cast<size_t>(us.parent) * n + site];
-      const int kq = scl[static_cast<size_t>(us.parent) * n + site];
-      int ks = 0;
-      if (us.sib == NO_CHILD) {
-#pragma unroll
-        for (int i = 0; i < 4; ++i) y[i] = 1.f;
-      } else {
-        float xs[4];
-        if (us.sib >= 0) {
-          load4(clv + (static_cast<size_t>(us.sib) * n + site) * row + c * 4, xs);
-          ks = scl[static_cast<size_t>(us.sib) * n + site];
+    for (int j = 0; j < G; ++j) {
+      gq[j] = EXP_EMPTY;
+      kq[j] = ks[j] = 0;
+      if (j < nb) {
+        const DevOp op = ops[g0 + j];
+        kq[j] = scl[static_cast<size_t>(op.out) * n + site];
+        if (op.flags & OP_HAS_CONSUMERS) {
+          load4(adj + (static_cast<size_t>(op.out) * n + site) * row + c * 4, aq[j]);
+          gq[j] = adj_exp[static_cast<size_t>(op.out) * n + site];
         } else {
-          tip_vec4(tips[static_cast<size_t>(-us.sib - 1) * n + site], xs);
+#pragma unroll
+          for (int i = 0; i < 4; ++i) aq[j][i] = 0.f;
         }
-        mv4(sP + (us.sib_edge * C + c) * 16, xs, y);
+#pragma unroll
+        for (int kk = 0; kk < 2; ++kk) {
+          if (kk < op.nk) {
+            const int32_t kid = op.kid[kk];
+            if (kid >= 0) {
+              load4(clv + (static_cast<size_t>(kid) * n + site) * row + c * 4, x[j][kk]);
+              ks[j] += scl[static_cast<size_t>(kid) * n + site];
+            } else {
+              tip_vec4(tips[static_cast<size_t>(-kid - 1) * n + site], x[j][kk]);
+            }
+          }
+        }
       }
-#pragma unroll
-      for (int i = 0; i < 4; ++i) z[i] = aq[i] * y[i];
-      const int ez = gq + kq - ko - ks;
-      const float *Pe = sP + (us.edge * C + c) * 16;
-      float back[4], px[4], qx[4];
-      mv4t(Pe, z, back);
-      if (ez > ea) {
-#pragma unroll
-        for (int i = 0; i < 4; ++i) a[i] = ldexpf(a[i], ea - ez) + back[i];
-        ea = ez;
-      } else {
-#pragma unroll
-        for (int i = 0; i < 4; ++i) a[i] += ldexpf(back[i], ez - ea);
-      }
-      mv4(Pe, xo, px);
-      mv4(sQ + c * 16, px, qx);
-      const float gd = fmaf(z[3], qx[3], fmaf(z[2], qx[2], fmaf(z[1], qx[1], z[0] * qx[0])));
-      double gv = ldexp(static_cast<double>(gd), ez);
-      gv = wave_sum(gv);
-      if ((threadIdx.x & 63) == 0) grad_partials[static_cast<size_t>(us.grad_slot) * n_wt + wt] = gv;
     }
-    if (ch.child >= 0) {
-      const float m = cat_maxf<C>(fmaxf(fmaxf(a[0], a[1]), fmaxf(a[2], a[3])));
-      int e = 0;
-      if (m > 0.f) {
-        (void)frexpf(m, &e);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) a[i] = ldexpf(a[i], -e);
-        ea += e;
-      } else {
-        ea = EXP_EMPTY;
+    for (int j = 0; j < G; ++j) {
+      if (j < nb) {
+        const DevOp op = ops[g0 + j];
+        float a[4] = {aq[j][0], aq[j][1], aq[j][2], aq[j][3]};
+        int ea = gq[j];
+        if (op.root >= 0) {
+          float xq[4];
+          load4(clv + (static_cast<size_t>(op.out) * n + site) * row + c * 4, xq);
+          double t = 0.0;
+#pragma unroll
+          for (int i = 0; i < 4; ++i) t = fma(pi[i], static_cast<double>(xq[i]), t);
+          t = cat_sum<C>(t) * (1.0 / C);
+          const double b = beta[part * n_root + op.root] * w;
+          int ef = EXP_EMPTY;
+          float sd[4] = {0.f, 0.f, 0.f, 0.f};
+          if (b != 0.0) {
+            const double mf = frexp(b / (C * t), &ef);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) sd[i] = static_cast<float>(mf * pi[i]);
+          }
+          if (ef > ea) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = sd[i] + ldexpf(a[i], ea - ef);
+            ea = ef;
+          } else {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] += ldexpf(sd[i], ef - ea);
+          }
+        }
+        float y[2][4];
+#pragma unroll
+        for (int kk = 0; kk < 2; ++kk)
+          if (kk < op.nk) mv4(sP + (op.edge[kk] * C + c) * 16, x[j][kk], y[kk]);
+        const int ez = ea + kq[j] - ks[j];
+#pragma unroll
+        for (int kk = 0; kk < 2; ++kk) {
+          if (kk < op.nk) {
+            float z[4], back[4], qx[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) z[i] = op.nk == 2 ? a[i] * y[1 - kk][i] : a[i];
+            const float *Pe = sP + (op.edge[kk] * C + c) * 16;
+            mv4t(Pe, z, back);
+            mv4(sQ, y[kk], qx);
+            const float gd = fmaf(z[3], qx[3], fmaf(z[2], qx[2], fmaf(z[1], qx[1], z[0] * qx[0])));
+            double gv = wave_sum(ldexp(static_cast<double>(gd), ez));
+            if ((threadIdx.x & 63) == 0)
+              grad_partials[static_cast<size_t>(op.gslot[kk]) * n_wt + wt] = gv;
+            const int32_t kid = op.kid[kk];
+            if (kid >= 0)
+              deliver_adjoint<C>(adj + (static_cast<size_t>(kid) * n + site) * row + c * 4,
+                                 adj_exp + static_cast<size_t>(kid) * n + site,
+                                 (op.flags & (kk == 0 ? OP_FIRST_KID0 : OP_FIRST_KID1)) != 0,
+                                 lane0, back, ez);
+          }
+        }
       }
-      store4(adj + (static_cast<size_t>(ch.child) * n + site) * row + c * 4, a);
-      if (c == 0) adj_exp[static_cast<size_t>(ch.child) * n + site] = ea;
     }
   }
 }
@@ -779,15 +863,18 @@ void launch_pmatrices(Engine &e) {
   NRX_HIP(hipGetLastError());
 }
 
-void launch_forward_level(Engine &e, const DevOp *ops, int n_ops) {
+void launch_forward_level(Engine &e, const DevOp *ops, int n_ops, const LevelChunk &ch) {
   if (n_ops == 0) return;
+  const int n_le = ch.edge_end - ch.edge_begin;
   dispatch_sc(e.S, e.C, [&](auto s, auto c) {
     constexpr int S = decltype(s)::value, C = decltype(c)::value;
     constexpr int G = S == 4 ? 4 : 1;
     using TT = typename TipType<S>::type;
-    hipLaunchKernelGGL((k_forward_level<S, C, G>), dim3(e.L.n_tiles), dim3(TILE * C), 0,
-                       e.stream, ops, n_ops, e.L, e.d_clv.as<double>(), e.d_scl.as<int32_t>(),
-                       e.d_tips.as<TT>(), e.d_P.as<double>(), e.d_tile_part.as<int32_t>());
+    const size_t lds = S == 4 ? sizeof(double) * n_le * C * S * S : 0;
+    hipLaunchKernelGGL((k_forward_level<S, C, G>), dim3(e.L.n_tiles), dim3(TILE * C), lds,
+                       e.stream, ops, n_ops, e.d_chunk_edges.as<int32_t>() + ch.edge_begin, n_le,
+                       e.L, e.d_clv.as<double>(), e.d_scl.as<int32_t>(), e.d_tips.as<TT>(),
+                       e.d_P.as<double>(), e.d_tile_part.as<int32_t>());
   });
   NRX_HIP(hipGetLastError());
 }
@@ -818,15 +905,18 @@ void launch_mixture(Engine &e, int variant, int pw) {
   NRX_HIP(hipGetLastError());
 }
 
-void launch_adjoint_level(Engine &e, const int32_t *children, int n_children) {
-  if (n_children == 0) return;
+void launch_adjoint_level(Engine &e, const LevelChunk &ch) {
+  const int n_ops = ch.op_end - ch.op_begin, n_le = ch.edge_end - ch.edge_begin;
+  if (n_ops == 0) return;
   dispatch_sc(e.S, e.C, [&](auto s, auto c) {
     constexpr int S = decltype(s)::value, C = decltype(c)::value;
     using TT = typename TipType<S>::type;
-    hipLaunchKernelGGL((k_adjoint_level<S, C>), dim3(e.L.n_tiles), dim3(TILE * C), 0, e.stream,
-                       children, n_children, e.d_children.as<DevChild>(), e.d_uses.as<DevUse>(),
-                       e.L, e.d_clv.as<double>(), e.d_scl.as<int32_t>(), e.d_adj.as<double>(),
-                       e.d_tips.as<TT>(), e.d_P.as<double>(), e.d_dP.as<double>(),
+    const size_t lds = S == 4 ? sizeof(double) * n_le * C * S * S : 0;
+    hipLaunchKernelGGL((k_adjoint_level<S, C>), dim3(e.L.n_tiles), dim3(TILE * C), lds, e.stream,
+                       e.d_ops.as<DevOp>() + ch.op_begin, n_ops,
+                       e.d_chunk_edges.as<int32_t>() + ch.edge_begin, n_le, e.L,
+                       e.d_clv.as<double>(), e.d_scl.as<int32_t>(), e.d_adj.as<double>(),
+                       e.d_tips.as<TT>(), e.d_P.as<double>(), e.d_Qd.as<double>(),
                        e.d_freqs.as<double>(), e.d_weights.as<double>(),
                        e.d_tile_part.as<int32_t>(), e.d_beta.as<double>(),
                        static_cast<int>(e.root_slots.size()), e.d_grad_partials.as<double>());
@@ -875,13 +965,13 @@ void launch_adjoint_persistent(Engine &e) {
   const size_t lds = sizeof(float) * (e.E * e.C * 16 + e.C * 16);
   dispatch_c(e.C, [&](auto c) {
     constexpr int C = decltype(c)::value;
-    hipLaunchKernelGGL((k_adjoint_persistent<C>), dim3(e.L.n_tiles), dim3(TILE * C), lds,
-                       e.stream, e.d_child_order_dfs.as<int32_t>(),
-                       static_cast<int>(e.child_order_dfs.size()), e.d_children.as<DevChild>(),
-                       e.d_uses.as<DevUse>(), e.L, e.d_clv.as<float>(), e.d_scl.as<int32_t>(),
-                       e.d_adj.as<float>(), e.d_adj_exp.as<int32_t>(), e.d_tips.as<uint8_t>(),
-                       e.d_Pf.as<float>(), e.d_Qf.as<float>(), e.d_tile_part.as<int32_t>(),
-                       e.d_freqs.as<double>(), e.d_weights.as<double>(), e.d_beta.as<double>(),
+    hipLaunchKernelGGL((k_adjoint_persistent<C, 4>), dim3(e.L.n_tiles), dim3(TILE * C), lds,
+                       e.stream, e.d_ops_dfs.as<DevOp>(), e.d_group_begin.as<int32_t>(),
+                       static_cast<int>(e.group_begin.size()) - 1, e.L, e.d_clv.as<float>(),
+                       e.d_scl.as<int32_t>(), e.d_adj.as<float>(), e.d_adj_exp.as<int32_t>(),
+                       e.d_tips.as<uint8_t>(), e.d_Pf.as<float>(), e.d_Qf.as<float>(),
+                       e.d_tile_part.as<int32_t>(), e.d_freqs.as<double>(),
+                       e.d_weights.as<double>(), e.d_beta.as<double>(),
                        static_cast<int>(e.root_slots.size()), e.d_grad_partials.as<double>());
   });
   NRX_HIP(hipGetLastError());
