@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstring>
 #include <exception>
+#include <functional>
 #include <stdexcept>
 #include <string>
 
@@ -346,6 +347,7 @@ static void set_probs(Engine &e, const double *p) {
   const std::vector<char> old_use = e.class_use;
   compute_class_probs(e);
   if (old_use != e.class_use) {
+    e.deriv_valid = false;
     build_schedule(e);
     std::fill(e.clv_dirty.begin(), e.clv_dirty.end(), 1);
   } else {
@@ -566,6 +568,76 @@ static int evaluate(Engine &e, int variant, bool incremental, bool update_pm, bo
   return NRX_OK;
 }
 
+
+// Sumtables of branch `edge` for every used displayed tree that contains it
+// (alive edge set, as isActiveBranch + the target variant existing,
+// LikelihoodDerivatives.cpp:313-343).  A tree whose root lies below the
+// branch (the branch is on the dead path above the displayed-tree root)
+// contributes derivative 0.
+static void build_sumtables(Engine &e, int edge) {
+  const int nc = static_cast<int>(e.dag.classes.size());
+  const int nv = static_cast<int>(e.dag.var.size());
+  if (e.d_ops_by_slot.p == nullptr || e.d_ops_by_slot.bytes < sizeof(DevOp) * nv) {
+    std::vector<DevOp> all(nv);
+    for (int s = 0; s < nv; ++s) {
+      const Variant &v = e.dag.var[s];
+      DevOp op{};
+      op.out = s;
+      op.nk = v.nkids;
+      for (int k = 0; k < 2; ++k) {
+        op.kid[k] = k < v.nkids ? v.kid[k] : NO_CHILD;
+        op.edge[k] = k < v.nkids ? v.edge[k] : 0;
+      }
+      op.root = -1;
+      all[s] = op;
+    }
+    upload(e.d_ops_by_slot, all, e.stream);
+  }
+  e.deriv_class.clear();
+  e.deriv_path.clear();
+  std::vector<PathStep> steps;
+  std::vector<int32_t> begin{0};
+  for (int k = 0; k < nc; ++k) {
+    if (!e.class_use[k]) continue;
+    const auto &ed = e.dag.classes[k].edges;
+    if (!((ed[edge / 64] >> (edge % 64)) & 1ull)) continue;
+    e.deriv_class.push_back(k);
+    // depth-first search below the displayed-tree root for the op holding `edge`
+    std::vector<PathStep> path;
+    std::function<bool(int)> find = [&](int slot) -> bool {
+      const Variant &v = e.dag.var[slot];
+      for (int c = 0; c < v.nkids; ++c) {
+        path.push_back({slot, c});
+        if (v.edge[c] == edge) return true;
+        if (v.kid[c] >= 0 && find(v.kid[c])) return true;
+        path.pop_back();
+      }
+      return false;
+    };
+    if (find(e.dag.classes[k].dtroot_slot)) {
+      e.deriv_path.push_back(static_cast<int>(begin.size()) - 1);
+      steps.insert(steps.end(), path.begin(), path.end());
+      begin.push_back(static_cast<int32_t>(steps.size()));
+    } else {
+      e.deriv_path.push_back(-1);
+    }
+  }
+  const int np = static_cast<int>(begin.size()) - 1;
+  e.deriv_n_paths = np;
+  if (np > 0) {
+    upload(e.d_steps, steps, e.stream);
+    upload(e.d_path_begin, begin, e.stream);
+    e.d_st.alloc(sizeof(double) * e.clv_elems_per_slot * np);
+    e.d_st_partials.alloc(sizeof(double) * np * 3 * static_cast<size_t>(e.L.n_tiles) * e.pw);
+    e.d_st_sums.alloc(sizeof(double) * np * 3 * e.P);
+    e.h_st_sums.assign(static_cast<size_t>(np) * 3 * e.P, 0.0);
+    launch_sumtables(e, np);
+  }
+  e.deriv_base_logl.assign(e.h_result + e.rl.tree_logl, e.h_result + e.rl.tree_logl + e.P * nc);
+  e.deriv_edge = edge;
+  e.deriv_valid = true;
+}
+
 }  // namespace nrx
 
 using namespace nrx;
@@ -769,6 +841,7 @@ int nrx_set_model(nrx_engine *eng, int32_t partition, const double *subst_rates,
     pi.rates = gamma_mean_rates(alpha, e.C);
     pi.model_set = true;
     e.model_dirty = true;
+    e.deriv_valid = false;
     return NRX_OK;
   });
 }
@@ -784,6 +857,7 @@ int nrx_set_category_rates(nrx_engine *eng, int32_t partition, const double *rat
       if (!(rates[c] >= 0.0)) throw std::invalid_argument("category rates must be >= 0");
     pi.rates.assign(rates, rates + e.C);
     e.model_dirty = true;
+    e.deriv_valid = false;
     return NRX_OK;
   });
 }
@@ -799,6 +873,8 @@ int nrx_set_branch_lengths(nrx_engine *eng, const double *brlen) {
       if (b != e.brlen[i]) {
         e.brlen[i] = b;
         e.edge_dirty[i] = 1;
+        // sumtables of a branch stay valid while only that branch moves
+        if (i != e.deriv_edge) e.deriv_valid = false;
       }
     }
     return NRX_OK;
@@ -848,16 +924,100 @@ int nrx_gradient(nrx_engine *eng, int32_t variant, double *total, double *dbrlen
 int nrx_brlen_derivatives(nrx_engine *eng, int32_t variant, int32_t edge, int32_t mode,
                           double brlen, double *d1, double *d2, double *part_d1,
                           double *part_d2) {
-  (void)eng;
-  (void)variant;
-  (void)edge;
-  (void)mode;
-  (void)brlen;
-  (void)d1;
-  (void)d2;
-  (void)part_d1;
-  (void)part_d2;
-  return fail(NRX_ERR_UNSUPPORTED, "nrx_brlen_derivatives is not available in this build");
+  return guarded([&]() -> int {
+    if (!eng || !d1 || !d2) throw std::invalid_argument("null argument");
+    Engine &e = *static_cast<Engine *>(eng);
+    if (edge < 0 || edge >= e.E) throw std::invalid_argument("edge index out of range");
+    if (mode != NRX_DERIV_REFERENCE && mode != NRX_DERIV_TRUE)
+      throw std::invalid_argument("unknown derivative mode");
+    if (variant != NRX_AVERAGE_DISPLAYED_TREES && variant != NRX_BEST_DISPLAYED_TREE)
+      throw std::invalid_argument("unknown likelihood variant");
+    if (!(brlen >= 0.0)) throw std::invalid_argument("branch length must be >= 0");
+    const int nc = static_cast<int>(e.dag.classes.size());
+    if (!e.deriv_valid || e.deriv_edge != edge) {
+      const int rc = evaluate(e, variant, true, true, false);
+      if (rc != NRX_OK) return rc;
+      build_sumtables(e, edge);
+    }
+    const int np = e.deriv_n_paths;
+    if (np > 0) {
+      launch_sumtable_eval(e, np, brlen);
+      NRX_HIP(hipMemcpyAsync(e.h_st_sums.data(), e.d_st_sums.p, sizeof(double) * np * 3 * e.P,
+                             hipMemcpyDeviceToHost, e.stream));
+      NRX_HIP(hipStreamSynchronize(e.stream));
+    }
+    // per partition: tree lnL at brlen and its first two derivatives, then
+    // the mixture of computePartitionLhData (LikelihoodDerivatives.cpp:30-188)
+    double D1 = 0.0, D2 = 0.0;
+    std::vector<double> ll(nc), l1(nc), l2(nc);
+    std::vector<char> has(nc, 0);
+    for (size_t i = 0; i < e.deriv_class.size(); ++i) has[e.deriv_class[i]] = 1;
+    for (int p = 0; p < e.P; ++p) {
+      for (int k = 0; k < nc; ++k) {
+        ll[k] = e.deriv_base_logl[static_cast<size_t>(p) * nc + k];
+        l1[k] = l2[k] = 0.0;
+      }
+      for (size_t i = 0; i < e.deriv_class.size(); ++i) {
+        const int k = e.deriv_class[i], j = e.deriv_path[i];
+        if (j < 0) continue;
+        const double *v = &e.h_st_sums[static_cast<size_t>(j) * 3 * e.P];
+        ll[k] += v[0 * e.P + p];
+        l1[k] = v[1 * e.P + p];
+        l2[k] = v[2 * e.P + p];
+      }
+      double pd1 = 0.0, pd2 = 0.0;
+      std::vector<int> members;
+      for (int k = 0; k < nc; ++k)
+        if (e.class_use[k] && (mode == NRX_DERIV_TRUE || has[k])) members.push_back(k);
+      if (members.size() == 1 && mode == NRX_DERIV_REFERENCE) {
+        pd1 = l1[members[0]];  // single-tree mode (LikelihoodDerivatives.cpp:145-151)
+        pd2 = l2[members[0]];
+      } else if (!members.empty()) {
+        if (variant == NRX_AVERAGE_DISPLAYED_TREES) {
+          double mx = -INFINITY;
+          for (int k : members) mx = std::max(mx, e.class_logprob[k] + ll[k]);
+          double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+          for (int k : members) {
+            const double wgt = std::exp(e.class_logprob[k] + ll[k] - mx);
+            s0 += wgt;
+            s1 += wgt * l1[k];
+            s2 += wgt * (l1[k] * l1[k] + l2[k]);
+          }
+          pd1 = s1 / s0;
+          pd2 = (s2 * s0 - s1 * s1) / (s0 * s0);
+        } else if (mode == NRX_DERIV_REFERENCE) {
+          // the reference picks the tree maximising tree_logl * tree_prob
+          // (LikelihoodDerivatives.cpp:161-167)
+          double best = -INFINITY;
+          for (int k : members) {
+            const double score = ll[k] * std::exp(e.class_logprob[k]);
+            if (score > best) {
+              best = score;
+              pd1 = l1[k];
+              pd2 = l2[k];
+            }
+          }
+        } else {
+          double best = -INFINITY;
+          for (int k : members) {
+            const double score = e.class_logprob[k] + ll[k];
+            if (score > best) {
+              best = score;
+              pd1 = l1[k];
+              pd2 = l2[k];
+            }
+          }
+        }
+      }
+      if (part_d1) part_d1[p] = pd1;
+      if (part_d2) part_d2[p] = pd2;
+      D1 += pd1;
+      D2 += pd2;
+    }
+    *d1 = D1;
+    *d2 = D2;
+    return NRX_OK;
+  });
 }
 
 int nrx_tree_logls(nrx_engine *eng, double *tree_logl, double *class_logprob) {

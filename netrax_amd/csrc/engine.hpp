@@ -83,6 +83,12 @@ struct LevelChunk {
   int level, op_begin, op_end, edge_begin, edge_end;
 };
 
+// One step of a root-to-branch path: op slot and the child index taken.
+struct PathStep {
+  int32_t op;
+  int32_t k;
+};
+
 struct DevClass {
   double logprob;
   int32_t root_index;
@@ -111,6 +117,8 @@ void launch_adjoint_level(Engine &e, const LevelChunk &ch);
 void launch_grad_reduce(Engine &e, int partials_per_tile);
 void launch_forward_persistent(Engine &e);
 void launch_adjoint_persistent(Engine &e);
+void launch_sumtables(Engine &e, int n_paths);
+void launch_sumtable_eval(Engine &e, int n_paths, double brlen);
 
 struct HipCheck {
   static void check(hipError_t err, const char *what);

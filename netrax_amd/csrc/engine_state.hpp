@@ -81,6 +81,17 @@ struct Engine : nrx_engine {
   int n_uses = 0;
   int pw = 1;  // site-sum partials per tile (one per wave)
 
+  // branch derivatives: sumtables of one branch for every displayed tree
+  // containing it (valid while nothing but that branch's length changes)
+  bool deriv_valid = false;
+  int deriv_edge = -1;
+  std::vector<int> deriv_class;       // used classes whose alive edges contain the edge
+  std::vector<int> deriv_path;        // per such class: sumtable row, or -1 (edge above the root)
+  int deriv_n_paths = 0;
+  std::vector<double> deriv_base_logl;  // [P][nc] tree lnLs when the sumtables were built
+  DevBuf d_ops_by_slot, d_steps, d_path_begin, d_st, d_st_partials, d_st_sums;
+  std::vector<double> h_st_sums;
+
   // dirty tracking
   bool model_dirty = true;
   std::vector<char> edge_dirty;  // P needs recompute

@@ -829,6 +829,163 @@ __global__ void __launch_bounds__(TILE *C)
   }
 }
 
+
+// ===================================================== branch derivatives
+// Sumtables of one branch e, one row per displayed tree t containing it
+// (the device-side counterpart of computePartitionSumtables,
+// src/likelihood/LikelihoodDerivatives.cpp:249-344).  A path adjoint walks
+// from t's root CLV down to e with the seed (1/C) pi_i / Ltilde (weight 1),
+// so at e the vector z = d log L_s / d (P_e x) satisfies z . (P_e x) = 1.
+// In the eigenbasis of the partition's rate matrix
+//   ST[s,c,k] = (z^T V)_k (V^-1 x)_k,   rho_s(b) = sum_{c,k} ST e^(lambda_k r_c b)
+// and rho_s(b) = L_s(b) / L_s(b_now).  Works on either form's CLVs (T =
+// double with 2^256 scaler units, or float with unit exponents) in FP64.
+template <int S>
+__device__ __forceinline__ void load_any(const double *p, double (&x)[S]) {
+  load_clv<S>(p, x);
+}
+template <int S>
+__device__ __forceinline__ void load_any(const float *p, double (&x)[S]) {
+#pragma unroll
+  for (int i = 0; i < S; ++i) x[i] = static_cast<double>(p[i]);
+}
+
+template <int S, int C, typename T>
+__global__ void __launch_bounds__(TILE *C)
+    k_sumtables(const PathStep *__restrict__ steps, const int32_t *__restrict__ path_begin,
+                int n_paths, const DevOp *__restrict__ ops_by_slot, DevLayout L,
+                const int32_t *__restrict__ tile_part, const T *__restrict__ clv,
+                const int32_t *__restrict__ scl, int scale_unit,
+                const typename TipType<S>::type *__restrict__ tips,
+                const double *__restrict__ P, const double *__restrict__ V,
+                const double *__restrict__ Vinv, const double *__restrict__ freqs,
+                double *__restrict__ st) {
+  const int tile = blockIdx.x;
+  const int part = tile_part[tile];
+  const int c = threadIdx.x % C;
+  const int64_t site = static_cast<int64_t>(tile) * TILE + threadIdx.x / C;
+  const int64_t n = L.n_sites_pad;
+  const size_t row = static_cast<size_t>(C * S);
+  const double *Vp = V + static_cast<size_t>(part) * S * S;
+  const double *Wp = Vinv + static_cast<size_t>(part) * S * S;
+  auto prow = [&](int edge) {
+    return P + ((static_cast<size_t>(part) * L.n_edges + edge) * C + c) * (S * S);
+  };
+  for (int j = 0; j < n_paths; ++j) {
+    const int b0 = path_begin[j], b1 = path_begin[j + 1];
+    const int root = steps[b0].op;
+    double a[S];
+    {
+      double xr[S];
+      load_any<S>(clv + (static_cast<size_t>(root) * n + site) * row + c * S, xr);
+      double t = 0.0;
+#pragma unroll
+      for (int i = 0; i < S; ++i) t = fma(freqs[part * S + i], xr[i], t);
+      t = cat_sum<C>(t) * (1.0 / C);
+#pragma unroll
+      for (int i = 0; i < S; ++i) a[i] = freqs[part * S + i] / (C * t);
+    }
+    for (int sidx = b0; sidx < b1; ++sidx) {
+      const DevOp op = ops_by_slot[steps[sidx].op];
+      const int k = steps[sidx].k;
+      const int kq = scl[static_cast<size_t>(op.out) * n + site];
+      const int32_t child = op.kid[k];
+      const int kc = child >= 0 ? scl[static_cast<size_t>(child) * n + site] : 0;
+      double y[S];
+      int ks = 0;
+      if (op.nk == 2) {
+        double xs[S];
+        const int32_t sib = op.kid[1 - k];
+        if (sib >= 0) {
+          load_any<S>(clv + (static_cast<size_t>(sib) * n + site) * row + c * S, xs);
+          ks = scl[static_cast<size_t>(sib) * n + site];
+        } else {
+          tip_vec<S>(tips[static_cast<size_t>(-sib - 1) * n + site], xs);
+        }
+        matvec<S>(prow(op.edge[1 - k]), xs, y);
+      } else {
+#pragma unroll
+        for (int i = 0; i < S; ++i) y[i] = 1.0;
+      }
+      const double F = ldexp(1.0, scale_unit * (kq - kc - ks));
+      double z[S];
+#pragma unroll
+      for (int i = 0; i < S; ++i) z[i] = a[i] * F * y[i];
+      if (sidx + 1 < b1) {
+        matvec_t<S>(prow(op.edge[k]), z, a);  // adjoint of the on-path child
+      } else {
+        double xc[S], u[S], v[S];
+        if (child >= 0)
+          load_any<S>(clv + (static_cast<size_t>(child) * n + site) * row + c * S, xc);
+        else
+          tip_vec<S>(tips[static_cast<size_t>(-child - 1) * n + site], xc);
+        matvec_t<S>(Vp, z, u);  // u_k = sum_i z_i V[i][k]
+        matvec<S>(Wp, xc, v);   // v_k = sum_j Vinv[k][j] x_j
+        double *out = st + (static_cast<size_t>(j) * n + site) * row + c * S;
+#pragma unroll
+        for (int kk = 0; kk < S; ++kk) out[kk] = u[kk] * v[kk];
+      }
+    }
+  }
+}
+
+// Per displayed tree j and wave: sums over sites of w log rho(b),
+// w rho'/rho and w (rho''/rho - (rho'/rho)^2) at branch length b.
+template <int S, int C>
+__global__ void __launch_bounds__(TILE *C)
+    k_sumtable_eval(const double *__restrict__ st, int n_paths, DevLayout L,
+                    const int32_t *__restrict__ tile_part,
+                    const double *__restrict__ ev, const double *__restrict__ rates,
+                    const double *__restrict__ weights, double brlen,
+                    double *__restrict__ partials) {
+  constexpr int PW = TILE * C / 64;
+  const int tile = blockIdx.x;
+  const int part = tile_part[tile];
+  const int c = threadIdx.x % C;
+  const int64_t site = static_cast<int64_t>(tile) * TILE + threadIdx.x / C;
+  const int64_t n = L.n_sites_pad;
+  const size_t row = static_cast<size_t>(C * S);
+  const size_t n_wt = static_cast<size_t>(L.n_tiles) * PW;
+  const int wt = tile * PW + threadIdx.x / 64;
+  const double w = weights[site];
+  const double r = rates[part * C + c];
+  double ex[S], lr[S];
+#pragma unroll
+  for (int k = 0; k < S; ++k) {
+    lr[k] = ev[part * S + k] * r;
+    ex[k] = exp(lr[k] * brlen);
+  }
+  for (int j = 0; j < n_paths; ++j) {
+    const double *in = st + (static_cast<size_t>(j) * n + site) * row + c * S;
+    double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+      const double q = in[k] * ex[k];
+      t0 += q;
+      t1 += q * lr[k];
+      t2 += q * lr[k] * lr[k];
+    }
+    t0 = cat_sum<C>(t0);
+    t1 = cat_sum<C>(t1);
+    t2 = cat_sum<C>(t2);
+    double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+    if (c == 0 && w != 0.0) {
+      const double r1 = t1 / t0;
+      v0 = w * log(t0);
+      v1 = w * r1;
+      v2 = w * (t2 / t0 - r1 * r1);
+    }
+    v0 = wave_sum(v0);
+    v1 = wave_sum(v1);
+    v2 = wave_sum(v2);
+    if ((threadIdx.x & 63) == 0) {
+      partials[(static_cast<size_t>(j) * 3 + 0) * n_wt + wt] = v0;
+      partials[(static_cast<size_t>(j) * 3 + 1) * n_wt + wt] = v1;
+      partials[(static_cast<size_t>(j) * 3 + 2) * n_wt + wt] = v2;
+    }
+  }
+}
+
 template <typename F>
 void dispatch_sc(int S, int C, F &&f) {
   // instantiated shapes: DNA (4 states) and protein (20 states) with
@@ -974,6 +1131,47 @@ void launch_adjoint_persistent(Engine &e) {
                        e.d_weights.as<double>(), e.d_beta.as<double>(),
                        static_cast<int>(e.root_slots.size()), e.d_grad_partials.as<double>());
   });
+  NRX_HIP(hipGetLastError());
+}
+
+
+void launch_sumtables(Engine &e, int n_paths) {
+  if (n_paths == 0) return;
+  dispatch_sc(e.S, e.C, [&](auto s, auto c) {
+    constexpr int S = decltype(s)::value, C = decltype(c)::value;
+    using TT = typename TipType<S>::type;
+    if (e.opt.precision == NRX_FP32) {
+      hipLaunchKernelGGL((k_sumtables<S, C, float>), dim3(e.L.n_tiles), dim3(TILE * C), 0,
+                         e.stream, e.d_steps.as<PathStep>(), e.d_path_begin.as<int32_t>(), n_paths,
+                         e.d_ops_by_slot.as<DevOp>(), e.L, e.d_tile_part.as<int32_t>(),
+                         e.d_clv.as<float>(), e.d_scl.as<int32_t>(), 1, e.d_tips.as<TT>(),
+                         e.d_P.as<double>(), e.d_V.as<double>(), e.d_Vinv.as<double>(),
+                         e.d_freqs.as<double>(), e.d_st.as<double>());
+    } else {
+      hipLaunchKernelGGL((k_sumtables<S, C, double>), dim3(e.L.n_tiles), dim3(TILE * C), 0,
+                         e.stream, e.d_steps.as<PathStep>(), e.d_path_begin.as<int32_t>(), n_paths,
+                         e.d_ops_by_slot.as<DevOp>(), e.L, e.d_tile_part.as<int32_t>(),
+                         e.d_clv.as<double>(), e.d_scl.as<int32_t>(), 256, e.d_tips.as<TT>(),
+                         e.d_P.as<double>(), e.d_V.as<double>(), e.d_Vinv.as<double>(),
+                         e.d_freqs.as<double>(), e.d_st.as<double>());
+    }
+  });
+  NRX_HIP(hipGetLastError());
+}
+
+void launch_sumtable_eval(Engine &e, int n_paths, double brlen) {
+  if (n_paths == 0) return;
+  dispatch_sc(e.S, e.C, [&](auto s, auto c) {
+    constexpr int S = decltype(s)::value, C = decltype(c)::value;
+    hipLaunchKernelGGL((k_sumtable_eval<S, C>), dim3(e.L.n_tiles), dim3(TILE * C), 0, e.stream,
+                       e.d_st.as<double>(), n_paths, e.L, e.d_tile_part.as<int32_t>(),
+                       e.d_eval.as<double>(), e.d_rates.as<double>(), e.d_weights.as<double>(),
+                       brlen, e.d_st_partials.as<double>());
+  });
+  NRX_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_reduce_segments, dim3(n_paths * 3 * e.P), dim3(256), 0, e.stream,
+                     e.d_st_partials.as<double>(), static_cast<size_t>(e.L.n_tiles) * e.pw, e.pw,
+                     e.d_part_tile_begin.as<int32_t>(), e.P, e.d_st_sums.as<double>());
   NRX_HIP(hipGetLastError());
 }
 

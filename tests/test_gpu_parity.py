@@ -269,3 +269,93 @@ def test_fp32_deep_underflow_matches_closed_form():
     for eng in (engine, engine32):
         got = computeLoglikelihood(eng(net, [part]))
         assert abs(got - want) <= 1e-6 * abs(want), (got, want)
+
+
+# ------------------------------------------------------ branch derivatives
+# computePartitionSumtables + computeLoglikelihoodDerivatives
+# (LikelihoodDerivatives.cpp:30-344): reference normalisation (mode 0) and
+# the true derivative (mode 1), AVERAGE and BEST, vs the oracle.
+
+@pytest.mark.parametrize("name", ["small", "two_reticulations", "three_reticulations",
+                                  "reticulation_in_reticulation", "celine_smaller_1", "tree"])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_brlen_derivatives_match_oracle(name, prec):
+    say("brlen derivatives", name, prec)
+    from netrax_amd import Precision
+    net, parts = load_sample(name, fixed_model())
+    tol = 1e-7 if prec == "fp64" else 2e-4
+    for var in (0, 1):
+        ann = engine(net, parts, var, precision=Precision.FP64 if prec == "fp64" else Precision.FP32)
+        for e in range(net.n_edges):
+            for mode in (0, 1):
+                d = ann.brlen_derivatives(e, mode=mode)
+                o1, o2, op1, op2 = O.brlen_derivatives(net, parts, e, var, mode=mode)
+                assert abs(d.logl_prime - o1) <= tol * max(1.0, abs(o1)), (name, var, e, mode, d.logl_prime, o1)
+                assert abs(d.logl_prime_prime - o2) <= tol * max(1.0, abs(o2)), (name, var, e, mode, d.logl_prime_prime, o2)
+        ann.close()
+
+
+def test_brlen_derivatives_at_moved_length_and_reuse():
+    """Newton-Raphson evaluates the same sumtables at several lengths."""
+    w = workload_30taxa_10k()
+    ann = engine(w.network, w.partitions)
+    computeLoglikelihood(ann)
+    rng = np.random.default_rng(5)
+    for e in rng.choice(w.network.n_edges, 4, replace=False):
+        b0 = float(w.network.brlen[e])
+        for b in (b0, b0 * 0.5 + 1e-4, b0 * 2.0 + 0.01):
+            for mode in (0, 1):
+                d = ann.brlen_derivatives(int(e), brlen=b, mode=mode)
+                bl = w.network.brlen.copy()
+                bl[e] = b
+                o1, o2, op1, op2 = O.brlen_derivatives(w.network, w.partitions, int(e), 0, mode=mode,
+                                                      brlen=bl)
+                assert abs(d.logl_prime - o1) <= 1e-7 * max(1.0, abs(o1)), (e, b, mode)
+                assert abs(d.logl_prime_prime - o2) <= 1e-7 * max(1.0, abs(o2)), (e, b, mode)
+                assert np.allclose(d.partition_logl_prime, op1, rtol=1e-7, atol=1e-7)
+        # moving only this branch keeps the sumtables valid
+        ann.set_branch_length(int(e), b0 * 1.3)
+        d = ann.brlen_derivatives(int(e), mode=1)
+        bl = w.network.brlen.copy()
+        o1, _, _, _ = O.brlen_derivatives(w.network, w.partitions, int(e), 0, mode=1, brlen=bl)
+        assert abs(d.logl_prime - o1) <= 1e-7 * max(1.0, abs(o1))
+
+
+def test_true_derivative_matches_gradient_50taxa():
+    w = make_workload("50taxa_4retic_50k", 50, 4, [20000, 30000], seed=3)
+    ann = engine(w.network, w.partitions)
+    _, g, _ = ann.gradient()
+    for e in (0, 7, 33, w.network.n_edges - 1):
+        d = ann.brlen_derivatives(e, mode=1)
+        assert abs(d.logl_prime - g[e]) <= 1e-7 * max(1.0, abs(g[e])), (e, d.logl_prime, g[e])
+
+
+def test_branch_length_sweep_improves_and_matches_oracle_driven_sweep():
+    """optimize_branches (BranchLengthOptimization.cpp:422-477) on perturbed
+    lengths: lnL never decreases, and the lengths equal those of the same
+    sweep driven by the oracle's derivatives."""
+    from netrax_amd.optimization import newton_maximise, optimize_branches
+    net, parts = load_sample("two_reticulations", fixed_model())
+    rng = np.random.default_rng(2)
+    net.brlen = net.brlen * rng.uniform(0.3, 3.0, net.n_edges)
+    start = net.brlen.copy()
+    ann = engine(net, parts)
+    before = computeLoglikelihood(ann)
+    after = optimize_branches(ann)
+    assert after >= before - 1e-9
+    assert abs(after - O.loglikelihood(net, parts, brlen=net.brlen)[0]) <= RTOL * abs(after)
+    # the same sweep with oracle derivatives and oracle acceptance checks
+    b = start.copy()
+    for e in range(net.n_edges):
+        def deriv(x):
+            bb = b.copy()
+            bb[e] = x
+            d1, d2, _, _ = O.brlen_derivatives(net, parts, e, O.AVERAGE, mode=0, brlen=bb)
+            return d1, d2
+        old = O.loglikelihood(net, parts, brlen=b)[0]
+        x = newton_maximise(deriv, float(b[e]), 1e-6, 100.0, 1e-7, 32)
+        bb = b.copy()
+        bb[e] = x
+        if O.loglikelihood(net, parts, brlen=bb)[0] >= old:
+            b = bb
+    assert np.allclose(net.brlen, b, rtol=1e-6, atol=1e-9), (net.brlen, b)
