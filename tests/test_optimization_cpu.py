@@ -34,4 +34,5 @@ def test_newton_on_oracle_branch_derivatives():
             return d1, d2
         x = newton_maximise(deriv, float(net.brlen[e]), 1e-6, 100.0, 1e-7, 32)
         d1, _ = deriv(x)
-        assert abs(d1) < 1e-4 or x <= 1e-6 * 1.0001 or x >= 100.0 * 0.9999, (e, x, d1)
+        # converged (tolerance 1e-7) at a stationary point or at a bound
+        assert abs(d1) < 1e-4 or x <= 1e-6 + 2e-7 or x >= 100.0 - 2e-7, (e, x, d1)
