@@ -15,6 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libnetrax_gpu.so")
+EXAMPLE = os.path.join(BUILD_DIR, "mirror_example")
 
 SOURCES = ["host_model.cpp", "dag.cpp", "engine.cpp", "kernels.hip"]
 
@@ -52,6 +53,13 @@ def build(verbose: bool = False) -> str:
     if res.returncode != 0:
         raise RuntimeError(f"linking failed:\n{res.stdout}\n{res.stderr}")
     os.replace(tmp, LIB)
+    # C++ program over the header-only mirror of the reference interface
+    example = os.path.join(BUILD_DIR, "mirror_example")
+    cmd = ["g++", "-O2", "-std=c++17", os.path.join(CSRC, "mirror_example.cpp"), "-o", example,
+           "-L" + HERE, "-lnetrax_gpu", "-Wl,-rpath,$ORIGIN/.."]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"building mirror_example failed:\n{res.stdout}\n{res.stderr}")
     if verbose:
         print("built", LIB)
     return LIB

@@ -688,6 +688,7 @@ int nrx_create(const nrx_network_desc *net, const nrx_partition_desc *parts, int
     nd.retic_second_edge = e->retic_second.data();
     e->dag = compile_dag(nd);
 
+    if (n_parts > 64) throw std::invalid_argument("at most 64 partitions per engine");
     e->P = n_parts;
     e->S = parts[0].states;
     e->C = parts[0].rate_cats;

@@ -324,66 +324,82 @@ __global__ void __launch_bounds__(TILE *C)
 // a max-shifted FP64 log-sum-exp stands in for the reference's MPFR sum.
 // Also produces the adjoint seeds beta[p][root CLV] = sum of the class
 // weights d lnL_p / d lnL_{p,t}, and d lnL / d p_r.
-__global__ void k_mixture(DevLayout L, int n_root, int nc, int R, int variant,
-                          const DevClass *__restrict__ classes,
-                          const double *__restrict__ class_dlogprob,
-                          const double *__restrict__ root_sums, double *__restrict__ beta,
-                          double *__restrict__ res, ResultLayout rl) {
-  const int P = L.n_parts;
+__global__ void __launch_bounds__(256)
+    k_mixture(DevLayout L, int n_root, int nc, int R, int variant,
+              const DevClass *__restrict__ classes, const double *__restrict__ class_dlogprob,
+              const double *__restrict__ root_sums, double *__restrict__ beta,
+              double *__restrict__ res, ResultLayout rl) {
+  __shared__ double best[64], ssum[64];
+  __shared__ int arg[64];
+  const int P = L.n_parts;  // <= 64 (checked on the host)
+  double *alpha = res + rl.alpha;
+  // per-(root CLV, partition) tree lnL and per-class copies
   for (int idx = threadIdx.x; idx < n_root * P; idx += blockDim.x) {
     const int r = idx / P, p = idx % P;
     res[rl.root_logl + p * n_root + r] = root_sums[idx];
   }
-  __syncthreads();
+  for (int idx = threadIdx.x; idx < nc * P; idx += blockDim.x) {
+    const int p = idx / nc, k = idx % nc;
+    const int ri = classes[k].root_index;
+    res[rl.tree_logl + p * nc + k] = ri >= 0 ? root_sums[ri * P + p] : 0.0;
+  }
+  // best class per partition (first maximum in class order)
   for (int p = threadIdx.x; p < P; p += blockDim.x) {
-    const double *rll = res + rl.root_logl + p * n_root;
-    double *alpha = res + rl.alpha + p * nc;
-    double best = -INFINITY;
-    int arg = -1;
+    double bv = -INFINITY;
+    int ba = -1;
     for (int k = 0; k < nc; ++k) {
-      const double tl = classes[k].root_index >= 0 ? rll[classes[k].root_index] : 0.0;
-      res[rl.tree_logl + p * nc + k] = tl;
       if (!classes[k].use) continue;
-      const double v = classes[k].logprob + tl;
-      if (v > best) {
-        best = v;
-        arg = k;
+      const double v = classes[k].logprob + root_sums[classes[k].root_index * P + p];
+      if (v > bv) {
+        bv = v;
+        ba = k;
       }
     }
-    double lnl;
-    if (variant == 1) {
-      for (int k = 0; k < nc; ++k) alpha[k] = (k == arg) ? 1.0 : 0.0;
-      lnl = best;
-    } else {
-      double s = 0.0;
-      for (int k = 0; k < nc; ++k) {
-        const double a =
-            classes[k].use && best > -INFINITY
-                ? exp(classes[k].logprob + rll[classes[k].root_index] - best)
-                : 0.0;
-        alpha[k] = a;
-        s += a;
-      }
-      for (int k = 0; k < nc; ++k) alpha[k] = s > 0.0 ? alpha[k] / s : 0.0;
-      lnl = best + log(s);
-    }
-    res[rl.per_part + p] = lnl;
-    for (int r = 0; r < n_root; ++r) beta[p * n_root + r] = 0.0;
-    for (int k = 0; k < nc; ++k)
-      if (alpha[k] != 0.0) beta[p * n_root + classes[k].root_index] += alpha[k];
+    best[p] = bv;
+    arg[p] = ba;
   }
   __syncthreads();
+  for (int idx = threadIdx.x; idx < nc * P; idx += blockDim.x) {
+    const int p = idx / nc, k = idx % nc;
+    double a = 0.0;
+    if (variant == 1)
+      a = k == arg[p] ? 1.0 : 0.0;
+    else if (classes[k].use && best[p] > -INFINITY)
+      a = exp(classes[k].logprob + root_sums[classes[k].root_index * P + p] - best[p]);
+    alpha[p * nc + k] = a;
+  }
+  __syncthreads();
+  for (int p = threadIdx.x; p < P; p += blockDim.x) {
+    double sacc = 0.0;
+    for (int k = 0; k < nc; ++k) sacc += alpha[p * nc + k];
+    ssum[p] = sacc;
+    res[rl.per_part + p] = variant == 1 ? best[p] : best[p] + log(sacc);
+  }
+  __syncthreads();
+  if (variant != 1)
+    for (int idx = threadIdx.x; idx < nc * P; idx += blockDim.x) {
+      const int p = idx / nc;
+      alpha[idx] = ssum[p] > 0.0 ? alpha[idx] / ssum[p] : 0.0;
+    }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < n_root * P; idx += blockDim.x) {
+    const int p = idx / n_root, r = idx % n_root;
+    double b = 0.0;
+    for (int k = 0; k < nc; ++k)
+      if (classes[k].root_index == r) b += alpha[p * nc + k];
+    beta[p * n_root + r] = b;
+  }
+  for (int r = threadIdx.x; r < R; r += blockDim.x) {
+    double g = 0.0;
+    for (int p = 0; p < P; ++p)
+      for (int k = 0; k < nc; ++k)
+        g += alpha[p * nc + k] * class_dlogprob[static_cast<size_t>(k) * R + r];
+    res[rl.dprob + r] = g;
+  }
   if (threadIdx.x == 0) {
     double tot = 0.0;
     for (int p = 0; p < P; ++p) tot += res[rl.per_part + p];
     res[rl.total] = tot;
-    for (int r = 0; r < R; ++r) {
-      double g = 0.0;
-      for (int p = 0; p < P; ++p)
-        for (int k = 0; k < nc; ++k)
-          g += res[rl.alpha + p * nc + k] * class_dlogprob[static_cast<size_t>(k) * R + r];
-      res[rl.dprob + r] = g;
-    }
   }
 }
 

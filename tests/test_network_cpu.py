@@ -100,3 +100,18 @@ def test_abi_rejects_null_arguments_without_touching_the_device():
     opt = _lib.Options()
     lib.nrx_default_options(C.byref(opt))
     assert opt.precision == _lib.NRX_FP64 and opt.brlen_min == pytest.approx(1e-6)
+
+
+def test_cpp_mirror_reports_engine_errors():
+    """The C++ mirror (include/netrax_likelihood.hpp) turns a failing engine
+    call into std::runtime_error with the engine's message; without a GPU the
+    example program reports the HIP failure and exits 2."""
+    import os
+    import subprocess
+    from netrax_amd import build as nb
+    if not os.path.exists(nb.EXAMPLE):
+        nb.build()
+    env = dict(os.environ, HIP_VISIBLE_DEVICES="-1")
+    res = subprocess.run([nb.EXAMPLE], capture_output=True, text=True, env=env, timeout=120)
+    assert res.returncode == 2
+    assert "error:" in res.stderr
