@@ -174,6 +174,22 @@ int nrx_tree_logls(nrx_engine *eng, double *tree_logl, double *class_logprob);
 
 int nrx_get_stats(nrx_engine *eng, nrx_stats *out);
 
+/* Batches of networks (FP32 form).  Replaces the sequential candidate loop
+ * of a search step (src/search/Filtering.cpp:236-270 scores candidates one
+ * at a time): every engine of the batch is scored by one launch per kernel.
+ * All engines must use NRX_FP32, 4-state data and the same number of rate
+ * categories on the same device, and must outlive the batch. */
+typedef struct nrx_batch nrx_batch;
+
+int nrx_batch_create(nrx_engine *const *engines, int32_t n_engines, nrx_batch **out);
+int nrx_batch_destroy(nrx_batch *batch);
+/* lnl[i] for every engine; when dbrlen (and dprob) are non-NULL also the
+ * full gradient: dbrlen is the concatenation of each engine's n_edges
+ * values, dprob of its n_retic values, in engine order.  device_ms (may be
+ * NULL) receives the device time of the whole batch. */
+int nrx_batch_evaluate(nrx_batch *batch, int32_t variant, double *lnl, double *dbrlen,
+                       double *dprob, double *device_ms);
+
 /* Device timing of the last evaluation (zeros unless timing is enabled). */
 int nrx_last_timing(nrx_engine *eng, nrx_timing *out);
 

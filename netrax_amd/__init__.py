@@ -6,13 +6,13 @@ binding, the host-side mirror of the reference interface, the network
 model and the seeded input generators.
 """
 from .likelihood import (AnnotatedNetwork, LikelihoodVariant, LoglDerivatives, NetraxOptions,
-                         Precision, computeLoglikelihood, computeLoglikelihoodDerivatives,
+                         NetworkBatch, Precision, computeLoglikelihood, computeLoglikelihoodDerivatives,
                          computePartitionSumtables)
 from .network import Network, parse_newick, random_network
 from .simulate import Model, Partition, Workload
 
 __all__ = [
-    "AnnotatedNetwork", "LikelihoodVariant", "LoglDerivatives", "NetraxOptions", "Precision",
+    "AnnotatedNetwork", "NetworkBatch", "LikelihoodVariant", "LoglDerivatives", "NetraxOptions", "Precision",
     "computeLoglikelihood", "computeLoglikelihoodDerivatives", "computePartitionSumtables",
     "Network", "parse_newick", "random_network", "Model", "Partition", "Workload",
 ]

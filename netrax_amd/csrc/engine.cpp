@@ -101,6 +101,8 @@ static void compute_class_probs(Engine &e) {
   }
 }
 
+static void upload_netdesc(Engine &e);
+
 // Sets OP_HAS_CONSUMERS / OP_FIRST_KID* for a list of ops walked by the
 // adjoint in the order given by `adj_order` (indices into ops).
 static void set_adjoint_flags(std::vector<DevOp> &ops, const std::vector<int> &adj_order, int nv) {
@@ -323,6 +325,42 @@ static void build_schedule(Engine &e) {
   e.d_root_sums.alloc(sizeof(double) * std::max<size_t>(1, rs.size()) * e.P);
   e.d_use_sums.alloc(sizeof(double) * std::max(1, e.n_uses) * static_cast<size_t>(e.P));
   NRX_HIP(hipStreamSynchronize(e.stream));
+  upload_netdesc(e);
+  e.schedule_version++;
+}
+
+
+// Device descriptor of this engine for the persistent kernels (FP32 form).
+NetDesc make_netdesc(const Engine &e) {
+  NetDesc d{};
+  d.L = e.L;
+  d.ops = e.d_ops_dfs.as<DevOp>();
+  d.group_begin = e.d_group_begin.as<int32_t>();
+  d.n_groups = static_cast<int32_t>(e.group_begin.size()) - 1;
+  d.n_root = static_cast<int32_t>(e.root_slots.size());
+  d.clv = e.d_clv.as<float>();
+  d.scl = e.d_scl.as<int32_t>();
+  d.adj = e.d_adj.as<float>();
+  d.adj_exp = e.d_adj_exp.as<int32_t>();
+  d.tips = e.d_tips.as<uint8_t>();
+  d.Pf = e.d_Pf.as<float>();
+  d.Qf = e.d_Qf.as<float>();
+  d.tile_part = e.d_tile_part.as<int32_t>();
+  d.freqs = e.d_freqs.as<double>();
+  d.weights = e.d_weights.as<double>();
+  d.beta = e.d_beta.as<double>();
+  d.root_partials = e.d_root_partials.as<double>();
+  d.grad_partials = e.d_grad_partials.as<double>();
+  return d;
+}
+
+static void upload_netdesc(Engine &e) {
+  if (e.opt.precision != NRX_FP32) return;
+  std::vector<NetDesc> nd{make_netdesc(e)};
+  upload(e.d_netdesc, nd, e.stream);
+  std::vector<WorkItem> work(e.L.n_tiles);
+  for (int t = 0; t < e.L.n_tiles; ++t) work[t] = WorkItem{0, t};
+  upload(e.d_work, work, e.stream);
 }
 
 static void upload_classes_only(Engine &e) {
@@ -637,6 +675,224 @@ static void build_sumtables(Engine &e, int edge) {
   e.deriv_edge = edge;
   e.deriv_valid = true;
 }
+
+
+// ------------------------------------------------------------ batches
+
+struct Batch {
+  std::vector<Engine *> engines;
+  std::vector<int> versions;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  DevBuf d_nets, d_work, d_small, d_offsets, d_out;
+  double *h_out = nullptr;
+  std::vector<int64_t> offsets;
+  int n_work = 0, C = 0, max_pm = 0, max_root_rows = 0, max_use_rows = 0, max_edges = 0,
+      max_res = 0;
+  size_t lds_fwd = 0, lds_adj = 0;
+  ~Batch() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (h_out) (void)hipHostFree(h_out);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+static SmallDesc make_smalldesc(const Engine &e) {
+  SmallDesc d{};
+  d.L = e.L;
+  d.V = e.d_V.as<double>();
+  d.Vinv = e.d_Vinv.as<double>();
+  d.ev = e.d_eval.as<double>();
+  d.rates = e.d_rates.as<double>();
+  d.brlen = e.d_brlen.as<double>();
+  d.P = e.d_P.as<double>();
+  d.dP = e.d_dP.as<double>();
+  d.d2P = e.d_d2P.as<double>();
+  d.Pf = e.d_Pf.as<float>();
+  d.n_root = static_cast<int32_t>(e.root_slots.size());
+  d.nc = static_cast<int32_t>(e.dag.classes.size());
+  d.R = e.R;
+  d.n_uses = e.n_uses;
+  d.part_tile_begin = e.d_part_tile_begin.as<int32_t>();
+  d.classes = e.d_classes.as<DevClass>();
+  d.class_dlogprob = e.d_class_dlogprob.as<double>();
+  d.root_partials = e.d_root_partials.as<double>();
+  d.root_sums = e.d_root_sums.as<double>();
+  d.grad_partials = e.d_grad_partials.as<double>();
+  d.use_sums = e.d_use_sums.as<double>();
+  d.beta = e.d_beta.as<double>();
+  d.edge_use_begin = e.d_edge_use_begin.as<int32_t>();
+  d.edge_use_list = e.d_edge_use_list.as<int32_t>();
+  d.res = e.d_result.as<double>();
+  d.rl_total = e.rl.total;
+  d.rl_per_part = e.rl.per_part;
+  d.rl_root_logl = e.rl.root_logl;
+  d.rl_tree_logl = e.rl.tree_logl;
+  d.rl_alpha = e.rl.alpha;
+  d.rl_dbrlen = e.rl.dbrlen;
+  d.rl_part_dbrlen = e.rl.part_dbrlen;
+  d.rl_dprob = e.rl.dprob;
+  d.pw = e.pw;
+  return d;
+}
+
+// Descriptors and the (network, tile) work table; tiles of networks with
+// longer op lists come first so the longest workgroups start earliest.
+static void build_batch(Batch &b) {
+  const int n = static_cast<int>(b.engines.size());
+  std::vector<NetDesc> nets(n);
+  std::vector<SmallDesc> small(n);
+  std::vector<std::pair<int, int>> order;  // (cost, engine)
+  b.offsets.assign(n + 1, 0);
+  b.max_pm = b.max_root_rows = b.max_use_rows = b.max_edges = b.max_res = 0;
+  b.lds_fwd = b.lds_adj = 0;
+  b.versions.resize(n);
+  for (int i = 0; i < n; ++i) {
+    const Engine &e = *b.engines[i];
+    nets[i] = make_netdesc(e);
+    small[i] = make_smalldesc(e);
+    b.versions[i] = e.schedule_version;
+    order.push_back({static_cast<int>(e.ops_dfs.size()), i});
+    b.offsets[i + 1] = b.offsets[i] + e.rl.size;
+    b.max_pm = std::max(b.max_pm, e.P * e.E * e.C * e.S * e.S);
+    b.max_root_rows = std::max(b.max_root_rows, static_cast<int>(e.root_slots.size()) * e.P);
+    b.max_use_rows = std::max(b.max_use_rows, e.n_uses * e.P);
+    b.max_edges = std::max(b.max_edges, e.E);
+    b.max_res = std::max(b.max_res, e.rl.size);
+    b.lds_fwd = std::max(b.lds_fwd, sizeof(float) * e.E * e.C * 16);
+    b.lds_adj = std::max(b.lds_adj, sizeof(float) * (e.E * e.C * 16 + e.C * 16));
+  }
+  std::stable_sort(order.begin(), order.end(),
+                   [](const std::pair<int, int> &x, const std::pair<int, int> &y) {
+                     return x.first > y.first;
+                   });
+  std::vector<WorkItem> work;
+  for (const auto &o : order)
+    for (int t = 0; t < b.engines[o.second]->L.n_tiles; ++t) work.push_back({o.second, t});
+  b.n_work = static_cast<int>(work.size());
+  upload(b.d_nets, nets, b.stream);
+  upload(b.d_small, small, b.stream);
+  upload(b.d_work, work, b.stream);
+  upload(b.d_offsets, b.offsets, b.stream);
+  b.d_out.alloc(sizeof(double) * b.offsets[n]);
+  if (b.h_out) NRX_HIP(hipHostFree(b.h_out));
+  NRX_HIP(hipHostMalloc(reinterpret_cast<void **>(&b.h_out), sizeof(double) * b.offsets[n],
+                        hipHostMallocDefault));
+}
+
+}  // namespace nrx
+
+extern "C" {
+
+int nrx_batch_create(nrx_engine *const *engines, int32_t n_engines, nrx_batch **out) {
+  using namespace nrx;
+  return guarded([&]() -> int {
+    if (!engines || n_engines <= 0 || !out) throw std::invalid_argument("null argument");
+    *out = nullptr;
+    auto b = std::make_unique<Batch>();
+    for (int i = 0; i < n_engines; ++i) {
+      Engine *e = static_cast<Engine *>(engines[i]);
+      if (!e) throw std::invalid_argument("null engine in batch");
+      if (e->opt.precision != NRX_FP32 || e->S != 4)
+        throw std::invalid_argument("batches take FP32-form engines with 4-state data");
+      if (i > 0 && (e->C != b->engines[0]->C || e->opt.device != b->engines[0]->opt.device))
+        throw std::invalid_argument("batch engines must share rate categories and device");
+      b->engines.push_back(e);
+    }
+    b->C = b->engines[0]->C;
+    NRX_HIP(hipSetDevice(b->engines[0]->opt.device));
+    NRX_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    NRX_HIP(hipEventCreate(&b->ev0));
+    NRX_HIP(hipEventCreate(&b->ev1));
+    build_batch(*b);
+    *out = reinterpret_cast<nrx_batch *>(b.release());
+    return NRX_OK;
+  });
+}
+
+int nrx_batch_destroy(nrx_batch *batch) {
+  using namespace nrx;
+  return guarded([&]() -> int {
+    delete reinterpret_cast<Batch *>(batch);
+    return NRX_OK;
+  });
+}
+
+int nrx_batch_evaluate(nrx_batch *batch, int32_t variant, double *lnl, double *dbrlen,
+                       double *dprob, double *device_ms) {
+  using namespace nrx;
+  return guarded([&]() -> int {
+    if (!batch || !lnl) throw std::invalid_argument("null argument");
+    if (variant != NRX_AVERAGE_DISPLAYED_TREES && variant != NRX_BEST_DISPLAYED_TREE)
+      throw std::invalid_argument("unknown likelihood variant");
+    Batch &b = *reinterpret_cast<Batch *>(batch);
+    const int n = static_cast<int>(b.engines.size());
+    bool rebuild = false;
+    for (int i = 0; i < n; ++i) {
+      Engine &e = *b.engines[i];
+      check_ready(e);
+      // models and branch lengths of every engine, as update_pmatrices does
+      if (e.model_dirty) upload_models(e);
+      upload(e.d_brlen, e.brlen, e.stream);
+      e.model_dirty = false;
+      std::fill(e.edge_dirty.begin(), e.edge_dirty.end(), 0);
+      rebuild |= b.versions[i] != e.schedule_version;
+    }
+    if (rebuild) build_batch(b);
+    const bool grad = dbrlen != nullptr;
+    NRX_HIP(hipEventRecord(b.ev0, b.stream));
+    const SmallDesc *sd = b.d_small.as<SmallDesc>();
+    launch_pmatrices_multi(b.stream, sd, n, b.max_pm);
+    launch_forward_persistent_multi(b.stream, b.C, b.d_nets.as<NetDesc>(), b.d_work.as<WorkItem>(),
+                                    b.n_work, b.lds_fwd);
+    launch_root_reduce_multi(b.stream, sd, n, b.max_root_rows);
+    launch_mixture_multi(b.stream, sd, n, variant);
+    if (grad) {
+      launch_adjoint_persistent_multi(b.stream, b.C, b.d_nets.as<NetDesc>(),
+                                      b.d_work.as<WorkItem>(), b.n_work, b.lds_adj);
+      launch_grad_reduce_multi(b.stream, sd, n, b.max_use_rows, b.max_edges);
+    }
+    launch_gather_results(b.stream, sd, n, b.d_offsets.as<int64_t>(), b.d_out.as<double>(),
+                          b.max_res);
+    NRX_HIP(hipEventRecord(b.ev1, b.stream));
+    NRX_HIP(hipMemcpyAsync(b.h_out, b.d_out.p, sizeof(double) * b.offsets[n],
+                           hipMemcpyDeviceToHost, b.stream));
+    NRX_HIP(hipStreamSynchronize(b.stream));
+    if (device_ms) {
+      float ms = 0.f;
+      NRX_HIP(hipEventElapsedTime(&ms, b.ev0, b.ev1));
+      *device_ms = ms;
+    }
+    int64_t eo = 0, ro = 0;
+    int bad = -1;
+    for (int i = 0; i < n; ++i) {
+      Engine &e = *b.engines[i];
+      const double *r = b.h_out + b.offsets[i];
+      lnl[i] = r[e.rl.total];
+      if (!(lnl[i] > -INFINITY) && bad < 0) bad = i;
+      if (grad) {
+        std::copy(r + e.rl.dbrlen, r + e.rl.dbrlen + e.E, dbrlen + eo);
+        if (dprob && e.R > 0) std::copy(r + e.rl.dprob, r + e.rl.dprob + e.R, dprob + ro);
+      }
+      eo += e.E;
+      ro += e.R;
+      // CLVs are now valid; the engine's own cached result is not
+      std::fill(e.clv_dirty.begin(), e.clv_dirty.end(), 0);
+      e.cache_valid = false;
+      e.deriv_valid = false;
+    }
+    if (bad >= 0)
+      return fail(NRX_ERR_NEGINF, "Invalid network likelihood: negative infinity (engine " +
+                                      std::to_string(bad) + ")");
+    return NRX_OK;
+  });
+}
+
+}  // extern "C"
+
+namespace nrx {
 
 }  // namespace nrx
 

@@ -106,7 +106,68 @@ struct DevLayout {
   int32_t C;
 };
 
+// Device view of one FP32-form engine, read by the persistent kernels; a
+// launch walks a table of (network, tile) work items, so one launch can
+// score one network or a batch of them.
+struct NetDesc {
+  DevLayout L;
+  const DevOp *ops;
+  const int32_t *group_begin;
+  int32_t n_groups;
+  int32_t n_root;
+  float *clv;
+  int32_t *scl;
+  float *adj;
+  int32_t *adj_exp;
+  const uint8_t *tips;
+  const float *Pf;
+  const float *Qf;
+  const int32_t *tile_part;
+  const double *freqs;
+  const double *weights;
+  const double *beta;
+  double *root_partials;
+  double *grad_partials;
+};
+
+struct WorkItem {
+  int32_t net;
+  int32_t tile;
+};
+
+// Per-network parameters of the small kernels when a batch is scored.
+struct SmallDesc {
+  DevLayout L;
+  // P matrices
+  const double *V, *Vinv, *ev, *rates, *brlen;
+  double *P, *dP, *d2P;
+  float *Pf;
+  // reductions and mixture
+  int32_t n_root, nc, R, n_uses;
+  const int32_t *part_tile_begin;
+  const DevClass *classes;
+  const double *class_dlogprob;
+  double *root_partials, *root_sums, *grad_partials, *use_sums, *beta;
+  const int32_t *edge_use_begin, *edge_use_list;
+  double *res;
+  int32_t rl_total, rl_per_part, rl_root_logl, rl_tree_logl, rl_alpha, rl_dbrlen,
+      rl_part_dbrlen, rl_dprob;
+  int32_t pw, pad;
+};
+
 struct Engine;
+
+// Batch launchers (kernels.hip): one launch per kernel for n networks.
+void launch_pmatrices_multi(hipStream_t s, const SmallDesc *d, int n, int max_elems);
+void launch_root_reduce_multi(hipStream_t s, const SmallDesc *d, int n, int max_rows);
+void launch_mixture_multi(hipStream_t s, const SmallDesc *d, int n, int variant);
+void launch_grad_reduce_multi(hipStream_t s, const SmallDesc *d, int n, int max_rows, int max_edges);
+void launch_gather_results(hipStream_t s, const SmallDesc *d, int n, const int64_t *offsets,
+                           double *out, int max_size);
+void launch_forward_persistent_multi(hipStream_t stream, int C, const NetDesc *nets,
+                                     const WorkItem *work, int n_work, size_t lds);
+void launch_adjoint_persistent_multi(hipStream_t stream, int C, const NetDesc *nets,
+                                     const WorkItem *work, int n_work, size_t lds);
 
 // Kernel launchers (kernels.hip)
 void launch_pmatrices(Engine &e);

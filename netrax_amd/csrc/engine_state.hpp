@@ -80,6 +80,7 @@ struct Engine : nrx_engine {
   std::vector<int32_t> edge_use_begin, edge_use_list;  // gradient rows per edge
   int n_uses = 0;
   int pw = 1;  // site-sum partials per tile (one per wave)
+  int schedule_version = 0;  // bumped by every schedule rebuild
 
   // branch derivatives: sumtables of one branch for every displayed tree
   // containing it (valid while nothing but that branch's length changes)
@@ -110,6 +111,7 @@ struct Engine : nrx_engine {
   DevBuf d_clv, d_scl, d_adj, d_adj_exp, d_Qf;
   DevBuf d_ops, d_ops_dirty, d_chunk_edges, d_Qd;
   DevBuf d_ops_dfs, d_group_begin, d_edge_use_begin, d_edge_use_list;
+  DevBuf d_netdesc, d_work;  // FP32 form: this engine's NetDesc and (0, tile) work table
   DevBuf d_classes, d_class_dlogprob, d_root_slots;
   DevBuf d_root_partials, d_beta, d_grad_partials, d_root_sums, d_use_sums;
   DevBuf d_result;

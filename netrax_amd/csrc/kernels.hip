@@ -324,11 +324,11 @@ __global__ void __launch_bounds__(TILE *C)
 // a max-shifted FP64 log-sum-exp stands in for the reference's MPFR sum.
 // Also produces the adjoint seeds beta[p][root CLV] = sum of the class
 // weights d lnL_p / d lnL_{p,t}, and d lnL / d p_r.
-__global__ void __launch_bounds__(256)
-    k_mixture(DevLayout L, int n_root, int nc, int R, int variant,
-              const DevClass *__restrict__ classes, const double *__restrict__ class_dlogprob,
-              const double *__restrict__ root_sums, double *__restrict__ beta,
-              double *__restrict__ res, ResultLayout rl) {
+__device__ void mixture_body(const DevLayout &L, int n_root, int nc, int R, int variant,
+                             const DevClass *__restrict__ classes,
+                             const double *__restrict__ class_dlogprob,
+                             const double *__restrict__ root_sums, double *__restrict__ beta,
+                             double *__restrict__ res, const ResultLayout &rl) {
   __shared__ double best[64], ssum[64];
   __shared__ int arg[64];
   const int P = L.n_parts;  // <= 64 (checked on the host)
@@ -401,6 +401,15 @@ __global__ void __launch_bounds__(256)
     for (int p = 0; p < P; ++p) tot += res[rl.per_part + p];
     res[rl.total] = tot;
   }
+}
+
+
+__global__ void __launch_bounds__(256)
+    k_mixture(DevLayout L, int n_root, int nc, int R, int variant,
+              const DevClass *__restrict__ classes, const double *__restrict__ class_dlogprob,
+              const double *__restrict__ root_sums, double *__restrict__ beta,
+              double *__restrict__ res, ResultLayout rl) {
+  mixture_body(L, n_root, nc, R, variant, classes, class_dlogprob, root_sums, beta, res, rl);
 }
 
 // ------------------------------------------------------- adjoint (FP64)
@@ -598,15 +607,24 @@ __device__ __forceinline__ void mv4t(const float *__restrict__ M, const float (&
 
 template <int C, int G>
 __global__ void __launch_bounds__(TILE *C)
-    k_forward_persistent(const DevOp *__restrict__ ops, const int32_t *__restrict__ group_begin,
-                         int n_groups, DevLayout L, float *__restrict__ clv,
-                         int32_t *__restrict__ scl, const uint8_t *__restrict__ tips,
-                         const float *__restrict__ Pf, const int32_t *__restrict__ tile_part,
-                         const double *__restrict__ freqs, const double *__restrict__ weights,
-                         double *__restrict__ root_partials) {
+    k_forward_persistent(const NetDesc *__restrict__ nets, const WorkItem *__restrict__ work) {
   float *sP = dyn_lds<float>();  // P of this tile's partition: [edge][cat][4][4]
   constexpr int PW = TILE * C / 64;
-  const int tile = blockIdx.x;
+  const WorkItem wi = work[blockIdx.x];
+  const NetDesc &nd = nets[wi.net];
+  const DevOp *__restrict__ ops = nd.ops;
+  const int32_t *__restrict__ group_begin = nd.group_begin;
+  const int n_groups = nd.n_groups;
+  const DevLayout L = nd.L;
+  float *__restrict__ clv = nd.clv;
+  int32_t *__restrict__ scl = nd.scl;
+  const uint8_t *__restrict__ tips = nd.tips;
+  const float *__restrict__ Pf = nd.Pf;
+  const int32_t *__restrict__ tile_part = nd.tile_part;
+  const double *__restrict__ freqs = nd.freqs;
+  const double *__restrict__ weights = nd.weights;
+  double *__restrict__ root_partials = nd.root_partials;
+  const int tile = wi.tile;
   const int part = tile_part[tile];
   const int np = L.n_edges * C * 16;
   const float *gP = Pf + static_cast<size_t>(part) * np;
@@ -721,17 +739,29 @@ __device__ __forceinline__ void deliver_adjoint(float *__restrict__ dst_m, int32
 
 template <int C, int G>
 __global__ void __launch_bounds__(TILE *C)
-    k_adjoint_persistent(const DevOp *__restrict__ ops, const int32_t *__restrict__ group_begin,
-                         int n_groups, DevLayout L, const float *__restrict__ clv,
-                         const int32_t *__restrict__ scl, float *__restrict__ adj,
-                         int32_t *__restrict__ adj_exp, const uint8_t *__restrict__ tips,
-                         const float *__restrict__ Pf, const float *__restrict__ Qf,
-                         const int32_t *__restrict__ tile_part, const double *__restrict__ freqs,
-                         const double *__restrict__ weights, const double *__restrict__ beta,
-                         int n_root, double *__restrict__ grad_partials) {
+    k_adjoint_persistent(const NetDesc *__restrict__ nets, const WorkItem *__restrict__ work) {
   float *sP = dyn_lds<float>();  // [edge][cat][16] then Q: [cat][16]
   constexpr int PW = TILE * C / 64;
-  const int tile = blockIdx.x;
+  const WorkItem wi = work[blockIdx.x];
+  const NetDesc &nd = nets[wi.net];
+  const DevOp *__restrict__ ops = nd.ops;
+  const int32_t *__restrict__ group_begin = nd.group_begin;
+  const int n_groups = nd.n_groups;
+  const DevLayout L = nd.L;
+  const float *__restrict__ clv = nd.clv;
+  const int32_t *__restrict__ scl = nd.scl;
+  float *__restrict__ adj = nd.adj;
+  int32_t *__restrict__ adj_exp = nd.adj_exp;
+  const uint8_t *__restrict__ tips = nd.tips;
+  const float *__restrict__ Pf = nd.Pf;
+  const float *__restrict__ Qf = nd.Qf;
+  const int32_t *__restrict__ tile_part = nd.tile_part;
+  const double *__restrict__ freqs = nd.freqs;
+  const double *__restrict__ weights = nd.weights;
+  const double *__restrict__ beta = nd.beta;
+  const int n_root = nd.n_root;
+  double *__restrict__ grad_partials = nd.grad_partials;
+  const int tile = wi.tile;
   const int part = tile_part[tile];
   const int np = L.n_edges * C * 16;
   const float *gP = Pf + static_cast<size_t>(part) * np;
@@ -1002,6 +1032,108 @@ __global__ void __launch_bounds__(TILE *C)
   }
 }
 
+
+// ============================================= batches of networks (FP32)
+// blockIdx.y selects the network; every kernel reads that network's
+// SmallDesc and does exactly what the single-network kernel does.
+__global__ void k_pmatrices_multi(const SmallDesc *__restrict__ descs) {
+  const SmallDesc &d = descs[blockIdx.y];
+  const DevLayout L = d.L;
+  const int S = L.S, C = L.C;
+  const int64_t total = static_cast<int64_t>(L.n_parts) * L.n_edges * C * S * S;
+  const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (idx >= total) return;
+  const int j = idx % S;
+  const int i = (idx / S) % S;
+  const int c = (idx / (S * S)) % C;
+  const int e = (idx / (S * S * C)) % L.n_edges;
+  const int p = idx / (static_cast<int64_t>(S) * S * C * L.n_edges);
+  const double rt = d.rates[p * C + c];
+  const double t = d.brlen[e];
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  for (int k = 0; k < S; ++k) {
+    const double lr = d.ev[p * S + k] * rt;
+    const double x = d.V[(p * S + i) * S + k] * exp(lr * t) * d.Vinv[(p * S + k) * S + j];
+    a0 += x;
+    a1 += x * lr;
+    a2 += x * lr * lr;
+  }
+  d.P[idx] = a0;
+  d.dP[idx] = a1;
+  d.d2P[idx] = a2;
+  d.Pf[idx] = static_cast<float>(a0);
+}
+
+__device__ void reduce_segments_body(const double *__restrict__ in, size_t row_stride, int pw,
+                                     const int32_t *__restrict__ ptb, int P, int row, int p,
+                                     double *__restrict__ out) {
+  __shared__ double sh[4];
+  const double *r = in + static_cast<size_t>(row) * row_stride;
+  const int t0 = ptb[p] * pw, t1 = ptb[p + 1] * pw;
+  double s = 0.0;
+  for (int t = t0 + threadIdx.x; t < t1; t += 256) s += r[t];
+  s = block_sum<256>(s, sh);
+  if (threadIdx.x == 0) out[static_cast<size_t>(row) * P + p] = s;
+}
+
+__global__ void __launch_bounds__(256) k_root_reduce_multi(const SmallDesc *__restrict__ descs) {
+  const SmallDesc &d = descs[blockIdx.y];
+  const int P = d.L.n_parts;
+  if (static_cast<int>(blockIdx.x) >= d.n_root * P) return;
+  reduce_segments_body(d.root_partials, static_cast<size_t>(d.L.n_tiles) * d.pw, d.pw,
+                       d.part_tile_begin, P, blockIdx.x / P, blockIdx.x % P, d.root_sums);
+}
+
+__global__ void __launch_bounds__(256) k_use_reduce_multi(const SmallDesc *__restrict__ descs) {
+  const SmallDesc &d = descs[blockIdx.y];
+  const int P = d.L.n_parts;
+  if (static_cast<int>(blockIdx.x) >= d.n_uses * P) return;
+  reduce_segments_body(d.grad_partials, static_cast<size_t>(d.L.n_tiles) * d.pw, d.pw,
+                       d.part_tile_begin, P, blockIdx.x / P, blockIdx.x % P, d.use_sums);
+}
+
+__global__ void __launch_bounds__(256) k_mixture_multi(const SmallDesc *__restrict__ descs,
+                                                       int variant) {
+  const SmallDesc &d = descs[blockIdx.x];
+  ResultLayout rl{};
+  rl.total = d.rl_total;
+  rl.per_part = d.rl_per_part;
+  rl.root_logl = d.rl_root_logl;
+  rl.tree_logl = d.rl_tree_logl;
+  rl.alpha = d.rl_alpha;
+  rl.dbrlen = d.rl_dbrlen;
+  rl.part_dbrlen = d.rl_part_dbrlen;
+  rl.dprob = d.rl_dprob;
+  mixture_body(d.L, d.n_root, d.nc, d.R, variant, d.classes, d.class_dlogprob, d.root_sums,
+               d.beta, d.res, rl);
+}
+
+__global__ void k_grad_total_multi(const SmallDesc *__restrict__ descs) {
+  const SmallDesc &d = descs[blockIdx.y];
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= d.L.n_edges) return;
+  const int P = d.L.n_parts;
+  double tot = 0.0;
+  for (int p = 0; p < P; ++p) {
+    double s = 0.0;
+    for (int i = d.edge_use_begin[e]; i < d.edge_use_begin[e + 1]; ++i)
+      s += d.use_sums[static_cast<size_t>(d.edge_use_list[i]) * P + p];
+    d.res[d.rl_part_dbrlen + p * d.L.n_edges + e] = s;
+    tot += s;
+  }
+  d.res[d.rl_dbrlen + e] = tot;
+}
+
+// Copies every network's result block next to each other for one D2H copy.
+__global__ void k_gather_results(const SmallDesc *__restrict__ descs,
+                                 const int64_t *__restrict__ offsets, double *__restrict__ out) {
+  const SmallDesc &d = descs[blockIdx.y];
+  const int64_t n = offsets[blockIdx.y + 1] - offsets[blockIdx.y];
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    out[offsets[blockIdx.y] + i] = d.res[i];
+}
+
 template <typename F>
 void dispatch_sc(int S, int C, F &&f) {
   // instantiated shapes: DNA (4 states) and protein (20 states) with
@@ -1121,35 +1253,36 @@ void dispatch_c(int C, F &&f) {
 }
 
 void launch_forward_persistent(Engine &e) {
-  const size_t lds = sizeof(float) * e.E * e.C * 16;
-  dispatch_c(e.C, [&](auto c) {
-    constexpr int C = decltype(c)::value;
-    hipLaunchKernelGGL((k_forward_persistent<C, 4>), dim3(e.L.n_tiles), dim3(TILE * C), lds,
-                       e.stream, e.d_ops_dfs.as<DevOp>(), e.d_group_begin.as<int32_t>(),
-                       static_cast<int>(e.group_begin.size()) - 1, e.L,
-                       e.d_clv.as<float>(), e.d_scl.as<int32_t>(), e.d_tips.as<uint8_t>(),
-                       e.d_Pf.as<float>(), e.d_tile_part.as<int32_t>(), e.d_freqs.as<double>(),
-                       e.d_weights.as<double>(), e.d_root_partials.as<double>());
-  });
-  NRX_HIP(hipGetLastError());
+  launch_forward_persistent_multi(e.stream, e.C, e.d_netdesc.as<NetDesc>(),
+                                  e.d_work.as<WorkItem>(), e.L.n_tiles,
+                                  sizeof(float) * e.E * e.C * 16);
 }
 
 void launch_adjoint_persistent(Engine &e) {
-  const size_t lds = sizeof(float) * (e.E * e.C * 16 + e.C * 16);
-  dispatch_c(e.C, [&](auto c) {
-    constexpr int C = decltype(c)::value;
-    hipLaunchKernelGGL((k_adjoint_persistent<C, 4>), dim3(e.L.n_tiles), dim3(TILE * C), lds,
-                       e.stream, e.d_ops_dfs.as<DevOp>(), e.d_group_begin.as<int32_t>(),
-                       static_cast<int>(e.group_begin.size()) - 1, e.L, e.d_clv.as<float>(),
-                       e.d_scl.as<int32_t>(), e.d_adj.as<float>(), e.d_adj_exp.as<int32_t>(),
-                       e.d_tips.as<uint8_t>(), e.d_Pf.as<float>(), e.d_Qf.as<float>(),
-                       e.d_tile_part.as<int32_t>(), e.d_freqs.as<double>(),
-                       e.d_weights.as<double>(), e.d_beta.as<double>(),
-                       static_cast<int>(e.root_slots.size()), e.d_grad_partials.as<double>());
+  launch_adjoint_persistent_multi(e.stream, e.C, e.d_netdesc.as<NetDesc>(),
+                                  e.d_work.as<WorkItem>(), e.L.n_tiles,
+                                  sizeof(float) * (e.E * e.C * 16 + e.C * 16));
+}
+
+void launch_forward_persistent_multi(hipStream_t stream, int C, const NetDesc *nets,
+                                     const WorkItem *work, int n_work, size_t lds) {
+  dispatch_c(C, [&](auto c) {
+    constexpr int CC = decltype(c)::value;
+    hipLaunchKernelGGL((k_forward_persistent<CC, 4>), dim3(n_work), dim3(TILE * CC), lds, stream,
+                       nets, work);
   });
   NRX_HIP(hipGetLastError());
 }
 
+void launch_adjoint_persistent_multi(hipStream_t stream, int C, const NetDesc *nets,
+                                     const WorkItem *work, int n_work, size_t lds) {
+  dispatch_c(C, [&](auto c) {
+    constexpr int CC = decltype(c)::value;
+    hipLaunchKernelGGL((k_adjoint_persistent<CC, 4>), dim3(n_work), dim3(TILE * CC), lds, stream,
+                       nets, work);
+  });
+  NRX_HIP(hipGetLastError());
+}
 
 void launch_sumtables(Engine &e, int n_paths) {
   if (n_paths == 0) return;
@@ -1188,6 +1321,37 @@ void launch_sumtable_eval(Engine &e, int n_paths, double brlen) {
   hipLaunchKernelGGL(k_reduce_segments, dim3(n_paths * 3 * e.P), dim3(256), 0, e.stream,
                      e.d_st_partials.as<double>(), static_cast<size_t>(e.L.n_tiles) * e.pw, e.pw,
                      e.d_part_tile_begin.as<int32_t>(), e.P, e.d_st_sums.as<double>());
+  NRX_HIP(hipGetLastError());
+}
+
+
+void launch_pmatrices_multi(hipStream_t s, const SmallDesc *d, int n, int max_elems) {
+  hipLaunchKernelGGL(k_pmatrices_multi, dim3((max_elems + 255) / 256, n), dim3(256), 0, s, d);
+  NRX_HIP(hipGetLastError());
+}
+
+void launch_root_reduce_multi(hipStream_t s, const SmallDesc *d, int n, int max_rows) {
+  hipLaunchKernelGGL(k_root_reduce_multi, dim3(max_rows, n), dim3(256), 0, s, d);
+  NRX_HIP(hipGetLastError());
+}
+
+void launch_mixture_multi(hipStream_t s, const SmallDesc *d, int n, int variant) {
+  hipLaunchKernelGGL(k_mixture_multi, dim3(n), dim3(256), 0, s, d, variant);
+  NRX_HIP(hipGetLastError());
+}
+
+void launch_grad_reduce_multi(hipStream_t s, const SmallDesc *d, int n, int max_rows,
+                              int max_edges) {
+  hipLaunchKernelGGL(k_use_reduce_multi, dim3(max_rows, n), dim3(256), 0, s, d);
+  NRX_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_grad_total_multi, dim3((max_edges + 255) / 256, n), dim3(256), 0, s, d);
+  NRX_HIP(hipGetLastError());
+}
+
+void launch_gather_results(hipStream_t s, const SmallDesc *d, int n, const int64_t *offsets,
+                           double *out, int max_size) {
+  hipLaunchKernelGGL(k_gather_results, dim3((max_size + 255) / 256, n), dim3(256), 0, s, d,
+                     offsets, out);
   NRX_HIP(hipGetLastError());
 }
 

@@ -257,3 +257,54 @@ def computeLoglikelihoodDerivatives(ann_network: AnnotatedNetwork, sumtables: Su
     if sumtables.pmatrix_index != pmatrix_index:
         raise ValueError("sumtables belong to another branch")
     return ann_network.brlen_derivatives(pmatrix_index, brlen, mode)
+
+
+class NetworkBatch:
+    """Scores several FP32-form networks with one launch per kernel
+    (the candidate loop of a search step, src/search/Filtering.cpp:236-270,
+    scores them one at a time)."""
+
+    def __init__(self, networks: Sequence[AnnotatedNetwork]):
+        self.lib = _lib.load()
+        self.networks = list(networks)
+        arr = (C.c_void_p * len(self.networks))(*[a.handle.value for a in self.networks])
+        h = C.c_void_p()
+        _lib.check(self.lib.nrx_batch_create(arr, len(self.networks), C.byref(h)))
+        self.handle = h
+
+    def evaluate(self, variant: int = 0, gradient: bool = True):
+        """(lnL [n], per-network dlnL/dbrlen list, per-network dlnL/dprob
+        list, device ms); the lists are empty when gradient is False."""
+        n = len(self.networks)
+        lnl = np.zeros(n)
+        E = [a.network.n_edges for a in self.networks]
+        R = [a.network.n_retic for a in self.networks]
+        db = np.zeros(max(1, sum(E)))
+        dp = np.zeros(max(1, sum(R)))
+        ms = C.c_double()
+        rc = self.lib.nrx_batch_evaluate(self.handle, int(variant), _dptr(lnl),
+                                         _dptr(db) if gradient else None,
+                                         _dptr(dp) if gradient else None, C.byref(ms))
+        if rc == _lib.NRX_ERR_NEGINF:
+            raise RuntimeError(self.lib.nrx_last_error().decode())
+        _lib.check(rc)
+        dbs, dps = [], []
+        if gradient:
+            eo = ro = 0
+            for e, r in zip(E, R):
+                dbs.append(db[eo:eo + e].copy())
+                dps.append(dp[ro:ro + r].copy())
+                eo += e
+                ro += r
+        return lnl, dbs, dps, ms.value
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            self.lib.nrx_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -473,6 +473,12 @@ def random_network(n_taxa: int, n_retic: int, rng: np.random.Generator,
         a.kids, a.lens = [v1], [l1 * (1 - f1)]
         u2.kids[j2], u2.lens[j2] = b, l2 * f2
         b.kids, b.lens = [v2], [l2 * (1 - f2)]
+        # a reticulation whose parent edge was split now hangs below the new node
+        for new_parent, old_parent, child in ((a, u1, v1), (b, u2, v2)):
+            if child.retic is not None:
+                f, s_, p = child.retic
+                child.retic = (new_parent if f is old_parent else f,
+                               new_parent if s_ is old_parent else s_, p)
         a.kids.append(b)
         a.lens.append(rng.exponential(mean_brlen))
         b.retic = (u2, a, float(rng.uniform(*prob_range)))

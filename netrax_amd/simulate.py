@@ -218,3 +218,30 @@ def workload_50taxa_50k(seed: int = 3) -> Workload:
 def workload_protein_20k(seed: int = 4) -> Workload:
     """40 taxa, 2 reticulations, 20,000 amino-acid sites, one partition."""
     return make_workload("40taxa_2retic_20k_aa", 40, 2, [20000], seed, protein=True)
+
+
+def candidate_batch(n_bases: int = 8, per_base: int = 32, seed: int = 5,
+                    taxa=(20, 60), retic=(0, 5), sites=(2000, 50000)):
+    """Candidate networks of search steps: ``n_bases`` seeded base networks
+    (taxa uniform in ``taxa``, reticulations uniform in ``retic``, sites
+    log-uniform in ``sites``, GTR+G4, alignment simulated on the base), each
+    followed by ``per_base`` candidates from one seeded move (arc insertion,
+    arc removal or rSPR).  Returns a list of (Network, partitions)."""
+    from .moves import random_move
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n_bases):
+        nt = int(rng.integers(taxa[0], taxa[1] + 1))
+        nr = int(rng.integers(retic[0], retic[1] + 1))
+        ns = int(round(float(np.exp(rng.uniform(np.log(sites[0]), np.log(sites[1]))))))
+        base = random_network(nt, nr, rng)
+        model = random_dna_model(rng)
+        masks, weights = compress_patterns(states_to_masks(simulate_states(base, model, ns, rng)))
+        parts = [Partition(model, masks, weights)]
+        for _ in range(per_base):
+            cand = random_move(base, rng, max_retic=retic[1])
+            # the alignment rows follow the candidate's tip order
+            order = [base.tip_labels.index(l) for l in cand.tip_labels]
+            out.append((cand, [Partition(model, np.ascontiguousarray(masks[order]), weights)]))
+    return out

@@ -378,3 +378,58 @@ def test_cpp_mirror_example_matches_oracle():
         assert abs(lnl - want) <= RTOL * abs(want)
         o1, o2, _, _ = O.brlen_derivatives(net, parts, 1, var, mode=0)
         assert abs(d1 - o1) <= 1e-7 * max(1, abs(o1)) and abs(d2 - o2) <= 1e-7 * max(1, abs(o2))
+
+
+# ------------------------------------------------------------- protein data
+def test_protein_fp64_matches_oracle():
+    """20-state reversible model (Gamma(1,1) exchangeabilities, Dirichlet
+    frequencies), Gamma4: 40 taxa / 2 reticulations on a 1,500-site sample
+    of the protein workload's shape."""
+    say("protein 40 taxa / 2 reticulations")
+    w = make_workload("40taxa_2retic_aa", 40, 2, [1500], seed=4, protein=True)
+    for var in (0, 1):
+        ann = engine(w.network, w.partitions, var)
+        got = computeLoglikelihood(ann)
+        want = O.loglikelihood(w.network, w.partitions, var)[0]
+        assert abs(got - want) <= RTOL * abs(want), (var, got, want)
+        lnl, g, gp = ann.gradient()
+        _, og, ogp = O.gradient(w.network, w.partitions, var)
+        scale = max(1.0, np.abs(og).max())
+        assert np.allclose(g, og, rtol=1e-7, atol=1e-9 * scale)
+        assert np.allclose(gp, ogp, rtol=1e-7, atol=1e-9 * scale)
+        for e in (0, 5, w.network.n_edges - 1):
+            d = ann.brlen_derivatives(e, mode=0)
+            o1, o2, _, _ = O.brlen_derivatives(w.network, w.partitions, e, var, mode=0)
+            assert abs(d.logl_prime - o1) <= 1e-7 * max(1.0, abs(o1))
+        ann.close()
+
+
+# ---------------------------------------------------------- network batches
+def test_network_batch_matches_single_engines_and_oracle():
+    """One launch per kernel for a batch of candidate networks (different
+    sizes, reticulation counts and sites) == each network scored alone."""
+    from netrax_amd import NetworkBatch, Precision
+    from netrax_amd.simulate import candidate_batch
+    cands = candidate_batch(3, 3, seed=6, sites=(300, 3000))
+    anns = [engine(n, p, 0, precision=Precision.FP32) for n, p in cands]
+    batch = NetworkBatch(anns)
+    for var in (0, 1):
+        lnl, dbs, dps, ms = batch.evaluate(var)
+        assert ms > 0
+        for i, (n, p) in enumerate(cands):
+            single = engine(n, p, var, precision=Precision.FP32)
+            l1, g1, p1 = single.gradient()
+            assert abs(lnl[i] - l1) <= 1e-12 * abs(l1), (i, lnl[i], l1)
+            assert np.allclose(dbs[i], g1, rtol=1e-9, atol=1e-9)
+            assert np.allclose(dps[i], p1, rtol=1e-9, atol=1e-9)
+            want = O.loglikelihood(n, p, var)[0]
+            assert abs(lnl[i] - want) <= FP32_RTOL * abs(want)
+            single.close()
+    # parameters set between calls are picked up
+    n0, p0 = cands[0]
+    b = anns[0].network.brlen * 1.5
+    anns[0].set_branch_lengths(b)
+    lnl2, _, _, _ = batch.evaluate(0, gradient=False)
+    want = O.loglikelihood(n0, p0, 0, brlen=b)[0]
+    assert abs(lnl2[0] - want) <= FP32_RTOL * abs(want)
+    batch.close()
