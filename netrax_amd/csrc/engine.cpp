@@ -345,6 +345,7 @@ NetDesc make_netdesc(const Engine &e) {
   d.tips = e.d_tips.as<uint8_t>();
   d.Pf = e.d_Pf.as<float>();
   d.Qf = e.d_Qf.as<float>();
+  d.Pa = e.d_Pa.as<float>();
   d.tile_part = e.d_tile_part.as<int32_t>();
   d.freqs = e.d_freqs.as<double>();
   d.weights = e.d_weights.as<double>();
@@ -358,8 +359,10 @@ static void upload_netdesc(Engine &e) {
   if (e.opt.precision != NRX_FP32) return;
   std::vector<NetDesc> nd{make_netdesc(e)};
   upload(e.d_netdesc, nd, e.stream);
-  std::vector<WorkItem> work(e.L.n_tiles);
-  for (int t = 0; t < e.L.n_tiles; ++t) work[t] = WorkItem{0, t};
+  // work items are 64-site tiles, or 32-site tiles for 20-state data
+  const int n_work = e.S == AA_STATES ? 2 * e.L.n_tiles : e.L.n_tiles;
+  std::vector<WorkItem> work(n_work);
+  for (int t = 0; t < n_work; ++t) work[t] = WorkItem{0, t};
   upload(e.d_work, work, e.stream);
 }
 
@@ -459,6 +462,7 @@ static void update_pmatrices(Engine &e, bool do_update) {
   if (e.model_dirty) upload_models(e);
   upload(e.d_brlen, e.brlen, e.stream);
   launch_pmatrices(e);
+  if (e.opt.precision == NRX_FP32 && e.S == AA_STATES) launch_mfma_tables(e);
   e.model_dirty = false;
   std::fill(e.edge_dirty.begin(), e.edge_dirty.end(), 0);
 }
@@ -567,14 +571,21 @@ static int evaluate(Engine &e, int variant, bool incremental, bool update_pm, bo
   if (e.opt.precision == NRX_FP32) {
     // batched form: the whole op list in one persistent launch, root site
     // sums fused into it; every CLV is recomputed
+    const bool aa32 = e.S == AA_STATES;
     record_kernel_begin(e, 0);
-    launch_forward_persistent(e);
+    if (aa32)
+      launch_forward_mfma(e);
+    else
+      launch_forward_persistent(e);
     record_kernel_end(e, 0);
     std::fill(e.clv_dirty.begin(), e.clv_dirty.end(), 0);
     launch_mixture(e, variant, e.pw);
     if (gradient) {
       record_kernel_begin(e, 1);
-      launch_adjoint_persistent(e);
+      if (aa32)
+        launch_adjoint_mfma(e);
+      else
+        launch_adjoint_persistent(e);
       record_kernel_end(e, 1);
       launch_grad_reduce(e, e.pw);
     }
@@ -951,14 +962,14 @@ int nrx_create(const nrx_network_desc *net, const nrx_partition_desc *parts, int
     if (e->S != 4 && e->S != 20) throw std::invalid_argument("states must be 4 or 20");
     if (e->C != 1 && e->C != 2 && e->C != 4 && e->C != 8)
       throw std::invalid_argument("rate categories must be 1, 2, 4 or 8");
-    if (e->opt.precision == NRX_FP32) {
-      if (e->S != 4)
-        return fail(NRX_ERR_UNSUPPORTED, "the FP32 form is built for 4-state data");
-      // P (and Q) of one partition are staged in LDS (64 KiB per workgroup)
-      if (sizeof(float) * (static_cast<size_t>(e->E) + 1) * e->C * 16 > 65536)
-        return fail(NRX_ERR_UNSUPPORTED, "network too large for the FP32 form's LDS tables");
-    }
-    e->pw = TILE * e->C / 64;  // site-sum partials per tile: one per wave
+    const bool aa32 = e->opt.precision == NRX_FP32 && e->S == AA_STATES;
+    // P (and Q) of one partition are staged in LDS (64 KiB per workgroup)
+    if (e->opt.precision == NRX_FP32 && !aa32 &&
+        sizeof(float) * (static_cast<size_t>(e->E) + 1) * e->C * 16 > 65536)
+      return fail(NRX_ERR_UNSUPPORTED, "network too large for the FP32 form's LDS tables");
+    // site-sum partials per 64-site tile: one per wave (the 20-state FP32
+    // form runs two 32-site workgroups of C waves per tile)
+    e->pw = aa32 ? 2 * e->C : TILE * e->C / 64;
     e->parts.resize(n_parts);
     int64_t off = 0;
     int tile = 0;
@@ -1043,7 +1054,11 @@ int nrx_create(const nrx_network_desc *net, const nrx_partition_desc *parts, int
     if (e->opt.precision == NRX_FP32) {
       e->d_Pf.alloc(sizeof(float) * pm);
       e->d_adj.alloc(sizeof(float) * e->clv_elems_per_slot * std::max<size_t>(nv, 1));
-      e->d_adj_exp.alloc(sizeof(int32_t) * static_cast<size_t>(n) * std::max<size_t>(nv, 1));
+      // adjoint exponents: per site (4 states) or per (site, category) (20 states)
+      const size_t per_slot = aa32 ? static_cast<size_t>(n) * e->C : static_cast<size_t>(n);
+      e->d_adj_exp.alloc(sizeof(int32_t) * per_slot * std::max<size_t>(nv, 1));
+      if (aa32)
+        e->d_Pa.alloc(sizeof(float) * static_cast<size_t>(n_parts) * e->E * e->C * AA_KINDS * AA_TABLE);
     }
 
     const int nc = static_cast<int>(e->dag.classes.size());
@@ -1190,6 +1205,10 @@ int nrx_brlen_derivatives(nrx_engine *eng, int32_t variant, int32_t edge, int32_
     if (variant != NRX_AVERAGE_DISPLAYED_TREES && variant != NRX_BEST_DISPLAYED_TREE)
       throw std::invalid_argument("unknown likelihood variant");
     if (!(brlen >= 0.0)) throw std::invalid_argument("branch length must be >= 0");
+    if (e.opt.precision == NRX_FP32 && e.S == AA_STATES)
+      return fail(NRX_ERR_UNSUPPORTED,
+                  "per-branch sumtables of 20-state data need the FP64 form (the FP32 form "
+                  "returns the gradient of every branch through nrx_gradient)");
     const int nc = static_cast<int>(e.dag.classes.size());
     if (!e.deriv_valid || e.deriv_edge != edge) {
       const int rc = evaluate(e, variant, true, true, false);
@@ -1320,7 +1339,8 @@ int nrx_get_stats(nrx_engine *eng, nrx_stats *out) {
     // scaler, its children (and, for root CLVs, itself for the seed); per
     // inner child write the delivered adjoint, and read it back first when
     // another consumer delivered before
-    const double aexp = e.opt.precision == NRX_FP32 ? scl : 0.0;  // adjoint exponents
+    // adjoint exponents: per site, or per (site, category) for 20-state FP32
+    const double aexp = e.opt.precision == NRX_FP32 ? (e.S == AA_STATES ? scl * e.C : scl) : 0.0;
     double adj = 0.0;
     for (const DevOp &op : e.ops_all) {
       adj += scl;

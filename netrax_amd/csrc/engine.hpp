@@ -16,6 +16,15 @@ namespace nrx {
 constexpr int TILE = 64;
 constexpr int MAX_CATS = 8;
 
+// FP32 form for 20-state data (matrix cores): a workgroup handles a 32-site
+// tile, CLVs and adjoints are stored per tile as [C][20][32], and each
+// (partition, edge, category) has three operand tables (P, P^T, P') of
+// 64 lanes x 12 floats in the lane order of the matrix instruction.
+constexpr int AA_STATES = 20;
+constexpr int AA_TILE = 32;
+constexpr int AA_TABLE = 64 * 12;
+constexpr int AA_KINDS = 3;
+
 // ---------------------------------------------------------------- model
 struct EigenModel {
   int S = 0;
@@ -122,6 +131,7 @@ struct NetDesc {
   const uint8_t *tips;
   const float *Pf;
   const float *Qf;
+  const float *Pa;  // 20-state data: operand tables [part][edge][C][kind][AA_TABLE]
   const int32_t *tile_part;
   const double *freqs;
   const double *weights;
@@ -178,6 +188,9 @@ void launch_adjoint_level(Engine &e, const LevelChunk &ch);
 void launch_grad_reduce(Engine &e, int partials_per_tile);
 void launch_forward_persistent(Engine &e);
 void launch_adjoint_persistent(Engine &e);
+void launch_mfma_tables(Engine &e);
+void launch_forward_mfma(Engine &e);
+void launch_adjoint_mfma(Engine &e);
 void launch_sumtables(Engine &e, int n_paths);
 void launch_sumtable_eval(Engine &e, int n_paths, double brlen);
 

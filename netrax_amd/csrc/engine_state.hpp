@@ -109,6 +109,7 @@ struct Engine : nrx_engine {
   DevBuf d_V, d_Vinv, d_eval, d_rates, d_freqs, d_brlen;
   DevBuf d_P, d_dP, d_d2P, d_Pf, d_dPf;
   DevBuf d_clv, d_scl, d_adj, d_adj_exp, d_Qf;
+  DevBuf d_Pa;  // 20-state FP32 form: matrix-core operand tables
   DevBuf d_ops, d_ops_dirty, d_chunk_edges, d_Qd;
   DevBuf d_ops_dfs, d_group_begin, d_edge_use_begin, d_edge_use_list;
   DevBuf d_netdesc, d_work;  // FP32 form: this engine's NetDesc and (0, tile) work table

@@ -875,6 +875,383 @@ __global__ void __launch_bounds__(TILE *C)
   }
 }
 
+// ============================================= FP32 form, 20 states (MFMA)
+// CLV x P runs on the matrix cores.  For one rate category c and a tile of
+// 32 sites, D = P_c X is a chain of 10 v_mfma_f32_32x32x2f32 (M = parent
+// state, 20 padded to 32 with zero rows of P; K = child state, 2 per step;
+// N = site).  Operand layout of that instruction, lane l, h = l / 32,
+// col = l % 32:
+//   A (32 x 2)   lane l holds A[col][h]
+//   B (2 x 32)   lane l holds B[h][col]
+//   D (32 x 32)  lane l, register r holds D[8 (r / 4) + 4 h + r % 4][col]
+// CLVs and adjoints are stored per tile as [C][20 states][32 sites], so the
+// B operand of step t (state 2t + h) and every D register (one state per
+// half wave) is one coalesced pair of 128-byte rows; no transpose is
+// needed between a CLV's producer and its consumers.  The A operands (P,
+// P^T and P' = r_c Q P per partition, edge and category) are laid out in
+// lane order by k_mfma_tables.  A workgroup is C waves, wave c = rate
+// category c of one 32-site tile.  CLV scalers are per site, as in the
+// 4-state form: the waves combine their site maxima through LDS once per
+// group of independent ops.  Adjoint exponents are per (site, category),
+// so the reverse sweep only synchronises the waves at displayed-tree roots.
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+constexpr int AA_STEPS = 10;   // K = 20 states, 2 per instruction
+constexpr int AA_ROWS = 12;    // D registers holding real states (rows < 20 for h = 0)
+
+__device__ __forceinline__ int aa_row(int r, int h) { return 8 * (r >> 2) + 4 * h + (r & 3); }
+__device__ __forceinline__ bool aa_valid(int r, int h) { return r < 8 || h == 0; }
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ void load_a(const float *__restrict__ tab, int lane, float (&a)[AA_STEPS]) {
+  const float4 *p = reinterpret_cast<const float4 *>(tab + lane * (AA_TABLE / 64));
+  const float4 v0 = p[0], v1 = p[1], v2 = p[2];
+  a[0] = v0.x; a[1] = v0.y; a[2] = v0.z; a[3] = v0.w;
+  a[4] = v1.x; a[5] = v1.y; a[6] = v1.z; a[7] = v1.w;
+  a[8] = v2.x; a[9] = v2.y;
+}
+
+// B operand of a child: rows 2t + h of column col ([20][32] block of one
+// category), or the 0/1 vector of a tip's state mask
+__device__ __forceinline__ void load_b(const float *__restrict__ blk, int h, float (&b)[AA_STEPS]) {
+#pragma unroll
+  for (int t = 0; t < AA_STEPS; ++t) b[t] = blk[(2 * t + h) * AA_TILE];
+}
+__device__ __forceinline__ void tip_b(uint32_t mask, int h, float (&b)[AA_STEPS]) {
+#pragma unroll
+  for (int t = 0; t < AA_STEPS; ++t) b[t] = ((mask >> (2 * t + h)) & 1u) ? 1.0f : 0.0f;
+}
+
+__device__ __forceinline__ f32x16 mfma_chain(const float (&a)[AA_STEPS], const float (&b)[AA_STEPS]) {
+  f32x16 acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int t = 0; t < AA_STEPS; ++t) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[t], b[t], acc, 0, 0, 0);
+  return acc;
+}
+
+// Operand tables: Pa[((p E + e) C + c) 3 + kind][lane][12], entry t < 10 of
+// lane (h, col) = M[col][2t + h] with M = P (kind 0), P^T (1), P' (2); zero
+// for col >= 20 and t >= 10.
+__global__ void k_mfma_tables(DevLayout L, const double *__restrict__ P,
+                              const double *__restrict__ dP, float *__restrict__ Pa) {
+  const int64_t total = static_cast<int64_t>(L.n_parts) * L.n_edges * L.C * AA_KINDS * AA_TABLE;
+  const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (idx >= total) return;
+  const int t = idx % 12;
+  const int lane = (idx / 12) % 64;
+  const int kind = (idx / AA_TABLE) % AA_KINDS;
+  const int64_t mat = idx / (AA_TABLE * AA_KINDS);  // (p E + e) C + c
+  const int h = lane >> 5, col = lane & 31;
+  float v = 0.0f;
+  if (t < AA_STEPS && col < AA_STATES) {
+    const int k = 2 * t + h;
+    const double *M = (kind == 2 ? dP : P) + mat * (AA_STATES * AA_STATES);
+    v = static_cast<float>(kind == 1 ? M[k * AA_STATES + col] : M[col * AA_STATES + k]);
+  }
+  Pa[idx] = v;
+}
+
+template <int C, int G>
+__global__ void __launch_bounds__(64 * C)
+    k_forward_mfma(const NetDesc *__restrict__ nets, const WorkItem *__restrict__ work) {
+  __shared__ float smax[2][G][C][AA_TILE];
+  __shared__ double ssum[2][G][C][AA_TILE];
+  const WorkItem wi = work[blockIdx.x];
+  const NetDesc &nd = nets[wi.net];
+  const DevOp *__restrict__ ops = nd.ops;
+  const int32_t *__restrict__ group_begin = nd.group_begin;
+  const int n_groups = nd.n_groups;
+  const DevLayout L = nd.L;
+  float *__restrict__ clv = nd.clv;
+  int32_t *__restrict__ scl = nd.scl;
+  const uint32_t *__restrict__ tips = reinterpret_cast<const uint32_t *>(nd.tips);
+  const double *__restrict__ freqs = nd.freqs;
+  const int tile = wi.tile;  // 32-site tile
+  const int part = nd.tile_part[tile >> 1];
+  const int c = threadIdx.x >> 6, lane = threadIdx.x & 63, h = lane >> 5, col = lane & 31;
+  const int64_t n = L.n_sites_pad;
+  const int64_t site = static_cast<int64_t>(tile) * AA_TILE + col;
+  const size_t slot_stride = static_cast<size_t>(n) * C * AA_STATES;
+  const size_t tile_off = static_cast<size_t>(tile) * (C * AA_STATES * AA_TILE) +
+                          static_cast<size_t>(c) * (AA_STATES * AA_TILE) + col;
+  const float *__restrict__ Pa =
+      nd.Pa + static_cast<size_t>(part) * L.n_edges * C * AA_KINDS * AA_TABLE;
+  const double w = nd.weights[site];
+  const size_t n_wt = static_cast<size_t>(L.n_tiles) * 2 * C;
+  const size_t wt = static_cast<size_t>(tile) * C + c;
+  for (int gi = 0; gi < n_groups; ++gi) {
+    const int buf = gi & 1;
+    const int g0 = group_begin[gi];
+    const int nb = group_begin[gi + 1] - g0;
+    float v[G][AA_ROWS];
+    int ks[G];
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+      ks[j] = 0;
+      if (j < nb) {
+        const DevOp op = ops[g0 + j];
+#pragma unroll
+        for (int r = 0; r < AA_ROWS; ++r) v[j][r] = 1.0f;
+#pragma unroll
+        for (int kk = 0; kk < 2; ++kk) {
+          if (kk < op.nk) {
+            float a[AA_STEPS], b[AA_STEPS];
+            load_a(Pa + (static_cast<size_t>(op.edge[kk]) * C + c) * AA_KINDS * AA_TABLE, lane, a);
+            const int32_t kid = op.kid[kk];
+            if (kid >= 0) {
+              load_b(clv + kid * slot_stride + tile_off, h, b);
+              ks[j] += scl[static_cast<size_t>(kid) * n + site];
+            } else {
+              tip_b(tips[static_cast<size_t>(-kid - 1) * n + site], h, b);
+            }
+            const f32x16 d = mfma_chain(a, b);
+#pragma unroll
+            for (int r = 0; r < AA_ROWS; ++r) v[j][r] *= d[r];
+          }
+        }
+        float m = 0.0f;
+#pragma unroll
+        for (int r = 0; r < AA_ROWS; ++r) m = fmaxf(m, v[j][r]);
+        m = fmaxf(m, __shfl_xor(m, 32));
+        if (h == 0) smax[buf][j][c][col] = m;
+        if (op.root >= 0) {
+          double s = 0.0;
+#pragma unroll
+          for (int r = 0; r < AA_ROWS; ++r)
+            if (aa_valid(r, h)) s = fma(freqs[part * AA_STATES + aa_row(r, h)], static_cast<double>(v[j][r]), s);
+          s += __shfl_xor(s, 32);
+          if (h == 0) ssum[buf][j][c][col] = s;
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+      if (j < nb) {
+        const DevOp op = ops[g0 + j];
+        float m = 0.0f;
+#pragma unroll
+        for (int cc = 0; cc < C; ++cc) m = fmaxf(m, smax[buf][j][cc][col]);
+        int e = 0;
+        if (m > 0.f) (void)frexpf(m, &e);
+        const int k = ks[j] - e;
+        float *dst = clv + op.out * slot_stride + tile_off;
+#pragma unroll
+        for (int r = 0; r < AA_ROWS; ++r)
+          if (aa_valid(r, h)) dst[aa_row(r, h) * AA_TILE] = ldexpf(v[j][r], -e);
+        if (c == 0 && h == 0) scl[static_cast<size_t>(op.out) * n + site] = k;
+        if (op.root >= 0) {
+          double t = 0.0;
+#pragma unroll
+          for (int cc = 0; cc < C; ++cc) t += ssum[buf][j][cc][col];
+          t = ldexp(t, -e) * (1.0 / C);
+          double val = (c == 0 && h == 0 && w != 0.0) ? w * (log(t) - k * LN2) : 0.0;
+          val = wave_sum(val);
+          if (lane == 0) nd.root_partials[static_cast<size_t>(op.root) * n_wt + wt] = val;
+        }
+      }
+    }
+  }
+}
+
+// Adjoint delivery to a child (push form) with one exponent per (site,
+// category): lanes l and l ^ 32 hold the two halves of a site's states.
+__device__ __forceinline__ void deliver_adjoint_mfma(float *__restrict__ dst_m, int32_t *__restrict__ dst_e,
+                                                     bool first, int h, float (&b)[AA_ROWS], int eb) {
+  float mb = 0.0f;
+#pragma unroll
+  for (int r = 0; r < AA_ROWS; ++r) mb = fmaxf(mb, fabsf(b[r]));
+  mb = fmaxf(mb, __shfl_xor(mb, 32));
+  if (!(mb > 0.f)) eb = EXP_EMPTY;
+  if (!first) {
+    float acc[AA_ROWS];
+#pragma unroll
+    for (int r = 0; r < AA_ROWS; ++r) acc[r] = aa_valid(r, h) ? dst_m[aa_row(r, h) * AA_TILE] : 0.0f;
+    const int ea = *dst_e;
+    if (ea > eb) {
+#pragma unroll
+      for (int r = 0; r < AA_ROWS; ++r) b[r] = acc[r] + ldexpf(b[r], eb - ea);
+      eb = ea;
+    } else {
+#pragma unroll
+      for (int r = 0; r < AA_ROWS; ++r) b[r] += ldexpf(acc[r], ea - eb);
+    }
+    mb = 0.0f;
+#pragma unroll
+    for (int r = 0; r < AA_ROWS; ++r) mb = fmaxf(mb, fabsf(b[r]));
+    mb = fmaxf(mb, __shfl_xor(mb, 32));
+  }
+  if (mb > 0.f) {
+    int e;
+    (void)frexpf(mb, &e);
+#pragma unroll
+    for (int r = 0; r < AA_ROWS; ++r) b[r] = ldexpf(b[r], -e);
+    eb += e;
+  } else {
+    eb = EXP_EMPTY;
+  }
+#pragma unroll
+  for (int r = 0; r < AA_ROWS; ++r)
+    if (aa_valid(r, h)) dst_m[aa_row(r, h) * AA_TILE] = b[r];
+  if (h == 0) *dst_e = eb;
+}
+
+// Reverse sweep of the 20-state FP32 form (push form, as k_adjoint_persistent):
+// y_k = P_k x_k and P'_k x_k share the child's B operand; z_k = Abar * y_other
+// is turned from the D layout into a B operand through a wave-private LDS
+// block, and P_k^T z_k is delivered to the child.
+template <int C>
+__global__ void __launch_bounds__(64 * C)
+    k_adjoint_mfma(const NetDesc *__restrict__ nets, const WorkItem *__restrict__ work) {
+  __shared__ double rsum[2][C][AA_TILE];
+  __shared__ float zb[C][AA_STATES * AA_TILE];
+  const WorkItem wi = work[blockIdx.x];
+  const NetDesc &nd = nets[wi.net];
+  const DevOp *__restrict__ ops = nd.ops;
+  const int32_t *__restrict__ group_begin = nd.group_begin;
+  const int n_groups = nd.n_groups;
+  const DevLayout L = nd.L;
+  const float *__restrict__ clv = nd.clv;
+  const int32_t *__restrict__ scl = nd.scl;
+  float *__restrict__ adj = nd.adj;
+  int32_t *__restrict__ adj_exp = nd.adj_exp;
+  const uint32_t *__restrict__ tips = reinterpret_cast<const uint32_t *>(nd.tips);
+  const double *__restrict__ freqs = nd.freqs;
+  const int tile = wi.tile;
+  const int part = nd.tile_part[tile >> 1];
+  const int c = threadIdx.x >> 6, lane = threadIdx.x & 63, h = lane >> 5, col = lane & 31;
+  const int64_t n = L.n_sites_pad;
+  const int64_t site = static_cast<int64_t>(tile) * AA_TILE + col;
+  const size_t slot_stride = static_cast<size_t>(n) * C * AA_STATES;
+  const size_t tile_off = static_cast<size_t>(tile) * (C * AA_STATES * AA_TILE) +
+                          static_cast<size_t>(c) * (AA_STATES * AA_TILE) + col;
+  // adjoint exponents: [slot][tile][C][32]
+  const size_t exp_stride = static_cast<size_t>(n) * C;
+  const size_t exp_off = static_cast<size_t>(tile) * (C * AA_TILE) + static_cast<size_t>(c) * AA_TILE + col;
+  const float *__restrict__ Pa =
+      nd.Pa + static_cast<size_t>(part) * L.n_edges * C * AA_KINDS * AA_TABLE;
+  const double w = nd.weights[site];
+  const size_t n_wt = static_cast<size_t>(L.n_tiles) * 2 * C;
+  const size_t wt = static_cast<size_t>(tile) * C + c;
+  float *zw = zb[c];
+  int rb = 0;
+  for (int gi = n_groups - 1; gi >= 0; --gi) {
+    const int g0 = group_begin[gi];
+    const int g1 = group_begin[gi + 1];
+    for (int oi = g0; oi < g1; ++oi) {
+      const DevOp op = ops[oi];
+      const int kq = scl[static_cast<size_t>(op.out) * n + site];
+      float a[AA_ROWS];
+      int ea = EXP_EMPTY;
+      if (op.flags & OP_HAS_CONSUMERS) {
+        const float *src = adj + op.out * slot_stride + tile_off;
+#pragma unroll
+        for (int r = 0; r < AA_ROWS; ++r) a[r] = aa_valid(r, h) ? src[aa_row(r, h) * AA_TILE] : 0.0f;
+        ea = adj_exp[op.out * exp_stride + exp_off];
+      } else {
+#pragma unroll
+        for (int r = 0; r < AA_ROWS; ++r) a[r] = 0.0f;
+      }
+      float b[2][AA_STEPS];
+      int ks = 0;
+#pragma unroll
+      for (int kk = 0; kk < 2; ++kk) {
+        if (kk < op.nk) {
+          const int32_t kid = op.kid[kk];
+          if (kid >= 0) {
+            load_b(clv + kid * slot_stride + tile_off, h, b[kk]);
+            ks += scl[static_cast<size_t>(kid) * n + site];
+          } else {
+            tip_b(tips[static_cast<size_t>(-kid - 1) * n + site], h, b[kk]);
+          }
+        }
+      }
+      if (op.root >= 0) {
+        // seed beta * w * (1/C) pi_i / Ltilde, Ltilde = (1/C) sum_c pi . x_c
+        const float *xq = clv + op.out * slot_stride + tile_off;
+        double s = 0.0;
+#pragma unroll
+        for (int r = 0; r < AA_ROWS; ++r)
+          if (aa_valid(r, h))
+            s = fma(freqs[part * AA_STATES + aa_row(r, h)], static_cast<double>(xq[aa_row(r, h) * AA_TILE]), s);
+        s += __shfl_xor(s, 32);
+        if (h == 0) rsum[rb][c][col] = s;
+        __syncthreads();
+        double t = 0.0;
+#pragma unroll
+        for (int cc = 0; cc < C; ++cc) t += rsum[rb][cc][col];
+        t *= 1.0 / C;
+        rb ^= 1;
+        const double bw = nd.beta[part * nd.n_root + op.root] * w;
+        int ef = EXP_EMPTY;
+        float sd[AA_ROWS];
+        double mf = 0.0;
+        if (bw != 0.0) mf = frexp(bw / (C * t), &ef);
+#pragma unroll
+        for (int r = 0; r < AA_ROWS; ++r)
+          sd[r] = aa_valid(r, h) ? static_cast<float>(mf * freqs[part * AA_STATES + aa_row(r, h)]) : 0.0f;
+        if (ef > ea) {
+#pragma unroll
+          for (int r = 0; r < AA_ROWS; ++r) a[r] = sd[r] + ldexpf(a[r], ea - ef);
+          ea = ef;
+        } else {
+#pragma unroll
+          for (int r = 0; r < AA_ROWS; ++r) a[r] += ldexpf(sd[r], ef - ea);
+        }
+      }
+      f32x16 y[2];
+#pragma unroll
+      for (int kk = 0; kk < 2; ++kk) {
+        if (kk < op.nk) {
+          float pa[AA_STEPS];
+          load_a(Pa + (static_cast<size_t>(op.edge[kk]) * C + c) * AA_KINDS * AA_TABLE, lane, pa);
+          y[kk] = mfma_chain(pa, b[kk]);
+        }
+      }
+      const int ez = ea + kq - ks;
+#pragma unroll
+      for (int kk = 0; kk < 2; ++kk) {
+        if (kk < op.nk) {
+          const float *tab = Pa + (static_cast<size_t>(op.edge[kk]) * C + c) * AA_KINDS * AA_TABLE;
+          float z[AA_ROWS];
+#pragma unroll
+          for (int r = 0; r < AA_ROWS; ++r) z[r] = op.nk == 2 ? a[r] * y[1 - kk][r] : a[r];
+          float pd[AA_STEPS];
+          load_a(tab + 2 * AA_TABLE, lane, pd);
+          const f32x16 dy = mfma_chain(pd, b[kk]);
+          float gd = 0.0f;
+#pragma unroll
+          for (int r = 0; r < AA_ROWS; ++r) gd = fmaf(z[r], dy[r], gd);
+          const double gv = wave_sum(ldexp(static_cast<double>(gd), ez));
+          if (lane == 0) nd.grad_partials[static_cast<size_t>(op.gslot[kk]) * n_wt + wt] = gv;
+          const int32_t kid = op.kid[kk];
+          if (kid >= 0) {
+#pragma unroll
+            for (int r = 0; r < AA_ROWS; ++r)
+              if (aa_valid(r, h)) zw[aa_row(r, h) * AA_TILE + col] = z[r];
+            wave_sync();
+            float bz[AA_STEPS];
+#pragma unroll
+            for (int t = 0; t < AA_STEPS; ++t) bz[t] = zw[(2 * t + h) * AA_TILE + col];
+            wave_sync();
+            float pt[AA_STEPS];
+            load_a(tab + AA_TABLE, lane, pt);
+            const f32x16 bk = mfma_chain(pt, bz);
+            float back[AA_ROWS];
+#pragma unroll
+            for (int r = 0; r < AA_ROWS; ++r) back[r] = bk[r];
+            deliver_adjoint_mfma(adj + kid * slot_stride + tile_off, adj_exp + kid * exp_stride + exp_off,
+                                 (op.flags & (kk == 0 ? OP_FIRST_KID0 : OP_FIRST_KID1)) != 0, h, back, ez);
+          }
+        }
+      }
+    }
+  }
+}
 
 // ===================================================== branch derivatives
 // Sumtables of one branch e, one row per displayed tree t containing it
@@ -1280,6 +1657,33 @@ void launch_adjoint_persistent_multi(hipStream_t stream, int C, const NetDesc *n
     constexpr int CC = decltype(c)::value;
     hipLaunchKernelGGL((k_adjoint_persistent<CC, 4>), dim3(n_work), dim3(TILE * CC), lds, stream,
                        nets, work);
+  });
+  NRX_HIP(hipGetLastError());
+}
+
+void launch_mfma_tables(Engine &e) {
+  const int64_t total = static_cast<int64_t>(e.P) * e.E * e.C * AA_KINDS * AA_TABLE;
+  const int nb = static_cast<int>((total + 255) / 256);
+  hipLaunchKernelGGL(k_mfma_tables, dim3(nb), dim3(256), 0, e.stream, e.L, e.d_P.as<double>(),
+                     e.d_dP.as<double>(), e.d_Pa.as<float>());
+  NRX_HIP(hipGetLastError());
+}
+
+// one workgroup per 32-site tile: the work table holds 2 n_tiles items
+void launch_forward_mfma(Engine &e) {
+  dispatch_c(e.C, [&](auto c) {
+    constexpr int CC = decltype(c)::value;
+    hipLaunchKernelGGL((k_forward_mfma<CC, 4>), dim3(2 * e.L.n_tiles), dim3(64 * CC), 0, e.stream,
+                       e.d_netdesc.as<NetDesc>(), e.d_work.as<WorkItem>());
+  });
+  NRX_HIP(hipGetLastError());
+}
+
+void launch_adjoint_mfma(Engine &e) {
+  dispatch_c(e.C, [&](auto c) {
+    constexpr int CC = decltype(c)::value;
+    hipLaunchKernelGGL((k_adjoint_mfma<CC>), dim3(2 * e.L.n_tiles), dim3(64 * CC), 0, e.stream,
+                       e.d_netdesc.as<NetDesc>(), e.d_work.as<WorkItem>());
   });
   NRX_HIP(hipGetLastError());
 }

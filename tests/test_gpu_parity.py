@@ -404,6 +404,63 @@ def test_protein_fp64_matches_oracle():
         ann.close()
 
 
+def test_protein_fp32_matrix_cores_match_oracle():
+    """20-state FP32 form (CLV x P on v_mfma_f32_32x32x2f32): two uneven
+    partitions (tiles padded to 32/64 sites), AVERAGE and BEST, lnL within
+    1e-6 relative and the full gradient vs the FP64 oracle."""
+    say("protein fp32 40 taxa / 2 reticulations, partitions 700 + 333")
+    w = make_workload("40taxa_2retic_aa", 40, 2, [700, 333], seed=4, protein=True)
+    for var in (0, 1):
+        ann = engine32(w.network, w.partitions, var)
+        got = computeLoglikelihood(ann)
+        want = O.loglikelihood(w.network, w.partitions, var)[0]
+        assert abs(got - want) <= FP32_RTOL * abs(want), (var, got, want)
+        lnl, g, gp = ann.gradient()
+        _, og, ogp = O.gradient(w.network, w.partitions, var)
+        scale = max(1.0, np.abs(og).max())
+        assert np.abs(g - og).max() <= 1e-4 * scale, (var, np.abs(g - og).max(), scale)
+        assert np.abs(gp - ogp).max(initial=0.0) <= 1e-4 * max([1.0, *np.abs(ogp)])
+        ann.close()
+
+
+def test_protein_fp32_full_size_matches_fp64():
+    """The protein workload at full size (40 taxa, 2 reticulations, 20,000
+    sites): FP32 matrix-core form vs the FP64 form (itself pinned to the
+    oracle above)."""
+    from netrax_amd.simulate import workload_protein_20k
+    say("protein fp32 vs fp64 at 20k sites")
+    w = workload_protein_20k()
+    for var in (0, 1):
+        a64 = engine(w.network, w.partitions, var)
+        a32 = engine32(w.network, w.partitions, var)
+        l64, g64, p64 = a64.gradient()
+        l32, g32, p32 = a32.gradient()
+        assert abs(l32 - l64) <= FP32_RTOL * abs(l64), (var, l32, l64)
+        scale = max(1.0, np.abs(g64).max())
+        assert np.abs(g32 - g64).max() <= 1e-4 * scale, (var, np.abs(g32 - g64).max(), scale)
+        assert np.abs(p32 - p64).max(initial=0.0) <= 1e-4 * max([1.0, *np.abs(p64)])
+        a64.close()
+        a32.close()
+
+
+def test_protein_fp32_deep_underflow_matches_closed_form():
+    """Saturated branches on 150 tips with 20 states and 4 categories
+    (equal exchangeabilities, alpha = 50 so every category rate is near 1):
+    the site likelihoods underflow FP32 many times over; lnL = sum log pi."""
+    from netrax_amd.network import random_network
+    from netrax_amd.simulate import Model, Partition
+    rng = np.random.default_rng(12)
+    net = random_network(150, 0, rng)
+    net.brlen[:] = 100.0
+    freqs = rng.dirichlet(np.ones(20))
+    m = Model(20, np.ones(190), freqs, 50.0, 4)
+    states = rng.integers(20, size=(150, 90))
+    part = Partition(m, (1 << states).astype(np.uint32), np.ones(90, np.uint32))
+    want = float(np.log(freqs)[states].sum())
+    got = computeLoglikelihood(engine32(net, [part]))
+    assert abs(got - want) <= 1e-6 * abs(want), (got, want)
+
+
 # ---------------------------------------------------------- network batches
 def test_network_batch_matches_single_engines_and_oracle():
     """One launch per kernel for a batch of candidate networks (different
