@@ -97,7 +97,11 @@ typedef struct {
   double min_tree_logprob; /* NetraxOptions::min_interesting_tree_logprob, log(1e-6) */
   double brlen_min;        /* branch lengths are clamped to [brlen_min, brlen_max] */
   double brlen_max;
-  int32_t use_graph;       /* capture the evaluation in a hipGraph (FP32 form) */
+  int32_t use_graph;       /* FP32 form: each (variant, gradient) evaluation is captured
+                              once in a hipGraph and replayed as one graph launch (re-
+                              captured after a schedule change); per-kernel timings are
+                              then not recorded, only the total.  A batch follows the
+                              option of its first engine. */
   int32_t timing;          /* record HIP events around kernels (nrx_last_timing) */
 } nrx_options;
 
@@ -178,7 +182,10 @@ int nrx_get_stats(nrx_engine *eng, nrx_stats *out);
  * of a search step (src/search/Filtering.cpp:236-270 scores candidates one
  * at a time): every engine of the batch is scored by one launch per kernel.
  * All engines must use NRX_FP32, 4-state data and the same number of rate
- * categories on the same device, and must outlive the batch. */
+ * categories on the same device, and must outlive the batch.  With
+ * use_graph set on the first engine, the whole batch evaluation (P matrices,
+ * persistent forward, reductions, mixture, persistent adjoint, gradient
+ * totals, result gather) is one hipGraph launch. */
 typedef struct nrx_batch nrx_batch;
 
 int nrx_batch_create(nrx_engine *const *engines, int32_t n_engines, nrx_batch **out);
@@ -192,6 +199,31 @@ int nrx_batch_evaluate(nrx_batch *batch, int32_t variant, double *lnl, double *d
 
 /* Device timing of the last evaluation (zeros unless timing is enabled). */
 int nrx_last_timing(nrx_engine *eng, nrx_timing *out);
+
+/* Shape of the FP32 form's persistent kernels.  No counterpart in the
+ * reference (libpll's kernels have no launch shape); the autotuner picks
+ * it per engine.  4-state data: group_width (2, 4 or 8) independent ops
+ * per group, whose CLV loads are issued together, and tile_sites (16, 32
+ * or 64, with tile_sites * rate_cats a multiple of 64) sites per
+ * workgroup, i.e. tile_sites * rate_cats / 64 wavefronts per site block.
+ * 20-state data: group_width 1, 2 or 4 and 32 sites per workgroup (one
+ * matrix-instruction column block). */
+typedef struct {
+  int32_t group_width;
+  int32_t tile_sites;
+  double ms; /* device time of one lnL + gradient evaluation (autotune) */
+} nrx_tuning;
+
+/* Times one lnL + gradient evaluation (best of `reps`) for every candidate
+ * shape, keeps the fastest for the engine and returns it in *best;
+ * results[0 .. min(max_results, *n_results)) (may be NULL) receive every
+ * candidate's time.  Leaves the engine's CLVs recomputed. */
+int nrx_autotune(nrx_engine *eng, int32_t variant, int32_t reps, nrx_tuning *best,
+                 nrx_tuning *results, int32_t max_results, int32_t *n_results);
+/* Sets a shape (e.g. one found by an earlier nrx_autotune); NRX_ERR_ARG if
+ * the engine's data cannot take it.  Engines of one batch must share it. */
+int nrx_set_tuning(nrx_engine *eng, int32_t group_width, int32_t tile_sites);
+int nrx_get_tuning(nrx_engine *eng, nrx_tuning *out);
 
 #ifdef __cplusplus
 }

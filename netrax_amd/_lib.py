@@ -24,7 +24,7 @@ EXPORTED = [
     "nrx_set_category_rates", "nrx_set_branch_lengths", "nrx_set_reticulation_probs",
     "nrx_loglikelihood", "nrx_gradient", "nrx_brlen_derivatives", "nrx_tree_logls",
     "nrx_get_stats", "nrx_last_timing", "nrx_batch_create", "nrx_batch_destroy",
-    "nrx_batch_evaluate",
+    "nrx_batch_evaluate", "nrx_autotune", "nrx_set_tuning", "nrx_get_tuning",
 ]
 
 
@@ -58,6 +58,10 @@ class Stats(C.Structure):
 class Timing(C.Structure):
     _fields_ = [("total_ms", C.c_double), ("fwd_ms", C.c_double), ("fwd_launches", C.c_int32),
                 ("adj_launches", C.c_int32), ("adj_ms", C.c_double)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("group_width", C.c_int32), ("tile_sites", C.c_int32), ("ms", C.c_double)]
 
 
 _lib = None
@@ -95,6 +99,10 @@ def load(build_if_missing: bool = True):
     lib.nrx_batch_create.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_void_p)]
     lib.nrx_batch_destroy.argtypes = [C.c_void_p]
     lib.nrx_batch_evaluate.argtypes = [C.c_void_p, C.c_int32, pd, pd, pd, pd]
+    lib.nrx_autotune.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(Tuning),
+                                 C.POINTER(Tuning), C.c_int32, C.POINTER(C.c_int32)]
+    lib.nrx_set_tuning.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
+    lib.nrx_get_tuning.argtypes = [C.c_void_p, C.POINTER(Tuning)]
     _lib = lib
     return lib
 

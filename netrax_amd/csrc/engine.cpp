@@ -56,8 +56,37 @@ void upload(DevBuf &b, const std::vector<T> &v, hipStream_t s) {
 
 }  // namespace
 
+static void destroy_graphs(hipGraphExec_t (&g)[2][2]) {
+  for (auto &row : g)
+    for (auto &x : row) {
+      if (x) (void)hipGraphExecDestroy(x);
+      x = nullptr;
+    }
+}
+
+// Captures the work `enqueue` puts on `stream` into an instantiated graph.
+template <typename F>
+static hipGraphExec_t capture_graph(hipStream_t stream, F &&enqueue) {
+  hipGraph_t g = nullptr;
+  NRX_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+  try {
+    enqueue();
+  } catch (...) {
+    (void)hipStreamEndCapture(stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  NRX_HIP(hipStreamEndCapture(stream, &g));
+  hipGraphExec_t exec = nullptr;
+  const hipError_t err = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  NRX_HIP(err);
+  return exec;
+}
+
 Engine::~Engine() {
   if (stream) (void)hipStreamSynchronize(stream);
+  destroy_graphs(graph_exec);
   if (h_result) (void)hipHostFree(h_result);
   if (ev_begin) (void)hipEventDestroy(ev_begin);
   if (ev_end) (void)hipEventDestroy(ev_end);
@@ -265,7 +294,7 @@ static void build_schedule(Engine &e) {
       }
     }
   }
-  constexpr int GROUP = 4;
+  const int GROUP = e.tune_group;
   {
     std::vector<int> pos(nv, -1), deps(nv, 0);
     std::vector<std::vector<int>> parents(nv);
@@ -359,8 +388,8 @@ static void upload_netdesc(Engine &e) {
   if (e.opt.precision != NRX_FP32) return;
   std::vector<NetDesc> nd{make_netdesc(e)};
   upload(e.d_netdesc, nd, e.stream);
-  // work items are 64-site tiles, or 32-site tiles for 20-state data
-  const int n_work = e.S == AA_STATES ? 2 * e.L.n_tiles : e.L.n_tiles;
+  // work items: tiles of tune_tile sites, or of 32 sites for 20-state data
+  const int n_work = static_cast<int>(e.L.n_sites_pad / (e.S == AA_STATES ? AA_TILE : e.tune_tile));
   std::vector<WorkItem> work(n_work);
   for (int t = 0; t < n_work; ++t) work[t] = WorkItem{0, t};
   upload(e.d_work, work, e.stream);
@@ -572,23 +601,37 @@ static int evaluate(Engine &e, int variant, bool incremental, bool update_pm, bo
     // batched form: the whole op list in one persistent launch, root site
     // sums fused into it; every CLV is recomputed
     const bool aa32 = e.S == AA_STATES;
-    record_kernel_begin(e, 0);
-    if (aa32)
-      launch_forward_mfma(e);
-    else
-      launch_forward_persistent(e);
-    record_kernel_end(e, 0);
-    std::fill(e.clv_dirty.begin(), e.clv_dirty.end(), 0);
-    launch_mixture(e, variant, e.pw);
-    if (gradient) {
-      record_kernel_begin(e, 1);
+    auto enqueue = [&e, aa32, variant, gradient](bool timed) {
+      if (timed) record_kernel_begin(e, 0);
       if (aa32)
-        launch_adjoint_mfma(e);
+        launch_forward_mfma(e);
       else
-        launch_adjoint_persistent(e);
-      record_kernel_end(e, 1);
-      launch_grad_reduce(e, e.pw);
+        launch_forward_persistent(e);
+      if (timed) record_kernel_end(e, 0);
+      launch_mixture(e, variant, e.pw);
+      if (gradient) {
+        if (timed) record_kernel_begin(e, 1);
+        if (aa32)
+          launch_adjoint_mfma(e);
+        else
+          launch_adjoint_persistent(e);
+        if (timed) record_kernel_end(e, 1);
+        launch_grad_reduce(e, e.pw);
+      }
+    };
+    if (e.opt.use_graph) {
+      // one graph launch per evaluation; re-captured after a schedule change
+      if (e.graph_version != e.schedule_version) {
+        destroy_graphs(e.graph_exec);
+        e.graph_version = e.schedule_version;
+      }
+      hipGraphExec_t &ge = e.graph_exec[variant][gradient ? 1 : 0];
+      if (!ge) ge = capture_graph(e.stream, [&]() { enqueue(false); });
+      NRX_HIP(hipGraphLaunch(ge, e.stream));
+    } else {
+      enqueue(true);
     }
+    std::fill(e.clv_dirty.begin(), e.clv_dirty.end(), 0);
   } else {
     forward(e, incremental);
     launch_root_partials(e);
@@ -701,8 +744,12 @@ struct Batch {
   int n_work = 0, C = 0, max_pm = 0, max_root_rows = 0, max_use_rows = 0, max_edges = 0,
       max_res = 0;
   size_t lds_fwd = 0, lds_adj = 0;
+  bool use_graph = false;
+  int G = 4, TS = TILE;  // shape of the persistent kernels, shared by all engines
+  hipGraphExec_t graph_exec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [variant][gradient]
   ~Batch() {
     if (stream) (void)hipStreamSynchronize(stream);
+    destroy_graphs(graph_exec);
     if (h_out) (void)hipHostFree(h_out);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
@@ -753,6 +800,7 @@ static SmallDesc make_smalldesc(const Engine &e) {
 // longer op lists come first so the longest workgroups start earliest.
 static void build_batch(Batch &b) {
   const int n = static_cast<int>(b.engines.size());
+  destroy_graphs(b.graph_exec);  // they hold the old tables' addresses
   std::vector<NetDesc> nets(n);
   std::vector<SmallDesc> small(n);
   std::vector<std::pair<int, int>> order;  // (cost, engine)
@@ -760,8 +808,12 @@ static void build_batch(Batch &b) {
   b.max_pm = b.max_root_rows = b.max_use_rows = b.max_edges = b.max_res = 0;
   b.lds_fwd = b.lds_adj = 0;
   b.versions.resize(n);
+  b.G = b.engines[0]->tune_group;
+  b.TS = b.engines[0]->tune_tile;
   for (int i = 0; i < n; ++i) {
     const Engine &e = *b.engines[i];
+    if (e.tune_group != b.G || e.tune_tile != b.TS)
+      throw std::invalid_argument("batch engines must share the FP32 kernel shape (nrx_set_tuning)");
     nets[i] = make_netdesc(e);
     small[i] = make_smalldesc(e);
     b.versions[i] = e.schedule_version;
@@ -781,7 +833,7 @@ static void build_batch(Batch &b) {
                    });
   std::vector<WorkItem> work;
   for (const auto &o : order)
-    for (int t = 0; t < b.engines[o.second]->L.n_tiles; ++t) work.push_back({o.second, t});
+    for (int t = 0; t < b.engines[o.second]->L.n_sites_pad / b.TS; ++t) work.push_back({o.second, t});
   b.n_work = static_cast<int>(work.size());
   upload(b.d_nets, nets, b.stream);
   upload(b.d_small, small, b.stream);
@@ -813,6 +865,7 @@ int nrx_batch_create(nrx_engine *const *engines, int32_t n_engines, nrx_batch **
       b->engines.push_back(e);
     }
     b->C = b->engines[0]->C;
+    b->use_graph = b->engines[0]->opt.use_graph != 0;
     NRX_HIP(hipSetDevice(b->engines[0]->opt.device));
     NRX_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     NRX_HIP(hipEventCreate(&b->ev0));
@@ -853,20 +906,31 @@ int nrx_batch_evaluate(nrx_batch *batch, int32_t variant, double *lnl, double *d
     }
     if (rebuild) build_batch(b);
     const bool grad = dbrlen != nullptr;
-    NRX_HIP(hipEventRecord(b.ev0, b.stream));
-    const SmallDesc *sd = b.d_small.as<SmallDesc>();
-    launch_pmatrices_multi(b.stream, sd, n, b.max_pm);
-    launch_forward_persistent_multi(b.stream, b.C, b.d_nets.as<NetDesc>(), b.d_work.as<WorkItem>(),
-                                    b.n_work, b.lds_fwd);
-    launch_root_reduce_multi(b.stream, sd, n, b.max_root_rows);
-    launch_mixture_multi(b.stream, sd, n, variant);
-    if (grad) {
-      launch_adjoint_persistent_multi(b.stream, b.C, b.d_nets.as<NetDesc>(),
-                                      b.d_work.as<WorkItem>(), b.n_work, b.lds_adj);
-      launch_grad_reduce_multi(b.stream, sd, n, b.max_use_rows, b.max_edges);
+    auto enqueue = [&]() {
+      const SmallDesc *sd = b.d_small.as<SmallDesc>();
+      launch_pmatrices_multi(b.stream, sd, n, b.max_pm);
+      launch_forward_persistent_multi(b.stream, b.C, b.G, b.TS, b.d_nets.as<NetDesc>(),
+                                      b.d_work.as<WorkItem>(), b.n_work, b.lds_fwd);
+      launch_root_reduce_multi(b.stream, sd, n, b.max_root_rows);
+      launch_mixture_multi(b.stream, sd, n, variant);
+      if (grad) {
+        launch_adjoint_persistent_multi(b.stream, b.C, b.G, b.TS, b.d_nets.as<NetDesc>(),
+                                        b.d_work.as<WorkItem>(), b.n_work, b.lds_adj);
+        launch_grad_reduce_multi(b.stream, sd, n, b.max_use_rows, b.max_edges);
+      }
+      launch_gather_results(b.stream, sd, n, b.d_offsets.as<int64_t>(), b.d_out.as<double>(),
+                            b.max_res);
+    };
+    hipGraphExec_t *ge = nullptr;
+    if (b.use_graph) {
+      ge = &b.graph_exec[variant][grad ? 1 : 0];
+      if (!*ge) *ge = capture_graph(b.stream, enqueue);
     }
-    launch_gather_results(b.stream, sd, n, b.d_offsets.as<int64_t>(), b.d_out.as<double>(),
-                          b.max_res);
+    NRX_HIP(hipEventRecord(b.ev0, b.stream));
+    if (ge)
+      NRX_HIP(hipGraphLaunch(*ge, b.stream));
+    else
+      enqueue();
     NRX_HIP(hipEventRecord(b.ev1, b.stream));
     NRX_HIP(hipMemcpyAsync(b.h_out, b.d_out.p, sizeof(double) * b.offsets[n],
                            hipMemcpyDeviceToHost, b.stream));
@@ -1371,6 +1435,102 @@ int nrx_last_timing(nrx_engine *eng, nrx_timing *out) {
   return guarded([&]() -> int {
     if (!eng || !out) throw std::invalid_argument("null argument");
     *out = static_cast<Engine *>(eng)->last_timing;
+    return NRX_OK;
+  });
+}
+
+}  // extern "C"
+
+namespace nrx {
+
+// Shapes of the FP32 form's persistent kernels: 4-state data takes G ops
+// per group and TS sites per workgroup (TS C threads, a multiple of 64);
+// 20-state data has 32 sites per workgroup and tunes G only.
+static std::vector<std::pair<int, int>> shape_candidates(const Engine &e) {
+  std::vector<std::pair<int, int>> out;
+  if (e.S == AA_STATES) {
+    for (int g : {1, 2, 4}) out.push_back({g, AA_TILE});
+  } else {
+    for (int ts : {64, 32, 16})
+      if (ts * e.C % 64 == 0)
+        for (int g : {4, 2, 8}) out.push_back({g, ts});
+  }
+  return out;
+}
+
+static void apply_shape(Engine &e, int g, int ts) {
+  if (e.tune_group == g && e.tune_tile == ts) return;
+  e.tune_group = g;
+  e.tune_tile = ts;
+  build_schedule(e);
+  std::fill(e.clv_dirty.begin(), e.clv_dirty.end(), 1);
+  e.cache_valid = false;
+}
+
+}  // namespace nrx
+
+extern "C" {
+
+int nrx_set_tuning(nrx_engine *eng, int32_t group_width, int32_t tile_sites) {
+  return guarded([&]() -> int {
+    if (!eng) throw std::invalid_argument("null argument");
+    Engine &e = *static_cast<Engine *>(eng);
+    if (e.opt.precision != NRX_FP32)
+      return fail(NRX_ERR_UNSUPPORTED, "kernel shapes are tuned for the FP32 form only");
+    const auto cands = shape_candidates(e);
+    if (std::find(cands.begin(), cands.end(), std::make_pair(group_width, tile_sites)) == cands.end())
+      throw std::invalid_argument("unsupported kernel shape for this engine");
+    apply_shape(e, group_width, tile_sites);
+    return NRX_OK;
+  });
+}
+
+int nrx_get_tuning(nrx_engine *eng, nrx_tuning *out) {
+  return guarded([&]() -> int {
+    if (!eng || !out) throw std::invalid_argument("null argument");
+    const Engine &e = *static_cast<Engine *>(eng);
+    out->group_width = e.tune_group;
+    out->tile_sites = e.S == AA_STATES ? AA_TILE : e.tune_tile;
+    out->ms = 0.0;
+    return NRX_OK;
+  });
+}
+
+int nrx_autotune(nrx_engine *eng, int32_t variant, int32_t reps, nrx_tuning *best,
+                 nrx_tuning *results, int32_t max_results, int32_t *n_results) {
+  return guarded([&]() -> int {
+    if (!eng || !best || reps < 1) throw std::invalid_argument("bad autotune arguments");
+    Engine &e = *static_cast<Engine *>(eng);
+    if (e.opt.precision != NRX_FP32)
+      return fail(NRX_ERR_UNSUPPORTED, "kernel shapes are tuned for the FP32 form only");
+    const auto cands = shape_candidates(e);
+    const int32_t saved_timing = e.opt.timing;
+    e.opt.timing = 1;
+    std::vector<double> ms(cands.size(), INFINITY);
+    int rc = NRX_OK;
+    try {
+      for (size_t i = 0; i < cands.size() && rc == NRX_OK; ++i) {
+        apply_shape(e, cands[i].first, cands[i].second);
+        rc = evaluate(e, variant, false, true, true);  // warm-up (and graph capture)
+        for (int r = 0; r < reps && rc == NRX_OK; ++r) {
+          rc = evaluate(e, variant, false, true, true);
+          ms[i] = std::min(ms[i], static_cast<double>(e.last_timing.total_ms));
+        }
+      }
+    } catch (...) {
+      e.opt.timing = saved_timing;
+      throw;
+    }
+    e.opt.timing = saved_timing;
+    if (rc != NRX_OK) return rc;
+    const size_t bi = static_cast<size_t>(std::min_element(ms.begin(), ms.end()) - ms.begin());
+    apply_shape(e, cands[bi].first, cands[bi].second);
+    best->group_width = cands[bi].first;
+    best->tile_sites = cands[bi].second;
+    best->ms = ms[bi];
+    for (size_t i = 0; results && i < cands.size() && static_cast<int32_t>(i) < max_results; ++i)
+      results[i] = nrx_tuning{cands[i].first, cands[i].second, ms[i]};
+    if (n_results) *n_results = static_cast<int32_t>(cands.size());
     return NRX_OK;
   });
 }

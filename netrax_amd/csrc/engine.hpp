@@ -174,9 +174,10 @@ void launch_mixture_multi(hipStream_t s, const SmallDesc *d, int n, int variant)
 void launch_grad_reduce_multi(hipStream_t s, const SmallDesc *d, int n, int max_rows, int max_edges);
 void launch_gather_results(hipStream_t s, const SmallDesc *d, int n, const int64_t *offsets,
                            double *out, int max_size);
-void launch_forward_persistent_multi(hipStream_t stream, int C, const NetDesc *nets,
+// G = ops per group, TS = sites per workgroup (the autotuned shape)
+void launch_forward_persistent_multi(hipStream_t stream, int C, int G, int TS, const NetDesc *nets,
                                      const WorkItem *work, int n_work, size_t lds);
-void launch_adjoint_persistent_multi(hipStream_t stream, int C, const NetDesc *nets,
+void launch_adjoint_persistent_multi(hipStream_t stream, int C, int G, int TS, const NetDesc *nets,
                                      const WorkItem *work, int n_work, size_t lds);
 
 // Kernel launchers (kernels.hip)

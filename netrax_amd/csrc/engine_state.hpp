@@ -80,6 +80,10 @@ struct Engine : nrx_engine {
   std::vector<int32_t> edge_use_begin, edge_use_list;  // gradient rows per edge
   int n_uses = 0;
   int pw = 1;  // site-sum partials per tile (one per wave)
+  // FP32 form shape (nrx_autotune / nrx_set_tuning): ops per group of the
+  // persistent kernels and sites per workgroup (4-state data; 32 for 20)
+  int tune_group = 4;
+  int tune_tile = TILE;
   int schedule_version = 0;  // bumped by every schedule rebuild
 
   // branch derivatives: sumtables of one branch for every displayed tree
@@ -119,6 +123,11 @@ struct Engine : nrx_engine {
   double *h_result = nullptr;  // pinned
   ResultLayout rl{};
   size_t clv_elems_per_slot = 0;  // n_sites_pad * C * S
+
+  // FP32 form: instantiated graphs of one evaluation, [variant][gradient],
+  // valid for schedule `graph_version`
+  hipGraphExec_t graph_exec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+  int graph_version = -1;
 
   // timing
   hipEvent_t ev_begin = nullptr, ev_end = nullptr;

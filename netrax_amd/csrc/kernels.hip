@@ -605,11 +605,14 @@ __device__ __forceinline__ void mv4t(const float *__restrict__ M, const float (&
     y[j] = fmaf(M[12 + j], x[3], fmaf(M[8 + j], x[2], fmaf(M[4 + j], x[1], M[j] * x[0])));
 }
 
-template <int C, int G>
-__global__ void __launch_bounds__(TILE *C)
+// TS sites per workgroup (TS C threads, a multiple of 64, TS <= 64 so a
+// workgroup never straddles partitions) and G ops per group are the
+// autotuned shape parameters.
+template <int C, int G, int TS>
+__global__ void __launch_bounds__(TS *C)
     k_forward_persistent(const NetDesc *__restrict__ nets, const WorkItem *__restrict__ work) {
   float *sP = dyn_lds<float>();  // P of this tile's partition: [edge][cat][4][4]
-  constexpr int PW = TILE * C / 64;
+  constexpr int PW = TILE * C / 64;  // site-sum partials (waves) per 64-site tile
   const WorkItem wi = work[blockIdx.x];
   const NetDesc &nd = nets[wi.net];
   const DevOp *__restrict__ ops = nd.ops;
@@ -624,17 +627,17 @@ __global__ void __launch_bounds__(TILE *C)
   const double *__restrict__ freqs = nd.freqs;
   const double *__restrict__ weights = nd.weights;
   double *__restrict__ root_partials = nd.root_partials;
-  const int tile = wi.tile;
-  const int part = tile_part[tile];
+  const int64_t site0 = static_cast<int64_t>(wi.tile) * TS;
+  const int part = tile_part[site0 / TILE];
   const int np = L.n_edges * C * 16;
   const float *gP = Pf + static_cast<size_t>(part) * np;
   for (int i = threadIdx.x; i < np; i += blockDim.x) sP[i] = gP[i];
   __syncthreads();
   const int c = threadIdx.x % C;
-  const int64_t site = static_cast<int64_t>(tile) * TILE + threadIdx.x / C;
+  const int64_t site = site0 + threadIdx.x / C;
   const int64_t n = L.n_sites_pad;
   const size_t n_wt = static_cast<size_t>(L.n_tiles) * PW;
-  const int wt = tile * PW + threadIdx.x / 64;
+  const size_t wt = static_cast<size_t>(site0 * C / 64) + threadIdx.x / 64;  // global wave
   const double w = weights[site];
   double pi[4];
 #pragma unroll
@@ -737,8 +740,8 @@ __device__ __forceinline__ void deliver_adjoint(float *__restrict__ dst_m, int32
   if (lane0) *dst_e = eb;
 }
 
-template <int C, int G>
-__global__ void __launch_bounds__(TILE *C)
+template <int C, int G, int TS>
+__global__ void __launch_bounds__(TS *C)
     k_adjoint_persistent(const NetDesc *__restrict__ nets, const WorkItem *__restrict__ work) {
   float *sP = dyn_lds<float>();  // [edge][cat][16] then Q: [cat][16]
   constexpr int PW = TILE * C / 64;
@@ -761,8 +764,8 @@ __global__ void __launch_bounds__(TILE *C)
   const double *__restrict__ beta = nd.beta;
   const int n_root = nd.n_root;
   double *__restrict__ grad_partials = nd.grad_partials;
-  const int tile = wi.tile;
-  const int part = tile_part[tile];
+  const int64_t site0 = static_cast<int64_t>(wi.tile) * TS;
+  const int part = tile_part[site0 / TILE];
   const int np = L.n_edges * C * 16;
   const float *gP = Pf + static_cast<size_t>(part) * np;
   for (int i = threadIdx.x; i < np; i += blockDim.x) sP[i] = gP[i];
@@ -770,10 +773,10 @@ __global__ void __launch_bounds__(TILE *C)
   __syncthreads();
   const int c = threadIdx.x % C;
   const float *sQ = sP + np + c * 16;
-  const int64_t site = static_cast<int64_t>(tile) * TILE + threadIdx.x / C;
+  const int64_t site = site0 + threadIdx.x / C;
   const int64_t n = L.n_sites_pad;
   const size_t n_wt = static_cast<size_t>(L.n_tiles) * PW;
-  const int wt = tile * PW + threadIdx.x / 64;
+  const size_t wt = static_cast<size_t>(site0 * C / 64) + threadIdx.x / 64;  // global wave
   const bool lane0 = c == 0;
   const double w = weights[site];
   double pi[4];
@@ -1629,33 +1632,58 @@ void dispatch_c(int C, F &&f) {
   }
 }
 
+// Shapes of the 4-state persistent kernels: group width G in {2, 4, 8}
+// and TS in {16, 32, 64} sites per workgroup with TS C a multiple of 64.
+template <typename F>
+void dispatch_shape(int C, int G, int TS, F &&f) {
+  auto with_ts = [&](auto c, auto g) {
+    constexpr int CC = decltype(c)::value;
+    if (TS == 64) return f(c, g, std::integral_constant<int, 64>{});
+    if constexpr (32 * CC % 64 == 0)
+      if (TS == 32) return f(c, g, std::integral_constant<int, 32>{});
+    if constexpr (16 * CC % 64 == 0)
+      if (TS == 16) return f(c, g, std::integral_constant<int, 16>{});
+    throw std::invalid_argument("unsupported sites per workgroup for this number of categories");
+  };
+  dispatch_c(C, [&](auto c) {
+    switch (G) {
+      case 2: with_ts(c, std::integral_constant<int, 2>{}); break;
+      case 4: with_ts(c, std::integral_constant<int, 4>{}); break;
+      case 8: with_ts(c, std::integral_constant<int, 8>{}); break;
+      default: throw std::invalid_argument("unsupported group width");
+    }
+  });
+}
+
 void launch_forward_persistent(Engine &e) {
-  launch_forward_persistent_multi(e.stream, e.C, e.d_netdesc.as<NetDesc>(),
-                                  e.d_work.as<WorkItem>(), e.L.n_tiles,
+  launch_forward_persistent_multi(e.stream, e.C, e.tune_group, e.tune_tile,
+                                  e.d_netdesc.as<NetDesc>(), e.d_work.as<WorkItem>(),
+                                  static_cast<int>(e.L.n_sites_pad / e.tune_tile),
                                   sizeof(float) * e.E * e.C * 16);
 }
 
 void launch_adjoint_persistent(Engine &e) {
-  launch_adjoint_persistent_multi(e.stream, e.C, e.d_netdesc.as<NetDesc>(),
-                                  e.d_work.as<WorkItem>(), e.L.n_tiles,
+  launch_adjoint_persistent_multi(e.stream, e.C, e.tune_group, e.tune_tile,
+                                  e.d_netdesc.as<NetDesc>(), e.d_work.as<WorkItem>(),
+                                  static_cast<int>(e.L.n_sites_pad / e.tune_tile),
                                   sizeof(float) * (e.E * e.C * 16 + e.C * 16));
 }
 
-void launch_forward_persistent_multi(hipStream_t stream, int C, const NetDesc *nets,
+void launch_forward_persistent_multi(hipStream_t stream, int C, int G, int TS, const NetDesc *nets,
                                      const WorkItem *work, int n_work, size_t lds) {
-  dispatch_c(C, [&](auto c) {
-    constexpr int CC = decltype(c)::value;
-    hipLaunchKernelGGL((k_forward_persistent<CC, 4>), dim3(n_work), dim3(TILE * CC), lds, stream,
+  dispatch_shape(C, G, TS, [&](auto c, auto g, auto ts) {
+    constexpr int CC = decltype(c)::value, GG = decltype(g)::value, T = decltype(ts)::value;
+    hipLaunchKernelGGL((k_forward_persistent<CC, GG, T>), dim3(n_work), dim3(T * CC), lds, stream,
                        nets, work);
   });
   NRX_HIP(hipGetLastError());
 }
 
-void launch_adjoint_persistent_multi(hipStream_t stream, int C, const NetDesc *nets,
+void launch_adjoint_persistent_multi(hipStream_t stream, int C, int G, int TS, const NetDesc *nets,
                                      const WorkItem *work, int n_work, size_t lds) {
-  dispatch_c(C, [&](auto c) {
-    constexpr int CC = decltype(c)::value;
-    hipLaunchKernelGGL((k_adjoint_persistent<CC, 4>), dim3(n_work), dim3(TILE * CC), lds, stream,
+  dispatch_shape(C, G, TS, [&](auto c, auto g, auto ts) {
+    constexpr int CC = decltype(c)::value, GG = decltype(g)::value, T = decltype(ts)::value;
+    hipLaunchKernelGGL((k_adjoint_persistent<CC, GG, T>), dim3(n_work), dim3(T * CC), lds, stream,
                        nets, work);
   });
   NRX_HIP(hipGetLastError());
@@ -1673,8 +1701,17 @@ void launch_mfma_tables(Engine &e) {
 void launch_forward_mfma(Engine &e) {
   dispatch_c(e.C, [&](auto c) {
     constexpr int CC = decltype(c)::value;
-    hipLaunchKernelGGL((k_forward_mfma<CC, 4>), dim3(2 * e.L.n_tiles), dim3(64 * CC), 0, e.stream,
-                       e.d_netdesc.as<NetDesc>(), e.d_work.as<WorkItem>());
+    auto go = [&](auto g) {
+      constexpr int GG = decltype(g)::value;
+      hipLaunchKernelGGL((k_forward_mfma<CC, GG>), dim3(2 * e.L.n_tiles), dim3(64 * CC), 0,
+                         e.stream, e.d_netdesc.as<NetDesc>(), e.d_work.as<WorkItem>());
+    };
+    switch (e.tune_group) {
+      case 1: go(std::integral_constant<int, 1>{}); break;
+      case 2: go(std::integral_constant<int, 2>{}); break;
+      case 4: go(std::integral_constant<int, 4>{}); break;
+      default: throw std::invalid_argument("unsupported group width for 20-state data");
+    }
   });
   NRX_HIP(hipGetLastError());
 }

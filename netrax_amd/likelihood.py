@@ -215,6 +215,26 @@ class AnnotatedNetwork:
         _lib.check(self.lib.nrx_last_timing(self.handle, C.byref(t)))
         return t
 
+    def autotune(self, variant: Optional[int] = None, reps: int = 3):
+        """FP32 form: times every kernel shape (ops per group, sites per
+        workgroup), keeps the fastest; returns (best, [all candidates]) as
+        (group_width, tile_sites, ms) tuples."""
+        v = int(self.options.likelihood_variant if variant is None else variant)
+        best = _lib.Tuning()
+        res = (_lib.Tuning * 16)()
+        n = C.c_int32()
+        _lib.check(self.lib.nrx_autotune(self.handle, v, int(reps), C.byref(best), res, 16, C.byref(n)))
+        return ((best.group_width, best.tile_sites, best.ms),
+                [(r.group_width, r.tile_sites, r.ms) for r in res[:n.value]])
+
+    def set_tuning(self, group_width: int, tile_sites: int) -> None:
+        _lib.check(self.lib.nrx_set_tuning(self.handle, int(group_width), int(tile_sites)))
+
+    def tuning(self):
+        t = _lib.Tuning()
+        _lib.check(self.lib.nrx_get_tuning(self.handle, C.byref(t)))
+        return t.group_width, t.tile_sites
+
     def close(self) -> None:
         if getattr(self, "handle", None):
             self.lib.nrx_destroy(self.handle)

@@ -1,0 +1,60 @@
+"""Autotuned kernel shapes of the FP32 form: every candidate shape (ops per
+group x sites per workgroup) gives the same lnL and gradient as the FP64
+form, the autotuner keeps the fastest, and batches run with a tuned shape."""
+import numpy as np
+import pytest
+
+from netrax_amd import AnnotatedNetwork, NetraxOptions, NetworkBatch, Precision
+from netrax_amd.simulate import candidate_batch, make_workload, workload_30taxa_10k
+
+pytestmark = pytest.mark.gpu
+
+
+def _check_all_shapes(w):
+    a64 = AnnotatedNetwork(w.network, w.partitions, NetraxOptions())
+    a32 = AnnotatedNetwork(w.network, w.partitions, NetraxOptions(precision=Precision.FP32))
+    l64, g64, p64 = a64.gradient()
+    best, cands = a32.autotune(reps=2)
+    print("autotune", best, cands, flush=True)
+    assert len(cands) >= 3
+    assert best[2] == min(c[2] for c in cands)
+    assert a32.tuning() == best[:2]
+    for gw, ts, ms in cands:
+        a32.set_tuning(gw, ts)
+        l32, g32, p32 = a32.gradient()
+        assert abs(l32 - l64) <= 1e-6 * abs(l64), (gw, ts, l32, l64)
+        scale = max(1.0, np.abs(g64).max())
+        assert np.abs(g32 - g64).max() <= 1e-4 * scale, (gw, ts)
+        assert np.abs(p32 - p64).max(initial=0.0) <= 1e-4 * max([1.0, *np.abs(p64)])
+    with pytest.raises(Exception):
+        a32.set_tuning(3, 64)
+    a32.close()
+    a64.close()
+
+
+def test_every_shape_matches_fp64_and_autotune_picks_fastest():
+    print("shapes 30 taxa", flush=True)
+    _check_all_shapes(workload_30taxa_10k())
+
+
+def test_protein_fp32_every_shape_matches_fp64():
+    print("shapes protein", flush=True)
+    _check_all_shapes(make_workload("aa", 20, 2, [700], seed=8, protein=True))
+
+
+def test_batch_runs_with_a_tuned_shape():
+    print("batch with shape (2, 32)", flush=True)
+    cands = candidate_batch(2, 2, seed=10, sites=(300, 2000))
+    anns = [AnnotatedNetwork(n, p, NetraxOptions(precision=Precision.FP32)) for n, p in cands]
+    for a in anns:
+        a.set_tuning(2, 32)
+    batch = NetworkBatch(anns)
+    lnl, dbs, dps, _ = batch.evaluate(0)
+    for i, a in enumerate(anns):
+        l, g, p = a.gradient()
+        assert abs(l - lnl[i]) <= 1e-9 * abs(l)
+        assert np.allclose(g, dbs[i], rtol=1e-6, atol=1e-6 * max(1.0, np.abs(g).max()))
+    anns[0].set_tuning(4, 64)
+    with pytest.raises(Exception):
+        NetworkBatch(anns)
+    batch.close()
