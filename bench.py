@@ -42,9 +42,17 @@ def make_engine(w, precision, timing=True, use_graph=False):
     return AnnotatedNetwork(w.network.copy(), w.partitions, opts)
 
 
-def run_form(w, precision, steps, warmup, use_graph=False):
-    """Times `steps` lnL+gradient evaluations; returns a result dict."""
+def run_form(w, precision, steps, warmup, use_graph=False, tune=False):
+    """Times `steps` lnL+gradient evaluations; returns a result dict.  With
+    `tune` (FP32 form) the kernel shape is autotuned first, outside the
+    timed region."""
     ann = make_engine(w, precision, use_graph=use_graph)
+    tuning = None
+    if tune:
+        best, cands = ann.autotune(reps=3)
+        tuning = {"group_width": best[0], "tile_sites": best[1],
+                  "candidates_ms": {f"{g}x{t}": round(ms, 4) for g, t, ms in cands}}
+        log(f"autotune: {best} of {cands}")
     b0 = w.network.brlen.copy()
     b1 = b0 * (1.0 + 1e-7)
     fwd_ms = adj_ms = tot_ms = 0.0
@@ -74,7 +82,7 @@ def run_form(w, precision, steps, warmup, use_graph=False):
         "adj_ms_per_eval": adj_ms / steps,
         "fwd_launches_per_eval": fwd_l / steps,
         "adj_launches_per_eval": adj_l / steps,
-        "lnl": lnl, "grad": g, "dprob": gp, "stats": st,
+        "lnl": lnl, "grad": g, "dprob": gp, "stats": st, "tuning": tuning,
     }
     ann.close()
     return res
@@ -90,6 +98,122 @@ def roofline(res):
             "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None, "kernel": name,
             "algorithmic_bytes_per_eval": by, "kernel_ms_per_eval": round(ms, 4),
             "launches_per_eval": launches}
+
+
+MFMA_FP32_PEAK_TFLOPS = 157.3
+MFMA_FLOPS_32x32x2 = 32 * 32 * 2 * 2  # one v_mfma_f32_32x32x2f32
+
+
+def bench_protein(steps, warmup):
+    """40 taxa / 2 reticulations / 20,000 amino-acid sites, GTR-like 20-state
+    model + G4: plain FP64 form vs the FP32 matrix-core form; matrix-core
+    utilisation of the FP32 kernels from the instruction count the op list
+    implies (10 instructions per (op, child, category, 32-site tile) forward;
+    20 + 10 per inner child in the adjoint)."""
+    from netrax_amd.simulate import workload_protein_20k
+
+    w = workload_protein_20k()
+    out = {"workload": "40 taxa / 2 reticulations / 20,000 amino-acid sites (1 partition), "
+                       "20-state model + G4, lnL + full gradient",
+           "patterns": int(sum(p.n_sites for p in w.partitions))}
+    forms = {}
+    for name, prec in (("plain_fp64", 0), ("fp32_mfma", 1)):
+        r = run_form(w, prec, steps, warmup, tune=prec == 1)
+        forms[name] = r
+        st = r["stats"]
+        d = {"evals_per_s": round(r["evals_per_s"], 3),
+             "device_ms_per_eval": round(r["device_ms_per_eval"], 4),
+             "fwd_ms": round(r["fwd_ms_per_eval"], 4), "adj_ms": round(r["adj_ms_per_eval"], 4),
+             "fwd_hbm_gbps": round(st.fwd_op_bytes / (r["fwd_ms_per_eval"] * 1e-3) / 1e9, 1),
+             "adj_hbm_gbps": round(st.adj_bytes / (r["adj_ms_per_eval"] * 1e-3) / 1e9, 1)}
+        if prec == 1:
+            C = w.partitions[0].model.ncat
+            blocks = st.n_sites_padded // 32 * C
+            fwd_fl = st.n_uses * blocks * 10 * MFMA_FLOPS_32x32x2
+            adj_fl = (2 * st.n_uses + st.n_inner_uses) * blocks * 10 * MFMA_FLOPS_32x32x2
+            fu = fwd_fl / (r["fwd_ms_per_eval"] * 1e-3) / 1e12
+            au = adj_fl / (r["adj_ms_per_eval"] * 1e-3) / 1e12
+            d["mfma"] = {"peak_tflops": MFMA_FP32_PEAK_TFLOPS,
+                         "fwd_tflops": round(fu, 2), "fwd_util": round(fu / MFMA_FP32_PEAK_TFLOPS, 4),
+                         "adj_tflops": round(au, 2), "adj_util": round(au / MFMA_FP32_PEAK_TFLOPS, 4),
+                         "useful_fraction": round(400 / 640, 4),
+                         "note": "issued matrix-instruction flops (20 states padded to 32 rows)"}
+            d["tuning"] = r["tuning"]
+        out[name] = d
+        log(f"protein {name}: {r['evals_per_s']:.1f} evals/s, fwd {r['fwd_ms_per_eval']:.3f} ms, "
+            f"adj {r['adj_ms_per_eval']:.3f} ms")
+    p64, a32 = forms["plain_fp64"], forms["fp32_mfma"]
+    out["fp32_vs_fp64"] = {"speedup": round(a32["evals_per_s"] / p64["evals_per_s"], 3),
+                           "rel_lnl_diff": abs(a32["lnl"] - p64["lnl"]) / abs(p64["lnl"]),
+                           "max_grad_diff_rel_to_norm": float(np.abs(a32["grad"] - p64["grad"]).max()
+                                                              / max(1.0, np.abs(p64["grad"]).max()))}
+    return out
+
+
+def bench_candidates(n_cand, steps, warmup):
+    """A search step's candidates: n_cand networks (8 seeded bases, 20-60
+    taxa, 0-5 reticulations, 2k-50k DNA sites log-uniform, GTR+G4; each
+    candidate one seeded rSPR / arc insertion / arc removal) scored -- lnL +
+    full gradient -- by one hipGraph launch of the network batch, against
+    the same engines evaluated one at a time."""
+    from netrax_amd import AnnotatedNetwork, NetraxOptions, NetworkBatch, Precision
+    from netrax_amd.simulate import candidate_batch
+
+    t0 = time.perf_counter()
+    per_base = max(1, n_cand // 8)
+    cands = candidate_batch(8, per_base, seed=5)
+    anns = [AnnotatedNetwork(n, p, NetraxOptions(precision=Precision.FP32, use_graph=True, timing=True))
+            for n, p in cands]
+    stats = [a.stats() for a in anns]
+    log(f"candidates: {len(anns)} engines in {time.perf_counter() - t0:.1f}s, "
+        f"{sum(s.clv_bytes for s in stats) / 1e9:.1f} GB of CLVs/adjoints")
+    big = max(range(len(anns)), key=lambda i: stats[i].grad_bytes)
+    best, _ = anns[big].autotune(reps=2)
+    for a in anns:
+        a.set_tuning(best[0], best[1])
+    batch = NetworkBatch(anns)
+    for _ in range(warmup):
+        batch.evaluate(0)
+    dev = 0.0
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        lnl, dbs, dps, ms = batch.evaluate(0)
+        dev += ms
+    wall = time.perf_counter() - t1
+    # the same engines one at a time (each evaluation one graph launch)
+    seq_dev = 0.0
+    t2 = time.perf_counter()
+    for a in anns:
+        a.gradient()
+        seq_dev += a.last_timing().total_ms
+    seq_wall = time.perf_counter() - t2
+    # spot check against the FP64 form
+    errs = []
+    for i in range(0, len(anns), max(1, len(anns) // 4)):
+        n, p = cands[i]
+        a64 = AnnotatedNetwork(n, p, NetraxOptions())
+        l64 = a64.gradient()[0]
+        errs.append(abs(lnl[i] - l64) / abs(l64))
+        a64.close()
+    gb = sum(s.grad_bytes for s in stats)
+    out = {"workload": f"{len(anns)} candidate networks (8 bases x {per_base} seeded moves), "
+                       "20-60 taxa, 0-5 reticulations, 2k-50k DNA sites, GTR+G4, lnL + full gradient",
+           "networks_per_s": round(len(anns) * steps / wall, 2),
+           "batch_device_ms": round(dev / steps, 4),
+           "batch_wall_ms": round(1e3 * wall / steps, 4),
+           "one_at_a_time_device_ms": round(seq_dev, 4),
+           "one_at_a_time_wall_ms": round(1e3 * seq_wall, 4),
+           "batch_speedup_device": round(seq_dev / (dev / steps), 3),
+           "algorithmic_gbps": round(gb / (dev / steps * 1e-3) / 1e9, 1),
+           "patterns_total": int(sum(s.n_sites_padded for s in stats)),
+           "tuning": {"group_width": best[0], "tile_sites": best[1]},
+           "max_rel_lnl_diff_vs_fp64_sampled": max(errs)}
+    batch.close()
+    for a in anns:
+        a.close()
+    log(f"candidates: {out['networks_per_s']} networks/s, batch {out['batch_device_ms']} ms vs "
+        f"{out['one_at_a_time_device_ms']} ms one at a time")
+    return out
 
 
 def cpu_baseline(w, sample_patterns, threads, budget_s):
@@ -162,6 +286,9 @@ def main():
     ap.add_argument("--cpu-sample-patterns", type=int, default=4000)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, host cores)")
     ap.add_argument("--cpu-budget-s", type=float, default=12.0)
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the protein and candidate-batch sections")
+    ap.add_argument("--candidates", type=int, default=256)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device copy of BYTES (PMC counter calibration)")
     args = ap.parse_args()
@@ -185,7 +312,7 @@ def main():
     for name, prec in (("plain", 0), ("batched", 1)):
         if args.only and args.only != name:
             continue
-        forms[name] = run_form(w, prec, args.steps, args.warmup)
+        forms[name] = run_form(w, prec, args.steps, args.warmup, tune=prec == 1)
         r = forms[name]
         log(f"{name}: {r['evals_per_s']:.1f} evals/s, device {r['device_ms_per_eval']:.3f} ms/eval "
             f"(fwd {r['fwd_ms_per_eval']:.3f} ms, adj {r['adj_ms_per_eval']:.3f} ms)")
@@ -216,6 +343,7 @@ def main():
             "patterns": n_pat, "sites": 50000, "edges": int(w.network.n_edges),
             "clv_ops": st.n_ops_needed, "levels": st.n_levels, "displayed_trees": st.n_classes,
             "adjoint_uses": st.n_uses, "parallelism": "single GPU",
+            "tuning": head["tuning"],
         },
         "roofline": roofline(head),
         "device_ms_per_eval": round(head["device_ms_per_eval"], 4),
@@ -236,6 +364,9 @@ def main():
             "max_grad_diff_rel_to_norm": float(np.abs(b32["grad"] - p64["grad"]).max()
                                                / max(1.0, np.abs(p64["grad"]).max())),
         }
+    if not args.no_extras and not args.only:
+        out["protein"] = bench_protein(max(5, args.steps // 2), args.warmup)
+        out["candidate_batch"] = bench_candidates(args.candidates, 3, 1)
     if not args.no_oracle_check:
         out["accuracy"] = oracle_check(w, head["lnl"], head["grad"], head["dprob"], threads)
     if not args.no_cpu_baseline:

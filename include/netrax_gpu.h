@@ -112,6 +112,7 @@ typedef struct {
   int32_t n_classes;       /* root variants (distinct displayed trees) */
   int32_t n_root_clvs;     /* distinct CLVs the root kernel reads */
   int32_t n_uses;          /* (parent op, child) pairs walked by the adjoint */
+  int32_t n_inner_uses;    /* those pairs whose child is a CLV (not a tip) */
   int64_t n_sites_padded;  /* sites over all partitions after tile padding */
   /* algorithmic HBM bytes over the real (unpadded) sites: each CLV, scaler
    * and tip row counted once per read or write an op makes */

@@ -381,6 +381,15 @@ NetDesc make_netdesc(const Engine &e) {
   d.beta = e.d_beta.as<double>();
   d.root_partials = e.d_root_partials.as<double>();
   d.grad_partials = e.d_grad_partials.as<double>();
+  // every buffer the persistent kernels read must exist before its address
+  // is handed to the device
+  const bool aa = e.S == AA_STATES;
+  const void *need[] = {d.ops, d.group_begin, d.clv, d.scl, d.adj, d.adj_exp, d.tips,
+                        aa ? static_cast<const void *>(d.Pa) : d.Pf,
+                        aa ? static_cast<const void *>(d.Pa) : d.Qf,
+                        d.tile_part, d.freqs, d.weights, d.beta, d.root_partials, d.grad_partials};
+  for (const void *p : need)
+    if (!p) throw std::logic_error("device descriptor built before its buffers were allocated");
   return d;
 }
 
@@ -793,6 +802,12 @@ static SmallDesc make_smalldesc(const Engine &e) {
   d.rl_part_dbrlen = e.rl.part_dbrlen;
   d.rl_dprob = e.rl.dprob;
   d.pw = e.pw;
+  const void *need[] = {d.V, d.Vinv, d.ev, d.rates, d.brlen, d.P, d.dP, d.d2P, d.Pf,
+                        d.part_tile_begin, d.classes, d.class_dlogprob, d.root_partials,
+                        d.root_sums, d.grad_partials, d.use_sums, d.beta, d.edge_use_begin,
+                        d.edge_use_list, d.res};
+  for (const void *p : need)
+    if (!p) throw std::logic_error("device descriptor built before its buffers were allocated");
   return d;
 }
 
@@ -1106,6 +1121,17 @@ int nrx_create(const nrx_network_desc *net, const nrx_partition_desc *parts, int
     upload(e->d_tile_part, tile_part, e->stream);
     upload(e->d_part_tile_begin, ptb, e->stream);
 
+    // model and branch-length buffers have fixed sizes: allocate them now so
+    // their addresses, which the device descriptors hold, never change
+    const size_t ss = static_cast<size_t>(n_parts) * e->S * e->S;
+    e->d_V.alloc(sizeof(double) * ss);
+    e->d_Vinv.alloc(sizeof(double) * ss);
+    e->d_eval.alloc(sizeof(double) * n_parts * e->S);
+    e->d_rates.alloc(sizeof(double) * n_parts * e->C);
+    e->d_freqs.alloc(sizeof(double) * n_parts * e->S);
+    e->d_brlen.alloc(sizeof(double) * e->E);
+    e->d_Qd.alloc(sizeof(double) * ss * e->C);
+    if (e->opt.precision == NRX_FP32) e->d_Qf.alloc(sizeof(float) * ss * e->C);
     const size_t pm = static_cast<size_t>(n_parts) * e->E * e->C * e->S * e->S;
     e->d_P.alloc(sizeof(double) * pm);
     e->d_dP.alloc(sizeof(double) * pm);
@@ -1384,6 +1410,8 @@ int nrx_get_stats(nrx_engine *eng, nrx_stats *out) {
     out->n_classes = static_cast<int>(e.dag.classes.size());
     out->n_root_clvs = static_cast<int>(e.root_slots.size());
     out->n_uses = e.n_uses;
+    for (const DevOp &op : e.ops_all)
+      for (int k = 0; k < op.nk; ++k) out->n_inner_uses += op.kid[k] >= 0;
     out->n_sites_padded = e.L.n_sites_pad;
     // algorithmic bytes over the real (unpadded) sites
     double sites = 0.0;
