@@ -42,13 +42,19 @@ def make_engine(w, precision, timing=True, use_graph=False):
     return AnnotatedNetwork(w.network.copy(), w.partitions, opts)
 
 
+SHAPE = None  # --shape G x TS: fixed FP32 kernel shape instead of autotuning
+
+
 def run_form(w, precision, steps, warmup, use_graph=False, tune=False):
     """Times `steps` lnL+gradient evaluations; returns a result dict.  With
     `tune` (FP32 form) the kernel shape is autotuned first, outside the
-    timed region."""
+    timed region (or set from --shape)."""
     ann = make_engine(w, precision, use_graph=use_graph)
     tuning = None
-    if tune:
+    if tune and SHAPE:
+        ann.set_tuning(*SHAPE)
+        tuning = {"group_width": SHAPE[0], "tile_sites": SHAPE[1], "fixed": True}
+    elif tune:
         best, cands = ann.autotune(reps=3)
         tuning = {"group_width": best[0], "tile_sites": best[1],
                   "candidates_ms": {f"{g}x{t}": round(ms, 4) for g, t, ms in cands}}
@@ -88,16 +94,41 @@ def run_form(w, precision, steps, warmup, use_graph=False, tune=False):
     return res
 
 
-def roofline(res):
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "pmc_traffic_50taxa_4retic_50k.md")
+
+
+def pmc_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed PMC summary
+    (FETCH_SIZE and WRITE_SIZE runs scaled by the copy-kernel calibration),
+    or None when that kernel shape was not profiled."""
+    try:
+        text = open(PMC_SUMMARY).read()
+        block = text.split("```json", 1)[1].split("```", 1)[0]
+        k = json.loads(block)["kernels"].get(kernel)
+        return None if k is None else k["traffic_bytes"]
+    except (OSError, IndexError, ValueError, KeyError):
+        return None
+
+
+def roofline(res, ncat=4):
     st = res["stats"]
     kern = [("forward pruning", st.fwd_op_bytes, res["fwd_ms_per_eval"], res["fwd_launches_per_eval"]),
             ("adjoint pruning", st.adj_bytes, res["adj_ms_per_eval"], res["adj_launches_per_eval"])]
     name, by, ms, launches = max(kern, key=lambda k: k[2])
     achieved = by / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    t = res.get("tuning")
+    step = "forward" if name.startswith("forward") else "adjoint"
+    if t:  # persistent form: one launch per evaluation
+        kname = f"k_{step}_persistent<{ncat}, {t['group_width']}, {t['tile_sites']}>"
+    else:  # plain form: one launch per pruning level
+        kname = f"k_{step}_level<4, {ncat}>" if step == "adjoint" else f"k_forward_level<4, {ncat}, 4>"
+    traffic = pmc_traffic(kname)
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None, "kernel": name,
-            "algorithmic_bytes_per_eval": by, "kernel_ms_per_eval": round(ms, 4),
-            "launches_per_eval": launches}
+            "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic, "kernel": kname,
+            "algorithmic_bytes_per_eval": by, "algorithmic_bytes_per_launch": round(by / max(launches, 1)),
+            "kernel_ms_per_eval": round(ms, 4), "launches_per_eval": launches,
+            "traffic_source": "profiles/pmc_traffic_50taxa_4retic_50k.md (rocprofv3 --pmc FETCH_SIZE / "
+                              "WRITE_SIZE, copy-kernel calibrated), bytes per launch" if traffic else None}
 
 
 MFMA_FP32_PEAK_TFLOPS = 157.3
@@ -259,16 +290,16 @@ def oracle_check(w, lnl, g, gp, threads):
 
 
 def copy_calibration(nbytes):
-    """Device-to-device copy of a known size, for scaling PMC byte counters."""
-    hip = C.CDLL("libamdhip64.so")
-    a, b = C.c_void_p(), C.c_void_p()
-    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
-    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
-    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    """A copy kernel of known size (reads and writes nbytes each), three
+    times, for scaling the PMC byte counters of a profiled run."""
+    import torch
+
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
     for _ in range(3):
-        assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
-    assert hip.hipDeviceSynchronize() == 0
-    print(json.dumps({"copy_bytes": nbytes}), flush=True)
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": nbytes, "kernel": "torch copy_ (elementwise)"}), flush=True)
 
 
 def main():
@@ -279,8 +310,8 @@ def main():
     ap.add_argument("--form", choices=["batched", "plain"], default="batched",
                     help="headline form: batched = FP32 CLVs, persistent site-tile kernels; "
                          "plain = FP64 CLVs, one launch per pruning level")
-    ap.add_argument("--only", choices=["batched", "plain"], default=None,
-                    help="time only this form (profiling runs)")
+    ap.add_argument("--only", choices=["batched", "plain", "protein", "candidates"], default=None,
+                    help="time only this form or section (profiling runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-oracle-check", action="store_true")
     ap.add_argument("--cpu-sample-patterns", type=int, default=4000)
@@ -289,9 +320,15 @@ def main():
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the protein and candidate-batch sections")
     ap.add_argument("--candidates", type=int, default=256)
+    ap.add_argument("--shape", default=None, metavar="GxTS",
+                    help="fixed FP32 kernel shape (ops per group x sites per workgroup) "
+                         "instead of autotuning, e.g. 2x64 (profiling runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device copy of BYTES (PMC counter calibration)")
     args = ap.parse_args()
+    global SHAPE
+    if args.shape:
+        SHAPE = tuple(int(v) for v in args.shape.lower().split("x"))
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
         return 0
@@ -300,6 +337,13 @@ def main():
               "design only (DESIGN.md). Run with --gpus 1.", flush=True)
         return 1
     threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    if args.only == "protein":
+        print(json.dumps({"protein": bench_protein(args.steps, args.warmup)}), flush=True)
+        return 0
+    if args.only == "candidates":
+        print(json.dumps({"candidate_batch": bench_candidates(args.candidates, args.steps, args.warmup)}),
+              flush=True)
+        return 0
 
     from netrax_amd.simulate import workload_50taxa_50k
 

@@ -51,6 +51,22 @@ __device__ __forceinline__ T *dyn_lds() {
   return reinterpret_cast<T *>(lds_dyn);
 }
 
+// Pointers read from a device-side descriptor table are generic (flat)
+// pointers to the compiler; kernels that walk such tables retype them as
+// global-memory pointers so CLV traffic uses global (not flat) loads and
+// stores with scalar base addresses.
+// (The host pass of this file never runs these kernels and has no address
+// spaces, so the qualifier exists in the device pass only.)
+#ifdef __HIP_DEVICE_COMPILE__
+#define GAS __attribute__((address_space(1)))
+#else
+#define GAS
+#endif
+template <typename T>
+__device__ __forceinline__ GAS T *to_global(T *p) {
+  return (GAS T *)p;
+}
+
 template <int C>
 __device__ __forceinline__ double cat_max(double m) {
 #pragma unroll
@@ -593,6 +609,20 @@ __device__ __forceinline__ void store4(float *__restrict__ p, const float (&x)[4
   *reinterpret_cast<float4 *>(p) = make_float4(x[0], x[1], x[2], x[3]);
 }
 
+#ifdef __HIP_DEVICE_COMPILE__
+__device__ __forceinline__ void load4(const GAS float *__restrict__ p, float (&x)[4]) {
+  const float4 v = *reinterpret_cast<const GAS float4 *>(p);
+  x[0] = v.x;
+  x[1] = v.y;
+  x[2] = v.z;
+  x[3] = v.w;
+}
+
+__device__ __forceinline__ void store4(GAS float *__restrict__ p, const float (&x)[4]) {
+  *reinterpret_cast<GAS float4 *>(p) = make_float4(x[0], x[1], x[2], x[3]);
+}
+#endif
+
 __device__ __forceinline__ void mv4(const float *__restrict__ M, const float (&x)[4], float (&y)[4]) {
 #pragma unroll
   for (int i = 0; i < 4; ++i)
@@ -608,29 +638,23 @@ __device__ __forceinline__ void mv4t(const float *__restrict__ M, const float (&
 // TS sites per workgroup (TS C threads, a multiple of 64, TS <= 64 so a
 // workgroup never straddles partitions) and G ops per group are the
 // autotuned shape parameters.
+// The bodies of the persistent kernels take every buffer as a __restrict__
+// parameter: restrict only reaches the optimiser through parameters, and it
+// lets the compiler keep op-table loads scalar and move CLV loads past
+// stores to other buffers.
 template <int C, int G, int TS>
-__global__ void __launch_bounds__(TS *C)
-    k_forward_persistent(const NetDesc *__restrict__ nets, const WorkItem *__restrict__ work) {
+__device__ __forceinline__ void forward_persistent_body(
+    const GAS DevOp *__restrict__ ops, const GAS int32_t *__restrict__ group_begin, int n_groups,
+    const DevLayout L, GAS float *__restrict__ clv, GAS int32_t *__restrict__ scl,
+    const GAS uint8_t *__restrict__ tips, const GAS float *__restrict__ Pf,
+    const GAS int32_t *__restrict__ tile_part, const GAS double *__restrict__ freqs,
+    const GAS double *__restrict__ weights, GAS double *__restrict__ root_partials, int tile) {
   float *sP = dyn_lds<float>();  // P of this tile's partition: [edge][cat][4][4]
   constexpr int PW = TILE * C / 64;  // site-sum partials (waves) per 64-site tile
-  const WorkItem wi = work[blockIdx.x];
-  const NetDesc &nd = nets[wi.net];
-  const DevOp *__restrict__ ops = nd.ops;
-  const int32_t *__restrict__ group_begin = nd.group_begin;
-  const int n_groups = nd.n_groups;
-  const DevLayout L = nd.L;
-  float *__restrict__ clv = nd.clv;
-  int32_t *__restrict__ scl = nd.scl;
-  const uint8_t *__restrict__ tips = nd.tips;
-  const float *__restrict__ Pf = nd.Pf;
-  const int32_t *__restrict__ tile_part = nd.tile_part;
-  const double *__restrict__ freqs = nd.freqs;
-  const double *__restrict__ weights = nd.weights;
-  double *__restrict__ root_partials = nd.root_partials;
-  const int64_t site0 = static_cast<int64_t>(wi.tile) * TS;
+  const int64_t site0 = static_cast<int64_t>(tile) * TS;
   const int part = tile_part[site0 / TILE];
   const int np = L.n_edges * C * 16;
-  const float *gP = Pf + static_cast<size_t>(part) * np;
+  const GAS float *gP = Pf + static_cast<size_t>(part) * np;
   for (int i = threadIdx.x; i < np; i += blockDim.x) sP[i] = gP[i];
   __syncthreads();
   const int c = threadIdx.x % C;
@@ -703,13 +727,24 @@ __global__ void __launch_bounds__(TS *C)
   }
 }
 
+template <int C, int G, int TS>
+__global__ void __launch_bounds__(TS *C)
+    k_forward_persistent(const NetDesc *__restrict__ nets, const WorkItem *__restrict__ work) {
+  const WorkItem wi = work[blockIdx.x];
+  const NetDesc &nd = nets[wi.net];
+  forward_persistent_body<C, G, TS>(to_global(nd.ops), to_global(nd.group_begin), nd.n_groups, nd.L,
+                                    to_global(nd.clv), to_global(nd.scl), to_global(nd.tips),
+                                    to_global(nd.Pf), to_global(nd.tile_part), to_global(nd.freqs),
+                                    to_global(nd.weights), to_global(nd.root_partials), wi.tile);
+}
+
 // Push-form reverse sweep (as k_adjoint_level) over the groups of the
 // forward order, backwards.  The adjoint of every CLV is stored as FP32
 // mantissas with a per-site exponent g: dJ/d(stored CLV) = stored * 2^g,
 // renormalised at every write; contributions are aligned to the larger
 // exponent before they are added, so no intermediate can overflow.
 template <int C>
-__device__ __forceinline__ void deliver_adjoint(float *__restrict__ dst_m, int32_t *__restrict__ dst_e,
+__device__ __forceinline__ void deliver_adjoint(GAS float *__restrict__ dst_m, GAS int32_t *__restrict__ dst_e,
                                                 bool first, bool lane0, float (&b)[4], int eb) {
   float mb = cat_maxf<C>(fmaxf(fmaxf(fabsf(b[0]), fabsf(b[1])), fmaxf(fabsf(b[2]), fabsf(b[3]))));
   if (!(mb > 0.f)) eb = EXP_EMPTY;
@@ -741,33 +776,21 @@ __device__ __forceinline__ void deliver_adjoint(float *__restrict__ dst_m, int32
 }
 
 template <int C, int G, int TS>
-__global__ void __launch_bounds__(TS *C)
-    k_adjoint_persistent(const NetDesc *__restrict__ nets, const WorkItem *__restrict__ work) {
+__device__ __forceinline__ void adjoint_persistent_body(
+    const GAS DevOp *__restrict__ ops, const GAS int32_t *__restrict__ group_begin, int n_groups,
+    const DevLayout L, const GAS float *__restrict__ clv, const GAS int32_t *__restrict__ scl,
+    GAS float *__restrict__ adj, GAS int32_t *__restrict__ adj_exp,
+    const GAS uint8_t *__restrict__ tips, const GAS float *__restrict__ Pf,
+    const GAS float *__restrict__ Qf, const GAS int32_t *__restrict__ tile_part,
+    const GAS double *__restrict__ freqs, const GAS double *__restrict__ weights,
+    const GAS double *__restrict__ beta, int n_root, GAS double *__restrict__ grad_partials,
+    int tile) {
   float *sP = dyn_lds<float>();  // [edge][cat][16] then Q: [cat][16]
   constexpr int PW = TILE * C / 64;
-  const WorkItem wi = work[blockIdx.x];
-  const NetDesc &nd = nets[wi.net];
-  const DevOp *__restrict__ ops = nd.ops;
-  const int32_t *__restrict__ group_begin = nd.group_begin;
-  const int n_groups = nd.n_groups;
-  const DevLayout L = nd.L;
-  const float *__restrict__ clv = nd.clv;
-  const int32_t *__restrict__ scl = nd.scl;
-  float *__restrict__ adj = nd.adj;
-  int32_t *__restrict__ adj_exp = nd.adj_exp;
-  const uint8_t *__restrict__ tips = nd.tips;
-  const float *__restrict__ Pf = nd.Pf;
-  const float *__restrict__ Qf = nd.Qf;
-  const int32_t *__restrict__ tile_part = nd.tile_part;
-  const double *__restrict__ freqs = nd.freqs;
-  const double *__restrict__ weights = nd.weights;
-  const double *__restrict__ beta = nd.beta;
-  const int n_root = nd.n_root;
-  double *__restrict__ grad_partials = nd.grad_partials;
-  const int64_t site0 = static_cast<int64_t>(wi.tile) * TS;
+  const int64_t site0 = static_cast<int64_t>(tile) * TS;
   const int part = tile_part[site0 / TILE];
   const int np = L.n_edges * C * 16;
-  const float *gP = Pf + static_cast<size_t>(part) * np;
+  const GAS float *gP = Pf + static_cast<size_t>(part) * np;
   for (int i = threadIdx.x; i < np; i += blockDim.x) sP[i] = gP[i];
   for (int i = threadIdx.x; i < C * 16; i += blockDim.x) sP[np + i] = Qf[part * C * 16 + i];
   __syncthreads();
@@ -878,6 +901,19 @@ __global__ void __launch_bounds__(TS *C)
   }
 }
 
+template <int C, int G, int TS>
+__global__ void __launch_bounds__(TS *C)
+    k_adjoint_persistent(const NetDesc *__restrict__ nets, const WorkItem *__restrict__ work) {
+  const WorkItem wi = work[blockIdx.x];
+  const NetDesc &nd = nets[wi.net];
+  adjoint_persistent_body<C, G, TS>(to_global(nd.ops), to_global(nd.group_begin), nd.n_groups, nd.L,
+                                    to_global(nd.clv), to_global(nd.scl), to_global(nd.adj),
+                                    to_global(nd.adj_exp), to_global(nd.tips), to_global(nd.Pf),
+                                    to_global(nd.Qf), to_global(nd.tile_part), to_global(nd.freqs),
+                                    to_global(nd.weights), to_global(nd.beta), nd.n_root,
+                                    to_global(nd.grad_partials), wi.tile);
+}
+
 // ============================================= FP32 form, 20 states (MFMA)
 // CLV x P runs on the matrix cores.  For one rate category c and a tile of
 // 32 sites, D = P_c X is a chain of 10 v_mfma_f32_32x32x2f32 (M = parent
@@ -910,8 +946,8 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__device__ __forceinline__ void load_a(const float *__restrict__ tab, int lane, float (&a)[AA_STEPS]) {
-  const float4 *p = reinterpret_cast<const float4 *>(tab + lane * (AA_TABLE / 64));
+__device__ __forceinline__ void load_a(const GAS float *__restrict__ tab, int lane, float (&a)[AA_STEPS]) {
+  const GAS float4 *p = reinterpret_cast<const GAS float4 *>(tab + lane * (AA_TABLE / 64));
   const float4 v0 = p[0], v1 = p[1], v2 = p[2];
   a[0] = v0.x; a[1] = v0.y; a[2] = v0.z; a[3] = v0.w;
   a[4] = v1.x; a[5] = v1.y; a[6] = v1.z; a[7] = v1.w;
@@ -920,7 +956,7 @@ __device__ __forceinline__ void load_a(const float *__restrict__ tab, int lane, 
 
 // B operand of a child: rows 2t + h of column col ([20][32] block of one
 // category), or the 0/1 vector of a tip's state mask
-__device__ __forceinline__ void load_b(const float *__restrict__ blk, int h, float (&b)[AA_STEPS]) {
+__device__ __forceinline__ void load_b(const GAS float *__restrict__ blk, int h, float (&b)[AA_STEPS]) {
 #pragma unroll
   for (int t = 0; t < AA_STEPS; ++t) b[t] = blk[(2 * t + h) * AA_TILE];
 }
@@ -959,31 +995,25 @@ __global__ void k_mfma_tables(DevLayout L, const double *__restrict__ P,
 }
 
 template <int C, int G>
-__global__ void __launch_bounds__(64 * C)
-    k_forward_mfma(const NetDesc *__restrict__ nets, const WorkItem *__restrict__ work) {
+__device__ __forceinline__ void forward_mfma_body(
+    const GAS DevOp *__restrict__ ops, const GAS int32_t *__restrict__ group_begin, int n_groups,
+    const DevLayout L, GAS float *__restrict__ clv, GAS int32_t *__restrict__ scl,
+    const GAS uint32_t *__restrict__ tips, const GAS double *__restrict__ freqs,
+    const GAS int32_t *__restrict__ tile_part, const GAS float *__restrict__ Pa_all,
+    const GAS double *__restrict__ weights, GAS double *__restrict__ root_partials,
+    int tile /* 32-site tile */) {
   __shared__ float smax[2][G][C][AA_TILE];
   __shared__ double ssum[2][G][C][AA_TILE];
-  const WorkItem wi = work[blockIdx.x];
-  const NetDesc &nd = nets[wi.net];
-  const DevOp *__restrict__ ops = nd.ops;
-  const int32_t *__restrict__ group_begin = nd.group_begin;
-  const int n_groups = nd.n_groups;
-  const DevLayout L = nd.L;
-  float *__restrict__ clv = nd.clv;
-  int32_t *__restrict__ scl = nd.scl;
-  const uint32_t *__restrict__ tips = reinterpret_cast<const uint32_t *>(nd.tips);
-  const double *__restrict__ freqs = nd.freqs;
-  const int tile = wi.tile;  // 32-site tile
-  const int part = nd.tile_part[tile >> 1];
+  const int part = tile_part[tile >> 1];
   const int c = threadIdx.x >> 6, lane = threadIdx.x & 63, h = lane >> 5, col = lane & 31;
   const int64_t n = L.n_sites_pad;
   const int64_t site = static_cast<int64_t>(tile) * AA_TILE + col;
   const size_t slot_stride = static_cast<size_t>(n) * C * AA_STATES;
   const size_t tile_off = static_cast<size_t>(tile) * (C * AA_STATES * AA_TILE) +
                           static_cast<size_t>(c) * (AA_STATES * AA_TILE) + col;
-  const float *__restrict__ Pa =
-      nd.Pa + static_cast<size_t>(part) * L.n_edges * C * AA_KINDS * AA_TABLE;
-  const double w = nd.weights[site];
+  const GAS float *__restrict__ Pa =
+      Pa_all + static_cast<size_t>(part) * L.n_edges * C * AA_KINDS * AA_TABLE;
+  const double w = weights[site];
   const size_t n_wt = static_cast<size_t>(L.n_tiles) * 2 * C;
   const size_t wt = static_cast<size_t>(tile) * C + c;
   for (int gi = 0; gi < n_groups; ++gi) {
@@ -1042,7 +1072,7 @@ __global__ void __launch_bounds__(64 * C)
         int e = 0;
         if (m > 0.f) (void)frexpf(m, &e);
         const int k = ks[j] - e;
-        float *dst = clv + op.out * slot_stride + tile_off;
+        GAS float *dst = clv + op.out * slot_stride + tile_off;
 #pragma unroll
         for (int r = 0; r < AA_ROWS; ++r)
           if (aa_valid(r, h)) dst[aa_row(r, h) * AA_TILE] = ldexpf(v[j][r], -e);
@@ -1054,16 +1084,28 @@ __global__ void __launch_bounds__(64 * C)
           t = ldexp(t, -e) * (1.0 / C);
           double val = (c == 0 && h == 0 && w != 0.0) ? w * (log(t) - k * LN2) : 0.0;
           val = wave_sum(val);
-          if (lane == 0) nd.root_partials[static_cast<size_t>(op.root) * n_wt + wt] = val;
+          if (lane == 0) root_partials[static_cast<size_t>(op.root) * n_wt + wt] = val;
         }
       }
     }
   }
 }
 
+template <int C, int G>
+__global__ void __launch_bounds__(64 * C)
+    k_forward_mfma(const NetDesc *__restrict__ nets, const WorkItem *__restrict__ work) {
+  const WorkItem wi = work[blockIdx.x];
+  const NetDesc &nd = nets[wi.net];
+  forward_mfma_body<C, G>(to_global(nd.ops), to_global(nd.group_begin), nd.n_groups, nd.L,
+                          to_global(nd.clv), to_global(nd.scl),
+                          to_global(reinterpret_cast<const uint32_t *>(nd.tips)), to_global(nd.freqs),
+                          to_global(nd.tile_part), to_global(nd.Pa), to_global(nd.weights),
+                          to_global(nd.root_partials), wi.tile);
+}
+
 // Adjoint delivery to a child (push form) with one exponent per (site,
 // category): lanes l and l ^ 32 hold the two halves of a site's states.
-__device__ __forceinline__ void deliver_adjoint_mfma(float *__restrict__ dst_m, int32_t *__restrict__ dst_e,
+__device__ __forceinline__ void deliver_adjoint_mfma(GAS float *__restrict__ dst_m, GAS int32_t *__restrict__ dst_e,
                                                      bool first, int h, float (&b)[AA_ROWS], int eb) {
   float mb = 0.0f;
 #pragma unroll
@@ -1108,24 +1150,17 @@ __device__ __forceinline__ void deliver_adjoint_mfma(float *__restrict__ dst_m, 
 // is turned from the D layout into a B operand through a wave-private LDS
 // block, and P_k^T z_k is delivered to the child.
 template <int C>
-__global__ void __launch_bounds__(64 * C)
-    k_adjoint_mfma(const NetDesc *__restrict__ nets, const WorkItem *__restrict__ work) {
+__device__ __forceinline__ void adjoint_mfma_body(
+    const GAS DevOp *__restrict__ ops, const GAS int32_t *__restrict__ group_begin, int n_groups,
+    const DevLayout L, const GAS float *__restrict__ clv, const GAS int32_t *__restrict__ scl,
+    GAS float *__restrict__ adj, GAS int32_t *__restrict__ adj_exp,
+    const GAS uint32_t *__restrict__ tips, const GAS double *__restrict__ freqs,
+    const GAS int32_t *__restrict__ tile_part, const GAS float *__restrict__ Pa_all,
+    const GAS double *__restrict__ weights, const GAS double *__restrict__ beta, int n_root,
+    GAS double *__restrict__ grad_partials, int tile) {
   __shared__ double rsum[2][C][AA_TILE];
   __shared__ float zb[C][AA_STATES * AA_TILE];
-  const WorkItem wi = work[blockIdx.x];
-  const NetDesc &nd = nets[wi.net];
-  const DevOp *__restrict__ ops = nd.ops;
-  const int32_t *__restrict__ group_begin = nd.group_begin;
-  const int n_groups = nd.n_groups;
-  const DevLayout L = nd.L;
-  const float *__restrict__ clv = nd.clv;
-  const int32_t *__restrict__ scl = nd.scl;
-  float *__restrict__ adj = nd.adj;
-  int32_t *__restrict__ adj_exp = nd.adj_exp;
-  const uint32_t *__restrict__ tips = reinterpret_cast<const uint32_t *>(nd.tips);
-  const double *__restrict__ freqs = nd.freqs;
-  const int tile = wi.tile;
-  const int part = nd.tile_part[tile >> 1];
+  const int part = tile_part[tile >> 1];
   const int c = threadIdx.x >> 6, lane = threadIdx.x & 63, h = lane >> 5, col = lane & 31;
   const int64_t n = L.n_sites_pad;
   const int64_t site = static_cast<int64_t>(tile) * AA_TILE + col;
@@ -1135,9 +1170,9 @@ __global__ void __launch_bounds__(64 * C)
   // adjoint exponents: [slot][tile][C][32]
   const size_t exp_stride = static_cast<size_t>(n) * C;
   const size_t exp_off = static_cast<size_t>(tile) * (C * AA_TILE) + static_cast<size_t>(c) * AA_TILE + col;
-  const float *__restrict__ Pa =
-      nd.Pa + static_cast<size_t>(part) * L.n_edges * C * AA_KINDS * AA_TABLE;
-  const double w = nd.weights[site];
+  const GAS float *__restrict__ Pa =
+      Pa_all + static_cast<size_t>(part) * L.n_edges * C * AA_KINDS * AA_TABLE;
+  const double w = weights[site];
   const size_t n_wt = static_cast<size_t>(L.n_tiles) * 2 * C;
   const size_t wt = static_cast<size_t>(tile) * C + c;
   float *zw = zb[c];
@@ -1151,7 +1186,7 @@ __global__ void __launch_bounds__(64 * C)
       float a[AA_ROWS];
       int ea = EXP_EMPTY;
       if (op.flags & OP_HAS_CONSUMERS) {
-        const float *src = adj + op.out * slot_stride + tile_off;
+        const GAS float *src = adj + op.out * slot_stride + tile_off;
 #pragma unroll
         for (int r = 0; r < AA_ROWS; ++r) a[r] = aa_valid(r, h) ? src[aa_row(r, h) * AA_TILE] : 0.0f;
         ea = adj_exp[op.out * exp_stride + exp_off];
@@ -1175,7 +1210,7 @@ __global__ void __launch_bounds__(64 * C)
       }
       if (op.root >= 0) {
         // seed beta * w * (1/C) pi_i / Ltilde, Ltilde = (1/C) sum_c pi . x_c
-        const float *xq = clv + op.out * slot_stride + tile_off;
+        const GAS float *xq = clv + op.out * slot_stride + tile_off;
         double s = 0.0;
 #pragma unroll
         for (int r = 0; r < AA_ROWS; ++r)
@@ -1189,7 +1224,7 @@ __global__ void __launch_bounds__(64 * C)
         for (int cc = 0; cc < C; ++cc) t += rsum[rb][cc][col];
         t *= 1.0 / C;
         rb ^= 1;
-        const double bw = nd.beta[part * nd.n_root + op.root] * w;
+        const double bw = beta[part * n_root + op.root] * w;
         int ef = EXP_EMPTY;
         float sd[AA_ROWS];
         double mf = 0.0;
@@ -1219,7 +1254,7 @@ __global__ void __launch_bounds__(64 * C)
 #pragma unroll
       for (int kk = 0; kk < 2; ++kk) {
         if (kk < op.nk) {
-          const float *tab = Pa + (static_cast<size_t>(op.edge[kk]) * C + c) * AA_KINDS * AA_TABLE;
+          const GAS float *tab = Pa + (static_cast<size_t>(op.edge[kk]) * C + c) * AA_KINDS * AA_TABLE;
           float z[AA_ROWS];
 #pragma unroll
           for (int r = 0; r < AA_ROWS; ++r) z[r] = op.nk == 2 ? a[r] * y[1 - kk][r] : a[r];
@@ -1230,7 +1265,7 @@ __global__ void __launch_bounds__(64 * C)
 #pragma unroll
           for (int r = 0; r < AA_ROWS; ++r) gd = fmaf(z[r], dy[r], gd);
           const double gv = wave_sum(ldexp(static_cast<double>(gd), ez));
-          if (lane == 0) nd.grad_partials[static_cast<size_t>(op.gslot[kk]) * n_wt + wt] = gv;
+          if (lane == 0) grad_partials[static_cast<size_t>(op.gslot[kk]) * n_wt + wt] = gv;
           const int32_t kid = op.kid[kk];
           if (kid >= 0) {
 #pragma unroll
@@ -1254,6 +1289,18 @@ __global__ void __launch_bounds__(64 * C)
       }
     }
   }
+}
+
+template <int C>
+__global__ void __launch_bounds__(64 * C)
+    k_adjoint_mfma(const NetDesc *__restrict__ nets, const WorkItem *__restrict__ work) {
+  const WorkItem wi = work[blockIdx.x];
+  const NetDesc &nd = nets[wi.net];
+  adjoint_mfma_body<C>(to_global(nd.ops), to_global(nd.group_begin), nd.n_groups, nd.L,
+                       to_global(nd.clv), to_global(nd.scl), to_global(nd.adj), to_global(nd.adj_exp),
+                       to_global(reinterpret_cast<const uint32_t *>(nd.tips)), to_global(nd.freqs),
+                       to_global(nd.tile_part), to_global(nd.Pa), to_global(nd.weights),
+                       to_global(nd.beta), nd.n_root, to_global(nd.grad_partials), wi.tile);
 }
 
 // ===================================================== branch derivatives
