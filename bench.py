@@ -46,37 +46,52 @@ SHAPE = None  # --shape G x TS: fixed FP32 kernel shape instead of autotuning
 
 
 def run_form(w, precision, steps, warmup, use_graph=False, tune=False):
-    """Times `steps` lnL+gradient evaluations; returns a result dict.  With
-    `tune` (FP32 form) the kernel shape is autotuned first, outside the
-    timed region (or set from --shape)."""
-    ann = make_engine(w, precision, use_graph=use_graph)
+    """Times `steps` lnL+gradient evaluations; returns a result dict.
+
+    Throughput (`evals_per_s`) is measured on an engine without timing
+    events; kernel durations for the roofline come from a second engine of
+    the same shape whose every pruning / adjoint launch is bracketed by HIP
+    events on its stream (events between the per-level launches of the plain
+    form would otherwise add their own gaps to its throughput).  With `tune`
+    (FP32 form) the kernel shape is autotuned first, outside the timed
+    region (or set from --shape)."""
+    ann = make_engine(w, precision, timing=False, use_graph=use_graph)
+    tim = make_engine(w, precision, timing=True)
     tuning = None
     if tune and SHAPE:
-        ann.set_tuning(*SHAPE)
+        shape = SHAPE
         tuning = {"group_width": SHAPE[0], "tile_sites": SHAPE[1], "fixed": True}
     elif tune:
-        best, cands = ann.autotune(reps=3)
+        best, cands = tim.autotune(reps=3)
+        shape = best[:2]
         tuning = {"group_width": best[0], "tile_sites": best[1],
                   "candidates_ms": {f"{g}x{t}": round(ms, 4) for g, t, ms in cands}}
         log(f"autotune: {best} of {cands}")
+    if tune:
+        ann.set_tuning(*shape)
+        tim.set_tuning(*shape)
     b0 = w.network.brlen.copy()
     b1 = b0 * (1.0 + 1e-7)
-    fwd_ms = adj_ms = tot_ms = 0.0
-    fwd_l = adj_l = 0
     for i in range(warmup):
-        ann.set_branch_lengths(b1 if i % 2 else b0)
-        ann.gradient()
+        for e in (ann, tim):
+            e.set_branch_lengths(b1 if i % 2 else b0)
+            e.gradient()
     t0 = time.perf_counter()
     for i in range(steps):
         ann.set_branch_lengths(b1 if (warmup + i) % 2 else b0)
         lnl, g, gp = ann.gradient()
-        t = ann.last_timing()
+    wall = time.perf_counter() - t0
+    fwd_ms = adj_ms = tot_ms = 0.0
+    fwd_l = adj_l = 0
+    for i in range(steps):
+        tim.set_branch_lengths(b1 if (warmup + i) % 2 else b0)
+        tim.gradient()
+        t = tim.last_timing()
         fwd_ms += t.fwd_ms
         adj_ms += t.adj_ms
         tot_ms += t.total_ms
         fwd_l += t.fwd_launches
         adj_l += t.adj_launches
-    wall = time.perf_counter() - t0
     st = ann.stats()
     ann.set_branch_lengths(b0)
     lnl, g, gp = ann.gradient()
@@ -91,6 +106,7 @@ def run_form(w, precision, steps, warmup, use_graph=False, tune=False):
         "lnl": lnl, "grad": g, "dprob": gp, "stats": st, "tuning": tuning,
     }
     ann.close()
+    tim.close()
     return res
 
 
@@ -391,6 +407,8 @@ def main():
         },
         "roofline": roofline(head),
         "device_ms_per_eval": round(head["device_ms_per_eval"], 4),
+        "timing_note": "value/ms_per_step: engine without timing events; device_ms and roofline "
+                       "kernel durations: same-shape engine with HIP events around every launch",
     }
     for name, r in forms.items():
         out[name] = {"evals_per_s": round(r["evals_per_s"], 3),

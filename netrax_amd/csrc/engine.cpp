@@ -134,6 +134,8 @@ static void upload_netdesc(Engine &e);
 
 // Sets OP_HAS_CONSUMERS / OP_FIRST_KID* for a list of ops walked by the
 // adjoint in the order given by `adj_order` (indices into ops).
+// Sets OP_HAS_CONSUMERS / OP_FIRST_KID* for a list of ops walked by the
+// adjoint in the order given by `adj_order` (indices into ops).
 static void set_adjoint_flags(std::vector<DevOp> &ops, const std::vector<int> &adj_order, int nv) {
   std::vector<char> has_cons(nv, 0), delivered(nv, 0);
   for (const DevOp &op : ops)
@@ -215,7 +217,6 @@ static void build_schedule(Engine &e) {
     for (int k = 0; k < 2; ++k) {
       op.kid[k] = k < v.nkids ? v.kid[k] : NO_CHILD;
       op.edge[k] = k < v.nkids ? v.edge[k] : 0;
-      op.ledge[k] = 0;
       op.gslot[k] = k < v.nkids ? gslot[s][k] : 0;
     }
     op.root = root_of_slot[s];
@@ -223,41 +224,18 @@ static void build_schedule(Engine &e) {
     return op;
   };
 
-  // plain form: level order, each level cut into launches whose distinct
-  // edges fit the LDS P table (4-state data; 20-state reads P from memory)
-  const int max_edges = e.S == 4 ? std::max(8, 49152 / (e.C * 16 * 8)) : 1 << 30;
+  // plain form: one launch per pruning level, ops in slot order
   e.ops_all.clear();
   e.chunks.clear();
-  e.chunk_edges.clear();
   for (int l = 1; l <= max_level; ++l) {
-    size_t i = 0;
     const auto &lv = e.level_ops[l];
-    while (i < lv.size()) {
-      LevelChunk ch{};
-      ch.level = l;
-      ch.op_begin = static_cast<int>(e.ops_all.size());
-      ch.edge_begin = static_cast<int>(e.chunk_edges.size());
-      std::vector<int> local(e.E, -1);
-      int n_local = 0;
-      for (; i < lv.size(); ++i) {
-        const Variant &v = e.dag.var[lv[i]];
-        int extra = 0;
-        for (int k = 0; k < v.nkids; ++k) extra += local[v.edge[k]] < 0;
-        if (n_local + extra > max_edges && ch.op_begin != static_cast<int>(e.ops_all.size())) break;
-        DevOp op = make_op(lv[i]);
-        for (int k = 0; k < v.nkids; ++k) {
-          if (local[v.edge[k]] < 0) {
-            local[v.edge[k]] = n_local++;
-            e.chunk_edges.push_back(v.edge[k]);
-          }
-          op.ledge[k] = local[v.edge[k]];
-        }
-        e.ops_all.push_back(op);
-      }
-      ch.op_end = static_cast<int>(e.ops_all.size());
-      ch.edge_end = static_cast<int>(e.chunk_edges.size());
-      e.chunks.push_back(ch);
-    }
+    if (lv.empty()) continue;
+    LevelChunk ch{};
+    ch.level = l;
+    ch.op_begin = static_cast<int>(e.ops_all.size());
+    for (int s : lv) e.ops_all.push_back(make_op(s));
+    ch.op_end = static_cast<int>(e.ops_all.size());
+    e.chunks.push_back(ch);
   }
   {
     std::vector<int> order;  // adjoint: chunks backwards, ops in index order
@@ -338,7 +316,6 @@ static void build_schedule(Engine &e) {
     dc[k].use = e.class_use[k];
   }
   upload(e.d_ops, e.ops_all, e.stream);
-  upload(e.d_chunk_edges, e.chunk_edges, e.stream);
   upload(e.d_ops_dfs, e.ops_dfs, e.stream);
   upload(e.d_group_begin, e.group_begin, e.stream);
   upload(e.d_edge_use_begin, e.edge_use_begin, e.stream);

@@ -77,7 +77,6 @@ struct DevOp {
   int32_t nk;
   int32_t kid[2];    // child codes
   int32_t edge[2];   // edge to each child
-  int32_t ledge[2];  // row of that edge in the launch's LDS P table
   int32_t root;      // index of the root-CLV site sum this op feeds, or -1
   int32_t flags;     // OP_HAS_CONSUMERS | OP_FIRST_KID0 | OP_FIRST_KID1
   int32_t gslot[2];  // gradient-partial row of (this op, child k)
@@ -86,11 +85,15 @@ constexpr int32_t OP_HAS_CONSUMERS = 1;  // adjoint of `out` was accumulated by 
 constexpr int32_t OP_FIRST_KID0 = 2;     // first consumer of kid[0] in adjoint order
 constexpr int32_t OP_FIRST_KID1 = 4;
 
-// A launch of the plain form: ops [op_begin, op_end) of one pruning level
-// whose edges are rows [edge_begin, edge_end) of the LDS edge table.
+// A launch of the plain form: the ops [op_begin, op_end) of one pruning
+// level.  The forward runs one workgroup per (tile, PLAIN_G consecutive
+// ops); the adjoint one workgroup per tile walking all ops in order.
 struct LevelChunk {
-  int level, op_begin, op_end, edge_begin, edge_end;
+  int level, op_begin, op_end;
 };
+constexpr int PLAIN_G = 4;  // forward ops per workgroup
+// adjoint: ops of a level whose P matrices are staged in LDS (48 KiB)
+constexpr int plain_staged_ops(int C) { return 49152 / (2 * C * 16 * 8); }
 
 // One step of a root-to-branch path: op slot and the child index taken.
 struct PathStep {

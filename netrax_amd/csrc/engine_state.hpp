@@ -73,7 +73,6 @@ struct Engine : nrx_engine {
   // plain form: ops in level order, cut into launches with bounded LDS tables
   std::vector<DevOp> ops_all;
   std::vector<LevelChunk> chunks;
-  std::vector<int32_t> chunk_edges;
   // batched form: groups of independent ops in a depth-first-biased order
   std::vector<DevOp> ops_dfs;
   std::vector<int32_t> group_begin;  // [n_groups + 1] into ops_dfs
@@ -114,7 +113,7 @@ struct Engine : nrx_engine {
   DevBuf d_P, d_dP, d_d2P, d_Pf, d_dPf;
   DevBuf d_clv, d_scl, d_adj, d_adj_exp, d_Qf;
   DevBuf d_Pa;  // 20-state FP32 form: matrix-core operand tables
-  DevBuf d_ops, d_ops_dirty, d_chunk_edges, d_Qd;
+  DevBuf d_ops, d_ops_dirty, d_Qd;
   DevBuf d_ops_dfs, d_group_begin, d_edge_use_begin, d_edge_use_list;
   DevBuf d_netdesc, d_work;  // FP32 form: this engine's NetDesc and (0, tile) work table
   DevBuf d_classes, d_class_dlogprob, d_root_slots;

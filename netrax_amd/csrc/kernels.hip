@@ -80,6 +80,13 @@ __device__ __forceinline__ double cat_sum(double m) {
   return m;
 }
 
+// Waits until every vector-memory load this wave issued has returned
+// (s_waitcnt vmcnt(0); expcnt and lgkmcnt left unconstrained).  Placed after
+// a load phase whose loads depend on runtime conditions, it leaves every
+// path with no pending load, so the compiler needs no conservative waits
+// before the next phase's loads.
+__device__ __forceinline__ void wait_loads() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
@@ -200,74 +207,104 @@ __global__ void k_pmatrices(DevLayout L, const double *__restrict__ V,
 }
 
 // ------------------------------------------------------- forward (FP64)
-// Plain form: one launch per pruning level (or per part of a level whose
-// edges fit the LDS table).  Workgroup b handles tile b and walks every op
-// of the launch for that tile, G mutually independent ops at a time: all
-// children of the G ops are loaded before any is combined, so each wave
-// keeps 2G CLV rows in flight.  For 4-state data the P matrices of the
-// launch's edges are staged once in LDS.
+// Plain form: one launch per pruning level, a 2-D grid of (tile, group of
+// ops of the level).  blockIdx.x is the tile and gridDim.x a multiple of 8,
+// so every workgroup of a tile lands on the same XCD (workgroup ids are
+// dealt round-robin to the 8 XCDs) and finds the children it reads, written
+// by the same tile in earlier levels, in that XCD's L2.  For 4-state data
+// the P matrices of the group's ops are staged in LDS first: row 2j + k
+// holds op j's child-k edge.
 template <int S, int C>
-__device__ __forceinline__ const double *p_row(const double *sP, const double *P, int part,
-                                               const DevLayout &L, int ledge, int edge, int c) {
-  if constexpr (S == 4)
-    return sP + (ledge * C + c) * (S * S);
-  else
-    return P + ((static_cast<size_t>(part) * L.n_edges + edge) * C + c) * (S * S);
-}
-
-template <int S, int C>
-__device__ __forceinline__ void stage_p(double *sP, const double *P, const int32_t *edges,
-                                        int n_le, int part, const DevLayout &L) {
+__device__ __forceinline__ void stage_ops_p(double *sP, const double *__restrict__ P,
+                                            const DevOp *__restrict__ ops, int n_staged, int part,
+                                            const DevLayout &L) {
   if constexpr (S == 4) {
     constexpr int ROW = C * S * S;
-    for (int i = threadIdx.x; i < n_le * ROW; i += blockDim.x) {
-      const int le = i / ROW, r = i % ROW;
-      sP[i] = P[(static_cast<size_t>(part) * L.n_edges + edges[le]) * ROW + r];
+    for (int i = threadIdx.x; i < n_staged * 2 * ROW; i += blockDim.x) {
+      const int slot = i / ROW, r = i % ROW;
+      const int j = slot >> 1, k = slot & 1;
+      if (k < ops[j].nk)
+        sP[i] = P[(static_cast<size_t>(part) * L.n_edges + ops[j].edge[k]) * ROW + r];
     }
     __syncthreads();
   }
 }
 
+// P row block of op j's child k for category c: staged in LDS, or read from
+// memory (20 states, or ops of an adjoint group beyond the staged ones)
+template <int S, int C>
+__device__ __forceinline__ const double *op_p(const double *sP, int n_staged,
+                                              const double *__restrict__ P, int part,
+                                              const DevLayout &L, int j, int k, int edge, int c) {
+  if constexpr (S == 4)
+    if (j < n_staged) return sP + ((j * 2 + k) * C + c) * (S * S);
+  return P + ((static_cast<size_t>(part) * L.n_edges + edge) * C + c) * (S * S);
+}
+
+// Workgroup (tile, g) computes ops [gG, gG + G) of the launch for one tile;
+// all children of its ops are loaded before any is combined.
 template <int S, int C, int G>
 __global__ void __launch_bounds__(TILE *C)
-    k_forward_level(const DevOp *__restrict__ ops, int n_ops, const int32_t *__restrict__ edges,
-                    int n_le, DevLayout L, double *__restrict__ clv, int32_t *__restrict__ scl,
+    k_forward_level(const DevOp *__restrict__ ops, int n_ops, DevLayout L,
+                    double *__restrict__ clv, int32_t *__restrict__ scl,
                     const typename TipType<S>::type *__restrict__ tips,
                     const double *__restrict__ P, const int32_t *__restrict__ tile_part) {
-  double *sP = dyn_lds<double>();
+  __shared__ double sP[S == 4 ? G * 2 * C * S * S : 1];
   const int tile = blockIdx.x;
+  if (tile >= L.n_tiles) return;  // grid padded to a multiple of 8 tiles
+  const int g0 = blockIdx.y * G;
+  const int nb = min(G, n_ops - g0);
   const int part = tile_part[tile];
-  stage_p<S, C>(sP, P, edges, n_le, part, L);
+  stage_ops_p<S, C>(sP, P, ops + g0, nb, part, L);
   const int c = threadIdx.x % C;
   const int64_t site = static_cast<int64_t>(tile) * TILE + threadIdx.x / C;
   const int64_t n = L.n_sites_pad;
-  for (int g0 = 0; g0 < n_ops; g0 += G) {
-    const int nb = min(G, n_ops - g0);
+  {
+    // load phase: raw values only (CLV rows, scaler words, tip masks), so no
+    // load has to wait for the ones issued before it
     double x[G][2][S];
-    int sc[G];
+    int sk[G][2];
+    uint32_t tm[G][2];
 #pragma unroll
     for (int j = 0; j < G; ++j) {
-      sc[j] = 0;
-      if (j < nb) {
-        const DevOp op = ops[g0 + j];
 #pragma unroll
-        for (int k = 0; k < 2; ++k) {
-          if (k < op.nk) {
-            const int32_t kid = op.kid[k];
-            if (kid >= 0) {
-              load_clv<S>(clv + (static_cast<size_t>(kid) * n + site) * (C * S) + c * S, x[j][k]);
-              sc[j] += scl[static_cast<size_t>(kid) * n + site];
-            } else {
-              tip_vec<S>(tips[static_cast<size_t>(-kid - 1) * n + site], x[j][k]);
-            }
+      for (int k = 0; k < 2; ++k) {
+        if (j < nb) {
+          const DevOp op = ops[g0 + j];
+          const int32_t kid = op.kid[k];
+          if (k < op.nk && kid >= 0) {
+            load_clv<S>(clv + (static_cast<size_t>(kid) * n + site) * (C * S) + c * S, x[j][k]);
+            sk[j][k] = scl[static_cast<size_t>(kid) * n + site];
           }
         }
       }
     }
 #pragma unroll
     for (int j = 0; j < G; ++j) {
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        if (j < nb) {
+          const DevOp op = ops[g0 + j];
+          const int32_t kid = op.kid[k];
+          if (k < op.nk && kid < 0) tm[j][k] = tips[static_cast<size_t>(-kid - 1) * n + site];
+        }
+      }
+    }
+    wait_loads();
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
       if (j < nb) {
         const DevOp op = ops[g0 + j];
+        int sc_j = 0;
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+          if (k < op.nk) {
+            if (op.kid[k] < 0)
+              tip_vec<S>(tm[j][k], x[j][k]);
+            else
+              sc_j += sk[j][k];
+          }
+        }
         double v[S];
 #pragma unroll
         for (int i = 0; i < S; ++i) v[i] = 1.0;
@@ -275,7 +312,7 @@ __global__ void __launch_bounds__(TILE *C)
         for (int k = 0; k < 2; ++k) {
           if (k < op.nk) {
             double y[S];
-            matvec<S>(p_row<S, C>(sP, P, part, L, op.ledge[k], op.edge[k], c), x[j][k], y);
+            matvec<S>(op_p<S, C>(sP, nb, P, part, L, j, k, op.edge[k], c), x[j][k], y);
 #pragma unroll
             for (int i = 0; i < S; ++i) v[i] *= y[i];
           }
@@ -284,7 +321,7 @@ __global__ void __launch_bounds__(TILE *C)
 #pragma unroll
         for (int i = 1; i < S; ++i) m = fmax(m, v[i]);
         m = cat_max<C>(m);
-        int scj = sc[j];
+        int scj = sc_j;
         if (m < SCALE_THRESHOLD) {
 #pragma unroll
           for (int i = 0; i < S; ++i) v[i] *= SCALE_FACTOR;
@@ -441,21 +478,71 @@ __global__ void __launch_bounds__(256)
 // own op runs, the first one writes and later ones accumulate.  The same
 // thread owns (site, category) in every op, so accumulation needs neither
 // atomics nor barriers and is deterministic.
+// One workgroup per tile walks the level's ops in order.  The inputs of op
+// j + 1 (its own adjoint, its children and their scalers) are loaded before
+// op j delivers: no input of an op of this level is a delivery target of the
+// level (deliveries go to lower levels; CLVs are read-only here), so the
+// loads overlap the previous op's arithmetic and stores, while deliveries to
+// a child shared by several ops of the level stay in order.  The P matrices
+// of the first n_staged ops are staged in LDS.
+template <int S>
+struct AdjIn {
+  double a[S];
+  double x[2][S];  // inner children; tips are expanded from tm at use
+  double xq[S];    // the op's own CLV, for the seed of a displayed-tree root
+  uint32_t tm[2];
+  int kq, sk[2];
+};
+
+template <int S, int C>
+__device__ __forceinline__ void adjoint_inputs(const DevOp &op, int64_t site, int64_t n, int c,
+                                               const double *__restrict__ clv,
+                                               const int32_t *__restrict__ scl,
+                                               const double *__restrict__ adj,
+                                               const typename TipType<S>::type *__restrict__ tips,
+                                               AdjIn<S> &in) {
+  // loads only (defaults for absent inputs are applied at use), so that no
+  // load waits for an earlier one
+  const size_t row = static_cast<size_t>(C * S);
+  in.kq = scl[static_cast<size_t>(op.out) * n + site];
+  if (op.flags & OP_HAS_CONSUMERS)
+    load_clv<S>(adj + (static_cast<size_t>(op.out) * n + site) * row + c * S, in.a);
+  if (op.root >= 0) load_clv<S>(clv + (static_cast<size_t>(op.out) * n + site) * row + c * S, in.xq);
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const int32_t kid = op.kid[k];
+    if (k < op.nk && kid >= 0) {
+      load_clv<S>(clv + (static_cast<size_t>(kid) * n + site) * row + c * S, in.x[k]);
+      in.sk[k] = scl[static_cast<size_t>(kid) * n + site];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const int32_t kid = op.kid[k];
+    if (k < op.nk && kid < 0) in.tm[k] = tips[static_cast<size_t>(-kid - 1) * n + site];
+  }
+}
+
 template <int S, int C>
 __global__ void __launch_bounds__(TILE *C)
-    k_adjoint_level(const DevOp *__restrict__ ops, int n_ops, const int32_t *__restrict__ edges,
-                    int n_le, DevLayout L, const double *__restrict__ clv,
-                    const int32_t *__restrict__ scl, double *__restrict__ adj,
-                    const typename TipType<S>::type *__restrict__ tips,
+    k_adjoint_level(const DevOp *__restrict__ ops, int n_ops, int n_staged, DevLayout L,
+                    const double *__restrict__ clv, const int32_t *__restrict__ scl,
+                    double *__restrict__ adj, const typename TipType<S>::type *__restrict__ tips,
                     const double *__restrict__ P, const double *__restrict__ Qd,
                     const double *__restrict__ freqs, const double *__restrict__ weights,
                     const int32_t *__restrict__ tile_part, const double *__restrict__ beta,
                     int n_root, double *__restrict__ grad_partials) {
-  double *sP = dyn_lds<double>();
+  // LDS: r_c Q of every category (4 states), then the staged P matrices
+  double *sQ = dyn_lds<double>();
+  double *sP = sQ + (S == 4 ? C * S * S : 0);
   constexpr int PW = TILE * C / 64;
   const int tile = blockIdx.x;
+  if (tile >= L.n_tiles) return;  // grid padded to a multiple of 8 tiles
   const int part = tile_part[tile];
-  stage_p<S, C>(sP, P, edges, n_le, part, L);
+  if constexpr (S == 4)
+    for (int i = threadIdx.x; i < C * S * S; i += blockDim.x)
+      sQ[i] = Qd[static_cast<size_t>(part) * C * S * S + i];
+  stage_ops_p<S, C>(sP, P, ops, n_staged, part, L);  // ends with a barrier (4 states)
   const int c = threadIdx.x % C;
   const int64_t site = static_cast<int64_t>(tile) * TILE + threadIdx.x / C;
   const int64_t n = L.n_sites_pad;
@@ -463,23 +550,40 @@ __global__ void __launch_bounds__(TILE *C)
   const size_t n_wt = static_cast<size_t>(L.n_tiles) * PW;
   const int wt = tile * PW + threadIdx.x / 64;
   const double w = weights[site];
-  const double *Qc = Qd + (static_cast<size_t>(part) * C + c) * (S * S);
+  const double *Qc = (S == 4 ? sQ : Qd + static_cast<size_t>(part) * C * S * S) + c * (S * S);
+  // 20-state operands are too large to hold two ops' worth in registers
+  constexpr bool PREFETCH = S == 4;
+  AdjIn<S> cur, nxt;
+  if constexpr (PREFETCH) {
+    adjoint_inputs<S, C>(ops[0], site, n, c, clv, scl, adj, tips, cur);
+    wait_loads();  // both ways into the loop then carry no pending load
+  }
   for (int j = 0; j < n_ops; ++j) {
     const DevOp op = ops[j];
-    double a[S];
-    const int kq = scl[static_cast<size_t>(op.out) * n + site];
-    if (op.flags & OP_HAS_CONSUMERS) {
-      load_clv<S>(adj + (static_cast<size_t>(op.out) * n + site) * row + c * S, a);
+    if constexpr (PREFETCH) {
+      if (j + 1 < n_ops) adjoint_inputs<S, C>(ops[j + 1], site, n, c, clv, scl, adj, tips, nxt);
     } else {
+      adjoint_inputs<S, C>(op, site, n, c, clv, scl, adj, tips, cur);
+    }
+    double a[S];
+    const bool has_adj = (op.flags & OP_HAS_CONSUMERS) != 0;
 #pragma unroll
-      for (int i = 0; i < S; ++i) a[i] = 0.0;
+    for (int i = 0; i < S; ++i) a[i] = has_adj ? cur.a[i] : 0.0;
+    int ks = 0;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      if (k < op.nk) {
+        if (op.kid[k] < 0)
+          tip_vec<S>(cur.tm[k], cur.x[k]);
+        else
+          ks += cur.sk[k];
+      }
     }
     if (op.root >= 0) {
-      double xq[S];
-      load_clv<S>(clv + (static_cast<size_t>(op.out) * n + site) * row + c * S, xq);
+      // seed beta * w * (1/C) pi_i / Ltilde at a displayed-tree root
       double t = 0.0;
 #pragma unroll
-      for (int i = 0; i < S; ++i) t = fma(freqs[part * S + i], xq[i], t);
+      for (int i = 0; i < S; ++i) t = fma(freqs[part * S + i], cur.xq[i], t);
       t = cat_sum<C>(t) * (1.0 / C);
       const double b = beta[part * n_root + op.root] * w;
       if (b != 0.0) {
@@ -488,36 +592,36 @@ __global__ void __launch_bounds__(TILE *C)
         for (int i = 0; i < S; ++i) a[i] = fma(f, freqs[part * S + i], a[i]);
       }
     }
-    double x[2][S], y[2][S];
-    int ksum = 0;
+    double y[2][S];
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+      if (k < op.nk) matvec<S>(op_p<S, C>(sP, n_staged, P, part, L, j, k, op.edge[k], c), cur.x[k], y[k]);
+    const double F = (cur.kq - ks) > 0 ? SCALE_FACTOR : 1.0;
+    // arithmetic of both children first, memory traffic after
+    double back[2][S], g[2];
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
       if (k < op.nk) {
-        const int32_t kid = op.kid[k];
-        if (kid >= 0) {
-          load_clv<S>(clv + (static_cast<size_t>(kid) * n + site) * row + c * S, x[k]);
-          ksum += scl[static_cast<size_t>(kid) * n + site];
-        } else {
-          tip_vec<S>(tips[static_cast<size_t>(-kid - 1) * n + site], x[k]);
-        }
-        matvec<S>(p_row<S, C>(sP, P, part, L, op.ledge[k], op.edge[k], c), x[k], y[k]);
-      }
-    }
-    const double F = (kq - ksum) > 0 ? SCALE_FACTOR : 1.0;
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-      if (k < op.nk) {
-        double z[S], back[S], dx[S];
+        double z[S], dx[S];
 #pragma unroll
         for (int i = 0; i < S; ++i) z[i] = a[i] * F * (op.nk == 2 ? y[1 - k][i] : 1.0);
-        matvec_t<S>(p_row<S, C>(sP, P, part, L, op.ledge[k], op.edge[k], c), z, back);
         matvec<S>(Qc, y[k], dx);
-        double g = 0.0;
+        double gk = 0.0;
 #pragma unroll
-        for (int i = 0; i < S; ++i) g = fma(z[i], dx[i], g);
-        g = wave_sum(g);
+        for (int i = 0; i < S; ++i) gk = fma(z[i], dx[i], gk);
+        g[k] = wave_sum(gk);
+        if (op.kid[k] >= 0)
+          matvec_t<S>(op_p<S, C>(sP, n_staged, P, part, L, j, k, op.edge[k], c), z, back[k]);
+      }
+    }
+    // op j + 1's loads have had op j's arithmetic to arrive; waiting here,
+    // before op j's stores, leaves those stores off the critical path
+    if constexpr (PREFETCH) wait_loads();
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      if (k < op.nk) {
         if ((threadIdx.x & 63) == 0)
-          grad_partials[static_cast<size_t>(op.gslot[k]) * n_wt + wt] = g;
+          grad_partials[static_cast<size_t>(op.gslot[k]) * n_wt + wt] = g[k];
         const int32_t kid = op.kid[k];
         if (kid >= 0) {
           double *dst = adj + (static_cast<size_t>(kid) * n + site) * row + c * S;
@@ -525,12 +629,13 @@ __global__ void __launch_bounds__(TILE *C)
             double acc[S];
             load_clv<S>(dst, acc);
 #pragma unroll
-            for (int i = 0; i < S; ++i) back[i] += acc[i];
+            for (int i = 0; i < S; ++i) back[k][i] += acc[i];
           }
-          store_clv<S>(dst, back);
+          store_clv<S>(dst, back[k]);
         }
       }
     }
+    if constexpr (PREFETCH) cur = nxt;
   }
 }
 
@@ -669,32 +774,54 @@ __device__ __forceinline__ void forward_persistent_body(
   for (int gi = 0; gi < n_groups; ++gi) {
     const int g0 = group_begin[gi];
     const int nb = group_begin[gi + 1] - g0;
+    // load every child of the group first, as raw values (rows, scaler
+    // words, tip masks): nothing in this phase waits for an earlier load
     float x[G][2][4];
-    int ks[G];
-    // load every child of the group first
+    int sk[G][2];
+    uint32_t tm[G][2];
+    // inner children in one pass, tip masks in another (no joins of a
+    // load path with a non-load path between the loads)
 #pragma unroll
     for (int j = 0; j < G; ++j) {
-      ks[j] = 0;
-      if (j < nb) {
-        const DevOp op = ops[g0 + j];
 #pragma unroll
-        for (int kk = 0; kk < 2; ++kk) {
-          if (kk < op.nk) {
-            const int32_t kid = op.kid[kk];
-            if (kid >= 0) {
-              load4(clv + (static_cast<size_t>(kid) * n + site) * (C * 4) + c * 4, x[j][kk]);
-              ks[j] += scl[static_cast<size_t>(kid) * n + site];
-            } else {
-              tip_vec4(tips[static_cast<size_t>(-kid - 1) * n + site], x[j][kk]);
-            }
+      for (int kk = 0; kk < 2; ++kk) {
+        if (j < nb) {
+          const DevOp op = ops[g0 + j];
+          const int32_t kid = op.kid[kk];
+          if (kk < op.nk && kid >= 0) {
+            load4(clv + (static_cast<size_t>(kid) * n + site) * (C * 4) + c * 4, x[j][kk]);
+            sk[j][kk] = scl[static_cast<size_t>(kid) * n + site];
           }
         }
       }
     }
 #pragma unroll
     for (int j = 0; j < G; ++j) {
+#pragma unroll
+      for (int kk = 0; kk < 2; ++kk) {
+        if (j < nb) {
+          const DevOp op = ops[g0 + j];
+          const int32_t kid = op.kid[kk];
+          if (kk < op.nk && kid < 0) tm[j][kk] = tips[static_cast<size_t>(-kid - 1) * n + site];
+        }
+      }
+    }
+    wait_loads();
+    int ks[G];
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
       if (j < nb) {
         const DevOp op = ops[g0 + j];
+        ks[j] = 0;
+#pragma unroll
+        for (int kk = 0; kk < 2; ++kk) {
+          if (kk < op.nk) {
+            if (op.kid[kk] < 0)
+              tip_vec4(tm[j][kk], x[j][kk]);
+            else
+              ks[j] += sk[j][kk];
+          }
+        }
         float v[4] = {1.f, 1.f, 1.f, 1.f};
 #pragma unroll
         for (int kk = 0; kk < 2; ++kk) {
@@ -809,49 +936,73 @@ __device__ __forceinline__ void adjoint_persistent_body(
   for (int gi = n_groups - 1; gi >= 0; --gi) {
     const int g0 = group_begin[gi];
     const int nb = group_begin[gi + 1] - g0;
-    float aq[G][4], x[G][2][4];
-    int gq[G], kq[G], ks[G];
-    // load the adjoints and children of the whole group first
+    // load the adjoints and children of the whole group first, as raw
+    // values: nothing in this phase waits for an earlier load
+    float aq[G][4], x[G][2][4], xq[G][4];
+    int gq[G], kq[G], sk[G][2];
+    uint32_t tm[G][2];
 #pragma unroll
     for (int j = 0; j < G; ++j) {
-      gq[j] = EXP_EMPTY;
-      kq[j] = ks[j] = 0;
       if (j < nb) {
         const DevOp op = ops[g0 + j];
         kq[j] = scl[static_cast<size_t>(op.out) * n + site];
         if (op.flags & OP_HAS_CONSUMERS) {
           load4(adj + (static_cast<size_t>(op.out) * n + site) * row + c * 4, aq[j]);
           gq[j] = adj_exp[static_cast<size_t>(op.out) * n + site];
-        } else {
-#pragma unroll
-          for (int i = 0; i < 4; ++i) aq[j][i] = 0.f;
         }
+        if (op.root >= 0) load4(clv + (static_cast<size_t>(op.out) * n + site) * row + c * 4, xq[j]);
+      }
+    }
 #pragma unroll
-        for (int kk = 0; kk < 2; ++kk) {
-          if (kk < op.nk) {
-            const int32_t kid = op.kid[kk];
-            if (kid >= 0) {
-              load4(clv + (static_cast<size_t>(kid) * n + site) * row + c * 4, x[j][kk]);
-              ks[j] += scl[static_cast<size_t>(kid) * n + site];
-            } else {
-              tip_vec4(tips[static_cast<size_t>(-kid - 1) * n + site], x[j][kk]);
-            }
+    for (int j = 0; j < G; ++j) {
+#pragma unroll
+      for (int kk = 0; kk < 2; ++kk) {
+        if (j < nb) {
+          const DevOp op = ops[g0 + j];
+          const int32_t kid = op.kid[kk];
+          if (kk < op.nk && kid >= 0) {
+            load4(clv + (static_cast<size_t>(kid) * n + site) * row + c * 4, x[j][kk]);
+            sk[j][kk] = scl[static_cast<size_t>(kid) * n + site];
           }
         }
       }
     }
 #pragma unroll
     for (int j = 0; j < G; ++j) {
+#pragma unroll
+      for (int kk = 0; kk < 2; ++kk) {
+        if (j < nb) {
+          const DevOp op = ops[g0 + j];
+          const int32_t kid = op.kid[kk];
+          if (kk < op.nk && kid < 0) tm[j][kk] = tips[static_cast<size_t>(-kid - 1) * n + site];
+        }
+      }
+    }
+    wait_loads();
+    int ks[G];
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
       if (j < nb) {
         const DevOp op = ops[g0 + j];
-        float a[4] = {aq[j][0], aq[j][1], aq[j][2], aq[j][3]};
-        int ea = gq[j];
+        ks[j] = 0;
+#pragma unroll
+        for (int kk = 0; kk < 2; ++kk) {
+          if (kk < op.nk) {
+            if (op.kid[kk] < 0)
+              tip_vec4(tm[j][kk], x[j][kk]);
+            else
+              ks[j] += sk[j][kk];
+          }
+        }
+        const bool has_adj = (op.flags & OP_HAS_CONSUMERS) != 0;
+        float a[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a[i] = has_adj ? aq[j][i] : 0.f;
+        int ea = has_adj ? gq[j] : EXP_EMPTY;
         if (op.root >= 0) {
-          float xq[4];
-          load4(clv + (static_cast<size_t>(op.out) * n + site) * row + c * 4, xq);
           double t = 0.0;
 #pragma unroll
-          for (int i = 0; i < 4; ++i) t = fma(pi[i], static_cast<double>(xq[i]), t);
+          for (int i = 0; i < 4; ++i) t = fma(pi[i], static_cast<double>(xq[j][i]), t);
           t = cat_sum<C>(t) * (1.0 / C);
           const double b = beta[part * n_root + op.root] * w;
           int ef = EXP_EMPTY;
@@ -1029,19 +1180,29 @@ __device__ __forceinline__ void forward_mfma_body(
         const DevOp op = ops[g0 + j];
 #pragma unroll
         for (int r = 0; r < AA_ROWS; ++r) v[j][r] = 1.0f;
+        // both children's operands as raw values first, then the chains
+        float a[2][AA_STEPS], b[2][AA_STEPS];
+        int sk[2] = {0, 0};
+        uint32_t tm[2];
 #pragma unroll
         for (int kk = 0; kk < 2; ++kk) {
           if (kk < op.nk) {
-            float a[AA_STEPS], b[AA_STEPS];
-            load_a(Pa + (static_cast<size_t>(op.edge[kk]) * C + c) * AA_KINDS * AA_TABLE, lane, a);
+            load_a(Pa + (static_cast<size_t>(op.edge[kk]) * C + c) * AA_KINDS * AA_TABLE, lane, a[kk]);
             const int32_t kid = op.kid[kk];
             if (kid >= 0) {
-              load_b(clv + kid * slot_stride + tile_off, h, b);
-              ks[j] += scl[static_cast<size_t>(kid) * n + site];
+              load_b(clv + kid * slot_stride + tile_off, h, b[kk]);
+              sk[kk] = scl[static_cast<size_t>(kid) * n + site];
             } else {
-              tip_b(tips[static_cast<size_t>(-kid - 1) * n + site], h, b);
+              tm[kk] = tips[static_cast<size_t>(-kid - 1) * n + site];
             }
-            const f32x16 d = mfma_chain(a, b);
+          }
+        }
+#pragma unroll
+        for (int kk = 0; kk < 2; ++kk) {
+          if (kk < op.nk) {
+            if (op.kid[kk] < 0) tip_b(tm[kk], h, b[kk]);
+            ks[j] += sk[kk];
+            const f32x16 d = mfma_chain(a[kk], b[kk]);
 #pragma unroll
             for (int r = 0; r < AA_ROWS; ++r) v[j][r] *= d[r];
           }
@@ -1195,19 +1356,24 @@ __device__ __forceinline__ void adjoint_mfma_body(
         for (int r = 0; r < AA_ROWS; ++r) a[r] = 0.0f;
       }
       float b[2][AA_STEPS];
-      int ks = 0;
+      int sk[2] = {0, 0};
+      uint32_t tm[2];
 #pragma unroll
       for (int kk = 0; kk < 2; ++kk) {
         if (kk < op.nk) {
           const int32_t kid = op.kid[kk];
           if (kid >= 0) {
             load_b(clv + kid * slot_stride + tile_off, h, b[kk]);
-            ks += scl[static_cast<size_t>(kid) * n + site];
+            sk[kk] = scl[static_cast<size_t>(kid) * n + site];
           } else {
-            tip_b(tips[static_cast<size_t>(-kid - 1) * n + site], h, b[kk]);
+            tm[kk] = tips[static_cast<size_t>(-kid - 1) * n + site];
           }
         }
       }
+#pragma unroll
+      for (int kk = 0; kk < 2; ++kk)
+        if (kk < op.nk && op.kid[kk] < 0) tip_b(tm[kk], h, b[kk]);
+      const int ks = sk[0] + sk[1];
       if (op.root >= 0) {
         // seed beta * w * (1/C) pi_i / Ltilde, Ltilde = (1/C) sum_c pi . x_c
         const GAS float *xq = clv + op.out * slot_stride + tile_off;
@@ -1595,18 +1761,16 @@ void launch_pmatrices(Engine &e) {
   NRX_HIP(hipGetLastError());
 }
 
-void launch_forward_level(Engine &e, const DevOp *ops, int n_ops, const LevelChunk &ch) {
+void launch_forward_level(Engine &e, const DevOp *ops, int n_ops, const LevelChunk &) {
   if (n_ops == 0) return;
-  const int n_le = ch.edge_end - ch.edge_begin;
   dispatch_sc(e.S, e.C, [&](auto s, auto c) {
     constexpr int S = decltype(s)::value, C = decltype(c)::value;
-    constexpr int G = S == 4 ? 4 : 1;
+    constexpr int G = S == 4 ? PLAIN_G : 1;
     using TT = typename TipType<S>::type;
-    const size_t lds = S == 4 ? sizeof(double) * n_le * C * S * S : 0;
-    hipLaunchKernelGGL((k_forward_level<S, C, G>), dim3(e.L.n_tiles), dim3(TILE * C), lds,
-                       e.stream, ops, n_ops, e.d_chunk_edges.as<int32_t>() + ch.edge_begin, n_le,
-                       e.L, e.d_clv.as<double>(), e.d_scl.as<int32_t>(), e.d_tips.as<TT>(),
-                       e.d_P.as<double>(), e.d_tile_part.as<int32_t>());
+    hipLaunchKernelGGL((k_forward_level<S, C, G>), dim3(e.L.n_tiles_grid, (n_ops + G - 1) / G),
+                       dim3(TILE * C), 0, e.stream, ops, n_ops, e.L, e.d_clv.as<double>(),
+                       e.d_scl.as<int32_t>(), e.d_tips.as<TT>(), e.d_P.as<double>(),
+                       e.d_tile_part.as<int32_t>());
   });
   NRX_HIP(hipGetLastError());
 }
@@ -1638,19 +1802,18 @@ void launch_mixture(Engine &e, int variant, int pw) {
 }
 
 void launch_adjoint_level(Engine &e, const LevelChunk &ch) {
-  const int n_ops = ch.op_end - ch.op_begin, n_le = ch.edge_end - ch.edge_begin;
+  const int n_ops = ch.op_end - ch.op_begin;
   if (n_ops == 0) return;
   dispatch_sc(e.S, e.C, [&](auto s, auto c) {
     constexpr int S = decltype(s)::value, C = decltype(c)::value;
     using TT = typename TipType<S>::type;
-    const size_t lds = S == 4 ? sizeof(double) * n_le * C * S * S : 0;
-    hipLaunchKernelGGL((k_adjoint_level<S, C>), dim3(e.L.n_tiles), dim3(TILE * C), lds, e.stream,
-                       e.d_ops.as<DevOp>() + ch.op_begin, n_ops,
-                       e.d_chunk_edges.as<int32_t>() + ch.edge_begin, n_le, e.L,
-                       e.d_clv.as<double>(), e.d_scl.as<int32_t>(), e.d_adj.as<double>(),
-                       e.d_tips.as<TT>(), e.d_P.as<double>(), e.d_Qd.as<double>(),
-                       e.d_freqs.as<double>(), e.d_weights.as<double>(),
-                       e.d_tile_part.as<int32_t>(), e.d_beta.as<double>(),
+    const int n_staged = S == 4 ? std::min(n_ops, plain_staged_ops(C)) : 0;
+    const size_t lds = S == 4 ? sizeof(double) * (n_staged * 2 + 1) * C * S * S : 0;
+    hipLaunchKernelGGL((k_adjoint_level<S, C>), dim3(e.L.n_tiles_grid), dim3(TILE * C), lds, e.stream,
+                       e.d_ops.as<DevOp>() + ch.op_begin, n_ops, n_staged, e.L, e.d_clv.as<double>(),
+                       e.d_scl.as<int32_t>(), e.d_adj.as<double>(), e.d_tips.as<TT>(),
+                       e.d_P.as<double>(), e.d_Qd.as<double>(), e.d_freqs.as<double>(),
+                       e.d_weights.as<double>(), e.d_tile_part.as<int32_t>(), e.d_beta.as<double>(),
                        static_cast<int>(e.root_slots.size()), e.d_grad_partials.as<double>());
   });
   NRX_HIP(hipGetLastError());
