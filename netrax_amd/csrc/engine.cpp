@@ -154,6 +154,34 @@ static void set_adjoint_flags(std::vector<DevOp> &ops, const std::vector<int> &a
   }
 }
 
+// OP_ACC_EARLY*: for the FP32 form's reverse sweep (groups backwards, ops in
+// index order), a delivery that accumulates into a child adjoint last
+// written at least two groups earlier (and not earlier in its own group), so
+// the read-back can be issued with the group's other loads even while the
+// group before it is still delivering.
+static void set_acc_early(std::vector<DevOp> &ops, const std::vector<int32_t> &group_begin, int nv) {
+  std::vector<int> last(nv, -1);  // sweep position of the group that last delivered to a slot
+  const int ng = static_cast<int>(group_begin.size()) - 1;
+  std::vector<int32_t> mine;
+  for (int g = ng - 1, t = 0; g >= 0; --g, ++t) {
+    mine.clear();
+    for (int i = group_begin[g]; i < group_begin[g + 1]; ++i) {
+      DevOp &op = ops[i];
+      op.flags &= ~(OP_ACC_EARLY0 | OP_ACC_EARLY1);
+      for (int k = 0; k < op.nk; ++k) {
+        const int32_t kid = op.kid[k];
+        if (kid < 0) continue;
+        const bool first = (op.flags & (k == 0 ? OP_FIRST_KID0 : OP_FIRST_KID1)) != 0;
+        const bool in_group = std::find(mine.begin(), mine.end(), kid) != mine.end();
+        if (!first && !in_group && last[kid] >= 0 && t - last[kid] >= 2)
+          op.flags |= k == 0 ? OP_ACC_EARLY0 : OP_ACC_EARLY1;
+        mine.push_back(kid);
+      }
+    }
+    for (int32_t kid : mine) last[kid] = t;
+  }
+}
+
 // Builds the level schedule, the batched order, root CLV set and gradient
 // rows for the classes that pass the log-probability cut, and uploads them.
 static void build_schedule(Engine &e) {
@@ -283,23 +311,36 @@ static void build_schedule(Engine &e) {
           deps[s]++;
           parents[e.dag.var[s].kid[k]].push_back(s);
         }
-    std::vector<int> ready;  // kept sorted by post-order index
+    // `ready` holds ops whose inputs all come from groups before the last
+    // one formed, `fresh` those the last group made ready.  A group is taken
+    // from `ready` when possible, so that it does not consume the group
+    // before it (the kernels then overlap the two groups' memory traffic);
+    // only when `ready` is empty does a group take `fresh` ops.
+    auto by_pos = [&](int a, int b) { return pos[a] < pos[b]; };
+    std::vector<int> ready, fresh;  // both kept sorted by post-order index
     for (int s : post)
       if (deps[s] == 0) ready.push_back(s);
-    auto by_pos = [&](int a, int b) { return pos[a] < pos[b]; };
     std::sort(ready.begin(), ready.end(), by_pos);
+    std::vector<char> group_indep;
     e.ops_dfs.clear();
     e.group_begin.assign(1, 0);
-    while (!ready.empty()) {
-      const int take = std::min<int>(GROUP, static_cast<int>(ready.size()));
-      std::vector<int> grp(ready.begin(), ready.begin() + take);
-      ready.erase(ready.begin(), ready.begin() + take);
+    while (!ready.empty() || !fresh.empty()) {
+      const bool indep = !ready.empty();
+      std::vector<int> &src = indep ? ready : fresh;
+      const int take = std::min<int>(GROUP, static_cast<int>(src.size()));
+      std::vector<int> grp(src.begin(), src.begin() + take);
+      src.erase(src.begin(), src.begin() + take);
+      // what the previous group made ready no longer depends on the group
+      // after this one
+      for (int s : fresh) ready.insert(std::upper_bound(ready.begin(), ready.end(), s, by_pos), s);
+      fresh.clear();
       for (int s : grp) {
         e.ops_dfs.push_back(make_op(s));
         for (int par : parents[s])
-          if (--deps[par] == 0)
-            ready.insert(std::upper_bound(ready.begin(), ready.end(), par, by_pos), par);
+          if (--deps[par] == 0) fresh.push_back(par);
       }
+      std::sort(fresh.begin(), fresh.end(), by_pos);
+      group_indep.push_back(indep && !group_indep.empty());
       e.group_begin.push_back(static_cast<int32_t>(e.ops_dfs.size()));
     }
     if (e.ops_dfs.size() != post.size()) throw std::logic_error("op grouping lost ops");
@@ -307,6 +348,9 @@ static void build_schedule(Engine &e) {
     for (int g = static_cast<int>(e.group_begin.size()) - 2; g >= 0; --g)
       for (int i = e.group_begin[g]; i < e.group_begin[g + 1]; ++i) order.push_back(i);
     set_adjoint_flags(e.ops_dfs, order, nv);
+    for (size_t g = 0; g < group_indep.size(); ++g)
+      if (group_indep[g]) e.ops_dfs[e.group_begin[g]].flags |= OP_GROUP_INDEP;
+    set_acc_early(e.ops_dfs, e.group_begin, nv);
   }
 
   std::vector<DevClass> dc(nc);

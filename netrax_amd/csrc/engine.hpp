@@ -84,6 +84,16 @@ struct DevOp {
 constexpr int32_t OP_HAS_CONSUMERS = 1;  // adjoint of `out` was accumulated by consumers
 constexpr int32_t OP_FIRST_KID0 = 2;     // first consumer of kid[0] in adjoint order
 constexpr int32_t OP_FIRST_KID1 = 4;
+// FP32 form, first op of a group: no op of the group consumes a CLV of the
+// group before it, so a kernel may load this group's inputs before the
+// previous group's stores (and, walking backwards, load the previous
+// group's inputs before this group delivers)
+constexpr int32_t OP_GROUP_INDEP = 8;
+// FP32 form: child k's adjoint already holds deliveries made at least two
+// groups earlier in the reverse sweep and none since, so the accumulated
+// value may be loaded with the group's other inputs
+constexpr int32_t OP_ACC_EARLY0 = 16;
+constexpr int32_t OP_ACC_EARLY1 = 32;
 
 // A launch of the plain form: the ops [op_begin, op_end) of one pruning
 // level.  The forward runs one workgroup per (tile, PLAIN_G consecutive

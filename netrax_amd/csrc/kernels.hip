@@ -59,24 +59,62 @@ __device__ __forceinline__ T *dyn_lds() {
 // spaces, so the qualifier exists in the device pass only.)
 #ifdef __HIP_DEVICE_COMPILE__
 #define GAS __attribute__((address_space(1)))
+#define CAS __attribute__((address_space(4)))
 #else
 #define GAS
+#define CAS
 #endif
 template <typename T>
 __device__ __forceinline__ GAS T *to_global(T *p) {
   return (GAS T *)p;
 }
+// Op tables and group bounds never change during a launch: read through
+// the constant address space they are fetched with scalar loads, whatever
+// stores the kernel makes in between.
+template <typename T>
+__device__ __forceinline__ const CAS T *to_constant(const T *p) {
+  return (const CAS T *)p;
+}
+
+// Cross-lane moves by DPP (a VALU operand modifier, no LDS round trip).
+// Controls: quad_perm(p0, p1, p2, p3) = p0 | p1 << 2 | p2 << 4 | p3 << 6
+// within each group of 4 lanes; row_shr:n = 0x110 + n shifts up by n lanes
+// within each row of 16.  Lanes whose source lies outside the row read 0.
+constexpr int DPP_XOR1 = 0xB1;  // quad_perm(1, 0, 3, 2): lane ^ 1
+constexpr int DPP_XOR2 = 0x4E;  // quad_perm(2, 3, 0, 1): lane ^ 2
+template <int CTRL>
+__device__ __forceinline__ float dpp(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+// value of lane `off` apart within the C adjacent lanes of a site
+template <int OFF, typename T>
+__device__ __forceinline__ T lane_xor(T v) {
+  if constexpr (OFF == 1)
+    return dpp<DPP_XOR1>(v);
+  else if constexpr (OFF == 2)
+    return dpp<DPP_XOR2>(v);
+  else
+    return __shfl_xor(v, OFF);
+}
 
 template <int C>
 __device__ __forceinline__ double cat_max(double m) {
-#pragma unroll
-  for (int off = 1; off < C; off <<= 1) m = fmax(m, __shfl_xor(m, off));
+  if constexpr (C > 1) m = fmax(m, lane_xor<1>(m));
+  if constexpr (C > 2) m = fmax(m, lane_xor<2>(m));
+  if constexpr (C > 4) m = fmax(m, lane_xor<4>(m));
   return m;
 }
 template <int C>
 __device__ __forceinline__ double cat_sum(double m) {
-#pragma unroll
-  for (int off = 1; off < C; off <<= 1) m += __shfl_xor(m, off);
+  if constexpr (C > 1) m += lane_xor<1>(m);
+  if constexpr (C > 2) m += lane_xor<2>(m);
+  if constexpr (C > 4) m += lane_xor<4>(m);
   return m;
 }
 
@@ -87,10 +125,20 @@ __device__ __forceinline__ double cat_sum(double m) {
 // before the next phase's loads.
 __device__ __forceinline__ void wait_loads() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
+__device__ __forceinline__ double readlane(double v, int lane) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
+                          __builtin_amdgcn_readlane(__double2loint(v), lane));
+}
+
+// Sum over the 64 lanes of a wave, in a fixed order (deterministic); the
+// result is the same in every lane.  Row sums by four row_shr steps (lane
+// 15 of each row of 16 ends with its row's sum), then the four row sums.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
-  return v;
+  v += dpp<0x111>(v);
+  v += dpp<0x112>(v);
+  v += dpp<0x114>(v);
+  v += dpp<0x118>(v);
+  return (readlane(v, 15) + readlane(v, 31)) + (readlane(v, 47) + readlane(v, 63));
 }
 
 // Deterministic workgroup sum; result valid in thread 0.
@@ -692,8 +740,9 @@ constexpr int EXP_EMPTY = -(1 << 28);  // exponent of an all-zero adjoint
 
 template <int C>
 __device__ __forceinline__ float cat_maxf(float m) {
-#pragma unroll
-  for (int off = 1; off < C; off <<= 1) m = fmaxf(m, __shfl_xor(m, off));
+  if constexpr (C > 1) m = fmaxf(m, lane_xor<1>(m));
+  if constexpr (C > 2) m = fmaxf(m, lane_xor<2>(m));
+  if constexpr (C > 4) m = fmaxf(m, lane_xor<4>(m));
   return m;
 }
 
@@ -740,6 +789,51 @@ __device__ __forceinline__ void mv4t(const float *__restrict__ M, const float (&
     y[j] = fmaf(M[12 + j], x[3], fmaf(M[8 + j], x[2], fmaf(M[4 + j], x[1], M[j] * x[0])));
 }
 
+// Raw inputs of one group of the 4-state FP32 forward: child rows, child
+// scaler words and tip masks (expanded when used).
+template <int G>
+struct FwdIn {
+  float x[G][2][4];
+  int sk[G][2];
+  uint32_t tm[G][2];
+};
+
+// Issues every load of group [g0, g0 + nb): inner children in one pass, tip
+// masks in another (no join of a load path with a non-load path between two
+// loads), nothing waits.
+template <int C, int G>
+__device__ __forceinline__ void fwd_load(const CAS DevOp *__restrict__ ops, int g0, int nb,
+                                         const GAS float *__restrict__ clv,
+                                         const GAS int32_t *__restrict__ scl,
+                                         const GAS uint8_t *__restrict__ tips, int64_t n,
+                                         int64_t site, int c, FwdIn<G> &in) {
+#pragma unroll
+  for (int j = 0; j < G; ++j) {
+#pragma unroll
+    for (int kk = 0; kk < 2; ++kk) {
+      if (j < nb) {
+        const DevOp op = ops[g0 + j];
+        const int32_t kid = op.kid[kk];
+        if (kk < op.nk && kid >= 0) {
+          load4(clv + (static_cast<size_t>(kid) * n + site) * (C * 4) + c * 4, in.x[j][kk]);
+          in.sk[j][kk] = scl[static_cast<size_t>(kid) * n + site];
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < G; ++j) {
+#pragma unroll
+    for (int kk = 0; kk < 2; ++kk) {
+      if (j < nb) {
+        const DevOp op = ops[g0 + j];
+        const int32_t kid = op.kid[kk];
+        if (kk < op.nk && kid < 0) in.tm[j][kk] = tips[static_cast<size_t>(-kid - 1) * n + site];
+      }
+    }
+  }
+}
+
 // TS sites per workgroup (TS C threads, a multiple of 64, TS <= 64 so a
 // workgroup never straddles partitions) and G ops per group are the
 // autotuned shape parameters.
@@ -749,7 +843,7 @@ __device__ __forceinline__ void mv4t(const float *__restrict__ M, const float (&
 // stores to other buffers.
 template <int C, int G, int TS>
 __device__ __forceinline__ void forward_persistent_body(
-    const GAS DevOp *__restrict__ ops, const GAS int32_t *__restrict__ group_begin, int n_groups,
+    const CAS DevOp *__restrict__ ops, const CAS int32_t *__restrict__ group_begin, int n_groups,
     const DevLayout L, GAS float *__restrict__ clv, GAS int32_t *__restrict__ scl,
     const GAS uint8_t *__restrict__ tips, const GAS float *__restrict__ Pf,
     const GAS int32_t *__restrict__ tile_part, const GAS double *__restrict__ freqs,
@@ -771,85 +865,88 @@ __device__ __forceinline__ void forward_persistent_body(
   double pi[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) pi[i] = freqs[part * 4 + i];
+  // The inputs of a group are loaded as raw values (rows, scaler words, tip
+  // masks) in one phase with no wait inside it.  When the next group does
+  // not consume this one (OP_GROUP_INDEP on its first op), its loads are
+  // issued before this group's arithmetic and stores, so the two groups'
+  // memory round trips overlap.
+  FwdIn<G> cur, nxt;
+  fwd_load<C, G>(ops, group_begin[0], group_begin[1] - group_begin[0], clv, scl, tips, n, site, c, cur);
+  wait_loads();
   for (int gi = 0; gi < n_groups; ++gi) {
     const int g0 = group_begin[gi];
     const int nb = group_begin[gi + 1] - g0;
-    // load every child of the group first, as raw values (rows, scaler
-    // words, tip masks): nothing in this phase waits for an earlier load
-    float x[G][2][4];
-    int sk[G][2];
-    uint32_t tm[G][2];
-    // inner children in one pass, tip masks in another (no joins of a
-    // load path with a non-load path between the loads)
-#pragma unroll
-    for (int j = 0; j < G; ++j) {
-#pragma unroll
-      for (int kk = 0; kk < 2; ++kk) {
-        if (j < nb) {
-          const DevOp op = ops[g0 + j];
-          const int32_t kid = op.kid[kk];
-          if (kk < op.nk && kid >= 0) {
-            load4(clv + (static_cast<size_t>(kid) * n + site) * (C * 4) + c * 4, x[j][kk]);
-            sk[j][kk] = scl[static_cast<size_t>(kid) * n + site];
-          }
-        }
-      }
+    const bool has_next = gi + 1 < n_groups;
+    int n0 = 0, nn = 0;
+    bool early = false;
+    if (has_next) {
+      n0 = group_begin[gi + 1];
+      nn = group_begin[gi + 2] - n0;
+      early = (ops[n0].flags & OP_GROUP_INDEP) != 0;
     }
-#pragma unroll
-    for (int j = 0; j < G; ++j) {
-#pragma unroll
-      for (int kk = 0; kk < 2; ++kk) {
-        if (j < nb) {
-          const DevOp op = ops[g0 + j];
-          const int32_t kid = op.kid[kk];
-          if (kk < op.nk && kid < 0) tm[j][kk] = tips[static_cast<size_t>(-kid - 1) * n + site];
-        }
-      }
-    }
-    wait_loads();
-    int ks[G];
+    if (early) fwd_load<C, G>(ops, n0, nn, clv, scl, tips, n, site, c, nxt);
+    float v[G][4];
+    int kk_[G];
+    double rv[G];
 #pragma unroll
     for (int j = 0; j < G; ++j) {
       if (j < nb) {
         const DevOp op = ops[g0 + j];
-        ks[j] = 0;
+        int ks = 0;
 #pragma unroll
         for (int kk = 0; kk < 2; ++kk) {
           if (kk < op.nk) {
             if (op.kid[kk] < 0)
-              tip_vec4(tm[j][kk], x[j][kk]);
+              tip_vec4(cur.tm[j][kk], cur.x[j][kk]);
             else
-              ks[j] += sk[j][kk];
+              ks += cur.sk[j][kk];
           }
         }
-        float v[4] = {1.f, 1.f, 1.f, 1.f};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v[j][i] = 1.f;
 #pragma unroll
         for (int kk = 0; kk < 2; ++kk) {
           if (kk < op.nk) {
             float y[4];
-            mv4(sP + (op.edge[kk] * C + c) * 16, x[j][kk], y);
+            mv4(sP + (op.edge[kk] * C + c) * 16, cur.x[j][kk], y);
 #pragma unroll
-            for (int i = 0; i < 4; ++i) v[i] *= y[i];
+            for (int i = 0; i < 4; ++i) v[j][i] *= y[i];
           }
         }
-        const float m = cat_maxf<C>(fmaxf(fmaxf(v[0], v[1]), fmaxf(v[2], v[3])));
+        const float m = cat_maxf<C>(fmaxf(fmaxf(v[j][0], v[j][1]), fmaxf(v[j][2], v[j][3])));
         int e = 0;
         if (m > 0.f) (void)frexpf(m, &e);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) v[i] = ldexpf(v[i], -e);
-        const int k = ks[j] - e;
-        store4(clv + (static_cast<size_t>(op.out) * n + site) * (C * 4) + c * 4, v);
-        if (c == 0) scl[static_cast<size_t>(op.out) * n + site] = k;
+        for (int i = 0; i < 4; ++i) v[j][i] = ldexpf(v[j][i], -e);
+        kk_[j] = ks - e;
+        rv[j] = 0.0;
         if (op.root >= 0) {
           double t = 0.0;
 #pragma unroll
-          for (int i = 0; i < 4; ++i) t = fma(pi[i], static_cast<double>(v[i]), t);
+          for (int i = 0; i < 4; ++i) t = fma(pi[i], static_cast<double>(v[j][i]), t);
           t = cat_sum<C>(t) * (1.0 / C);
-          double val = (c == 0 && w != 0.0) ? w * (log(t) - k * LN2) : 0.0;
-          val = wave_sum(val);
-          if ((threadIdx.x & 63) == 0) root_partials[static_cast<size_t>(op.root) * n_wt + wt] = val;
+          const double val = (c == 0 && w != 0.0) ? w * (log(t) - kk_[j] * LN2) : 0.0;
+          rv[j] = wave_sum(val);
         }
       }
+    }
+    if (early) wait_loads();  // the next group's inputs, before this group's stores
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+      if (j < nb) {
+        const DevOp op = ops[g0 + j];
+        store4(clv + (static_cast<size_t>(op.out) * n + site) * (C * 4) + c * 4, v[j]);
+        if (c == 0) scl[static_cast<size_t>(op.out) * n + site] = kk_[j];
+        if (op.root >= 0 && (threadIdx.x & 63) == 0)
+          root_partials[static_cast<size_t>(op.root) * n_wt + wt] = rv[j];
+      }
+    }
+    if (has_next) {
+      if (!early) {
+        fwd_load<C, G>(ops, n0, nn, clv, scl, tips, n, site, c, nxt);
+        wait_loads();
+      }
+      cur = nxt;
     }
   }
 }
@@ -859,7 +956,7 @@ __global__ void __launch_bounds__(TS *C)
     k_forward_persistent(const NetDesc *__restrict__ nets, const WorkItem *__restrict__ work) {
   const WorkItem wi = work[blockIdx.x];
   const NetDesc &nd = nets[wi.net];
-  forward_persistent_body<C, G, TS>(to_global(nd.ops), to_global(nd.group_begin), nd.n_groups, nd.L,
+  forward_persistent_body<C, G, TS>(to_constant(nd.ops), to_constant(nd.group_begin), nd.n_groups, nd.L,
                                     to_global(nd.clv), to_global(nd.scl), to_global(nd.tips),
                                     to_global(nd.Pf), to_global(nd.tile_part), to_global(nd.freqs),
                                     to_global(nd.weights), to_global(nd.root_partials), wi.tile);
@@ -870,15 +967,16 @@ __global__ void __launch_bounds__(TS *C)
 // mantissas with a per-site exponent g: dJ/d(stored CLV) = stored * 2^g,
 // renormalised at every write; contributions are aligned to the larger
 // exponent before they are added, so no intermediate can overflow.
+// Adds delivered vector b (adjoint exponent eb) to the accumulated value
+// (acc, ea) unless this is the child's first delivery, renormalises and
+// stores.
 template <int C>
-__device__ __forceinline__ void deliver_adjoint(GAS float *__restrict__ dst_m, GAS int32_t *__restrict__ dst_e,
-                                                bool first, bool lane0, float (&b)[4], int eb) {
+__device__ __forceinline__ void finish_delivery(GAS float *__restrict__ dst_m, GAS int32_t *__restrict__ dst_e,
+                                                bool first, bool lane0, float (&b)[4], int eb,
+                                                const float (&acc)[4], int ea) {
   float mb = cat_maxf<C>(fmaxf(fmaxf(fabsf(b[0]), fabsf(b[1])), fmaxf(fabsf(b[2]), fabsf(b[3]))));
   if (!(mb > 0.f)) eb = EXP_EMPTY;
   if (!first) {
-    float acc[4];
-    load4(dst_m, acc);
-    const int ea = *dst_e;
     if (ea > eb) {
 #pragma unroll
       for (int i = 0; i < 4; ++i) b[i] = acc[i] + ldexpf(b[i], eb - ea);
@@ -904,7 +1002,7 @@ __device__ __forceinline__ void deliver_adjoint(GAS float *__restrict__ dst_m, G
 
 template <int C, int G, int TS>
 __device__ __forceinline__ void adjoint_persistent_body(
-    const GAS DevOp *__restrict__ ops, const GAS int32_t *__restrict__ group_begin, int n_groups,
+    const CAS DevOp *__restrict__ ops, const CAS int32_t *__restrict__ group_begin, int n_groups,
     const DevLayout L, const GAS float *__restrict__ clv, const GAS int32_t *__restrict__ scl,
     GAS float *__restrict__ adj, GAS int32_t *__restrict__ adj_exp,
     const GAS uint8_t *__restrict__ tips, const GAS float *__restrict__ Pf,
@@ -938,8 +1036,8 @@ __device__ __forceinline__ void adjoint_persistent_body(
     const int nb = group_begin[gi + 1] - g0;
     // load the adjoints and children of the whole group first, as raw
     // values: nothing in this phase waits for an earlier load
-    float aq[G][4], x[G][2][4], xq[G][4];
-    int gq[G], kq[G], sk[G][2];
+    float aq[G][4], x[G][2][4], xq[G][4], acc[G][2][4];
+    int gq[G], kq[G], sk[G][2], eacc[G][2];
     uint32_t tm[G][2];
 #pragma unroll
     for (int j = 0; j < G; ++j) {
@@ -963,6 +1061,11 @@ __device__ __forceinline__ void adjoint_persistent_body(
           if (kk < op.nk && kid >= 0) {
             load4(clv + (static_cast<size_t>(kid) * n + site) * row + c * 4, x[j][kk]);
             sk[j][kk] = scl[static_cast<size_t>(kid) * n + site];
+            if (op.flags & (kk == 0 ? OP_ACC_EARLY0 : OP_ACC_EARLY1)) {
+              // the child's accumulated adjoint, last written two or more groups ago
+              load4(adj + (static_cast<size_t>(kid) * n + site) * row + c * 4, acc[j][kk]);
+              eacc[j][kk] = adj_exp[static_cast<size_t>(kid) * n + site];
+            }
           }
         }
       }
@@ -1040,11 +1143,19 @@ __device__ __forceinline__ void adjoint_persistent_body(
             if ((threadIdx.x & 63) == 0)
               grad_partials[static_cast<size_t>(op.gslot[kk]) * n_wt + wt] = gv;
             const int32_t kid = op.kid[kk];
-            if (kid >= 0)
-              deliver_adjoint<C>(adj + (static_cast<size_t>(kid) * n + site) * row + c * 4,
-                                 adj_exp + static_cast<size_t>(kid) * n + site,
-                                 (op.flags & (kk == 0 ? OP_FIRST_KID0 : OP_FIRST_KID1)) != 0,
-                                 lane0, back, ez);
+            if (kid >= 0) {
+              GAS float *dst_m = adj + (static_cast<size_t>(kid) * n + site) * row + c * 4;
+              GAS int32_t *dst_e = adj_exp + static_cast<size_t>(kid) * n + site;
+              const bool first = (op.flags & (kk == 0 ? OP_FIRST_KID0 : OP_FIRST_KID1)) != 0;
+              if (first || (op.flags & (kk == 0 ? OP_ACC_EARLY0 : OP_ACC_EARLY1))) {
+                finish_delivery<C>(dst_m, dst_e, first, lane0, back, ez, acc[j][kk], eacc[j][kk]);
+              } else {
+                // an earlier op of this group or the previous group wrote it: read it now
+                float acc_now[4];
+                load4(dst_m, acc_now);
+                finish_delivery<C>(dst_m, dst_e, false, lane0, back, ez, acc_now, *dst_e);
+              }
+            }
           }
         }
       }
@@ -1057,7 +1168,7 @@ __global__ void __launch_bounds__(TS *C)
     k_adjoint_persistent(const NetDesc *__restrict__ nets, const WorkItem *__restrict__ work) {
   const WorkItem wi = work[blockIdx.x];
   const NetDesc &nd = nets[wi.net];
-  adjoint_persistent_body<C, G, TS>(to_global(nd.ops), to_global(nd.group_begin), nd.n_groups, nd.L,
+  adjoint_persistent_body<C, G, TS>(to_constant(nd.ops), to_constant(nd.group_begin), nd.n_groups, nd.L,
                                     to_global(nd.clv), to_global(nd.scl), to_global(nd.adj),
                                     to_global(nd.adj_exp), to_global(nd.tips), to_global(nd.Pf),
                                     to_global(nd.Qf), to_global(nd.tile_part), to_global(nd.freqs),
@@ -1147,7 +1258,7 @@ __global__ void k_mfma_tables(DevLayout L, const double *__restrict__ P,
 
 template <int C, int G>
 __device__ __forceinline__ void forward_mfma_body(
-    const GAS DevOp *__restrict__ ops, const GAS int32_t *__restrict__ group_begin, int n_groups,
+    const CAS DevOp *__restrict__ ops, const CAS int32_t *__restrict__ group_begin, int n_groups,
     const DevLayout L, GAS float *__restrict__ clv, GAS int32_t *__restrict__ scl,
     const GAS uint32_t *__restrict__ tips, const GAS double *__restrict__ freqs,
     const GAS int32_t *__restrict__ tile_part, const GAS float *__restrict__ Pa_all,
@@ -1257,7 +1368,7 @@ __global__ void __launch_bounds__(64 * C)
     k_forward_mfma(const NetDesc *__restrict__ nets, const WorkItem *__restrict__ work) {
   const WorkItem wi = work[blockIdx.x];
   const NetDesc &nd = nets[wi.net];
-  forward_mfma_body<C, G>(to_global(nd.ops), to_global(nd.group_begin), nd.n_groups, nd.L,
+  forward_mfma_body<C, G>(to_constant(nd.ops), to_constant(nd.group_begin), nd.n_groups, nd.L,
                           to_global(nd.clv), to_global(nd.scl),
                           to_global(reinterpret_cast<const uint32_t *>(nd.tips)), to_global(nd.freqs),
                           to_global(nd.tile_part), to_global(nd.Pa), to_global(nd.weights),
@@ -1312,7 +1423,7 @@ __device__ __forceinline__ void deliver_adjoint_mfma(GAS float *__restrict__ dst
 // block, and P_k^T z_k is delivered to the child.
 template <int C>
 __device__ __forceinline__ void adjoint_mfma_body(
-    const GAS DevOp *__restrict__ ops, const GAS int32_t *__restrict__ group_begin, int n_groups,
+    const CAS DevOp *__restrict__ ops, const CAS int32_t *__restrict__ group_begin, int n_groups,
     const DevLayout L, const GAS float *__restrict__ clv, const GAS int32_t *__restrict__ scl,
     GAS float *__restrict__ adj, GAS int32_t *__restrict__ adj_exp,
     const GAS uint32_t *__restrict__ tips, const GAS double *__restrict__ freqs,
@@ -1462,7 +1573,7 @@ __global__ void __launch_bounds__(64 * C)
     k_adjoint_mfma(const NetDesc *__restrict__ nets, const WorkItem *__restrict__ work) {
   const WorkItem wi = work[blockIdx.x];
   const NetDesc &nd = nets[wi.net];
-  adjoint_mfma_body<C>(to_global(nd.ops), to_global(nd.group_begin), nd.n_groups, nd.L,
+  adjoint_mfma_body<C>(to_constant(nd.ops), to_constant(nd.group_begin), nd.n_groups, nd.L,
                        to_global(nd.clv), to_global(nd.scl), to_global(nd.adj), to_global(nd.adj_exp),
                        to_global(reinterpret_cast<const uint32_t *>(nd.tips)), to_global(nd.freqs),
                        to_global(nd.tile_part), to_global(nd.Pa), to_global(nd.weights),
