@@ -372,6 +372,7 @@ static void build_schedule(Engine &e) {
   e.d_root_partials.alloc(sizeof(double) * std::max<size_t>(1, rs.size()) * n_wt);
   e.d_beta.alloc(sizeof(double) * std::max<size_t>(1, rs.size()) * e.P);
   e.d_grad_partials.alloc(sizeof(double) * std::max(1, e.n_uses) * n_wt);
+  e.d_site_seed.alloc(sizeof(double) * std::max<size_t>(1, rs.size()) * e.L.n_sites_pad);
   e.d_root_sums.alloc(sizeof(double) * std::max<size_t>(1, rs.size()) * e.P);
   e.d_use_sums.alloc(sizeof(double) * std::max(1, e.n_uses) * static_cast<size_t>(e.P));
   NRX_HIP(hipStreamSynchronize(e.stream));
@@ -664,7 +665,7 @@ static int evaluate(Engine &e, int variant, bool incremental, bool update_pm, bo
     std::fill(e.clv_dirty.begin(), e.clv_dirty.end(), 0);
   } else {
     forward(e, incremental);
-    launch_root_partials(e);
+    launch_root_partials(e, gradient);
     launch_mixture(e, variant, e.pw);
     if (gradient) {
       e.d_adj.alloc(sizeof(double) * e.clv_elems_per_slot * e.dag.var.size());

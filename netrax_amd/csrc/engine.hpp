@@ -196,7 +196,7 @@ void launch_adjoint_persistent_multi(hipStream_t stream, int C, int G, int TS, c
 // Kernel launchers (kernels.hip)
 void launch_pmatrices(Engine &e);
 void launch_forward_level(Engine &e, const DevOp *ops, int n_ops, const LevelChunk &ch);
-void launch_root_partials(Engine &e);
+void launch_root_partials(Engine &e, bool seeds);
 void launch_mixture(Engine &e, int variant, int partials_per_tile);
 void launch_adjoint_level(Engine &e, const LevelChunk &ch);
 void launch_grad_reduce(Engine &e, int partials_per_tile);

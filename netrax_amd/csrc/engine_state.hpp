@@ -118,6 +118,7 @@ struct Engine : nrx_engine {
   DevBuf d_netdesc, d_work;  // FP32 form: this engine's NetDesc and (0, tile) work table
   DevBuf d_classes, d_class_dlogprob, d_root_slots;
   DevBuf d_root_partials, d_beta, d_grad_partials, d_root_sums, d_use_sums;
+  DevBuf d_site_seed;  // plain form: adjoint seed factor w / (C Ltilde) [root][site]
   DevBuf d_result;
   double *h_result = nullptr;  // pinned
   ResultLayout rl{};

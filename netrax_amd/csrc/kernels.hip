@@ -391,7 +391,8 @@ __global__ void __launch_bounds__(TILE *C)
     k_root_partials(const int32_t *__restrict__ root_slots, int n_root, DevLayout L,
                     const double *__restrict__ clv, const int32_t *__restrict__ scl,
                     const double *__restrict__ freqs, const double *__restrict__ weights,
-                    const int32_t *__restrict__ tile_part, double *__restrict__ partials) {
+                    const int32_t *__restrict__ tile_part, double *__restrict__ partials,
+                    double *__restrict__ seed) {
   constexpr int PW = TILE * C / 64;
   const int tile = blockIdx.x;
   if (tile >= L.n_tiles) return;
@@ -410,6 +411,8 @@ __global__ void __launch_bounds__(TILE *C)
 #pragma unroll
     for (int i = 0; i < S; ++i) t = fma(freqs[part * S + i], x[i], t);
     t = cat_sum<C>(t) * (1.0 / C);
+    // the adjoint's seed factor at this root and site, w / (C Ltilde)
+    if (seed && c == 0) seed[static_cast<size_t>(r) * n + site] = w != 0.0 ? w * (1.0 / C) / t : 0.0;
     double val = 0.0;
     if (c == 0 && w != 0.0)
       val = w * (log(t) + scl[static_cast<size_t>(slot) * n + site] * LOG_SCALE_THRESHOLD);
@@ -537,7 +540,7 @@ template <int S>
 struct AdjIn {
   double a[S];
   double x[2][S];  // inner children; tips are expanded from tm at use
-  double xq[S];    // the op's own CLV, for the seed of a displayed-tree root
+  double sd;       // seed factor w / (C Ltilde) of a displayed-tree root's site
   uint32_t tm[2];
   int kq, sk[2];
 };
@@ -548,14 +551,14 @@ __device__ __forceinline__ void adjoint_inputs(const DevOp &op, int64_t site, in
                                                const int32_t *__restrict__ scl,
                                                const double *__restrict__ adj,
                                                const typename TipType<S>::type *__restrict__ tips,
-                                               AdjIn<S> &in) {
+                                               const double *__restrict__ seed, AdjIn<S> &in) {
   // loads only (defaults for absent inputs are applied at use), so that no
   // load waits for an earlier one
   const size_t row = static_cast<size_t>(C * S);
   in.kq = scl[static_cast<size_t>(op.out) * n + site];
   if (op.flags & OP_HAS_CONSUMERS)
     load_clv<S>(adj + (static_cast<size_t>(op.out) * n + site) * row + c * S, in.a);
-  if (op.root >= 0) load_clv<S>(clv + (static_cast<size_t>(op.out) * n + site) * row + c * S, in.xq);
+  if (op.root >= 0) in.sd = seed[static_cast<size_t>(op.root) * n + site];
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
     const int32_t kid = op.kid[k];
@@ -577,7 +580,7 @@ __global__ void __launch_bounds__(TILE *C)
                     const double *__restrict__ clv, const int32_t *__restrict__ scl,
                     double *__restrict__ adj, const typename TipType<S>::type *__restrict__ tips,
                     const double *__restrict__ P, const double *__restrict__ Qd,
-                    const double *__restrict__ freqs, const double *__restrict__ weights,
+                    const double *__restrict__ freqs, const double *__restrict__ seed,
                     const int32_t *__restrict__ tile_part, const double *__restrict__ beta,
                     int n_root, double *__restrict__ grad_partials) {
   // LDS: r_c Q of every category (4 states), then the staged P matrices
@@ -597,21 +600,20 @@ __global__ void __launch_bounds__(TILE *C)
   const size_t row = static_cast<size_t>(C * S);
   const size_t n_wt = static_cast<size_t>(L.n_tiles) * PW;
   const int wt = tile * PW + threadIdx.x / 64;
-  const double w = weights[site];
   const double *Qc = (S == 4 ? sQ : Qd + static_cast<size_t>(part) * C * S * S) + c * (S * S);
   // 20-state operands are too large to hold two ops' worth in registers
   constexpr bool PREFETCH = S == 4;
   AdjIn<S> cur, nxt;
   if constexpr (PREFETCH) {
-    adjoint_inputs<S, C>(ops[0], site, n, c, clv, scl, adj, tips, cur);
+    adjoint_inputs<S, C>(ops[0], site, n, c, clv, scl, adj, tips, seed, cur);
     wait_loads();  // both ways into the loop then carry no pending load
   }
   for (int j = 0; j < n_ops; ++j) {
     const DevOp op = ops[j];
     if constexpr (PREFETCH) {
-      if (j + 1 < n_ops) adjoint_inputs<S, C>(ops[j + 1], site, n, c, clv, scl, adj, tips, nxt);
+      if (j + 1 < n_ops) adjoint_inputs<S, C>(ops[j + 1], site, n, c, clv, scl, adj, tips, seed, nxt);
     } else {
-      adjoint_inputs<S, C>(op, site, n, c, clv, scl, adj, tips, cur);
+      adjoint_inputs<S, C>(op, site, n, c, clv, scl, adj, tips, seed, cur);
     }
     double a[S];
     const bool has_adj = (op.flags & OP_HAS_CONSUMERS) != 0;
@@ -629,16 +631,9 @@ __global__ void __launch_bounds__(TILE *C)
     }
     if (op.root >= 0) {
       // seed beta * w * (1/C) pi_i / Ltilde at a displayed-tree root
-      double t = 0.0;
+      const double f = beta[part * n_root + op.root] * cur.sd;
 #pragma unroll
-      for (int i = 0; i < S; ++i) t = fma(freqs[part * S + i], cur.xq[i], t);
-      t = cat_sum<C>(t) * (1.0 / C);
-      const double b = beta[part * n_root + op.root] * w;
-      if (b != 0.0) {
-        const double f = b * (1.0 / C) / t;
-#pragma unroll
-        for (int i = 0; i < S; ++i) a[i] = fma(f, freqs[part * S + i], a[i]);
-      }
+      for (int i = 0; i < S; ++i) a[i] = fma(f, freqs[part * S + i], a[i]);
     }
     double y[2][S];
 #pragma unroll
@@ -1886,14 +1881,15 @@ void launch_forward_level(Engine &e, const DevOp *ops, int n_ops, const LevelChu
   NRX_HIP(hipGetLastError());
 }
 
-void launch_root_partials(Engine &e) {
+void launch_root_partials(Engine &e, bool seeds) {
   const int nr = static_cast<int>(e.root_slots.size());
   dispatch_sc(e.S, e.C, [&](auto s, auto c) {
     constexpr int S = decltype(s)::value, C = decltype(c)::value;
     hipLaunchKernelGGL((k_root_partials<S, C>), dim3(e.L.n_tiles), dim3(TILE * C), 0, e.stream,
                        e.d_root_slots.as<int32_t>(), nr, e.L, e.d_clv.as<double>(),
                        e.d_scl.as<int32_t>(), e.d_freqs.as<double>(), e.d_weights.as<double>(),
-                       e.d_tile_part.as<int32_t>(), e.d_root_partials.as<double>());
+                       e.d_tile_part.as<int32_t>(), e.d_root_partials.as<double>(),
+                       seeds ? e.d_site_seed.as<double>() : nullptr);
   });
   NRX_HIP(hipGetLastError());
 }
@@ -1924,7 +1920,7 @@ void launch_adjoint_level(Engine &e, const LevelChunk &ch) {
                        e.d_ops.as<DevOp>() + ch.op_begin, n_ops, n_staged, e.L, e.d_clv.as<double>(),
                        e.d_scl.as<int32_t>(), e.d_adj.as<double>(), e.d_tips.as<TT>(),
                        e.d_P.as<double>(), e.d_Qd.as<double>(), e.d_freqs.as<double>(),
-                       e.d_weights.as<double>(), e.d_tile_part.as<int32_t>(), e.d_beta.as<double>(),
+                       e.d_site_seed.as<double>(), e.d_tile_part.as<int32_t>(), e.d_beta.as<double>(),
                        static_cast<int>(e.root_slots.size()), e.d_grad_partials.as<double>());
   });
   NRX_HIP(hipGetLastError());

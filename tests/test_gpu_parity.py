@@ -71,6 +71,24 @@ def test_samples_match_oracle(name):
         ann.close()
 
 
+@pytest.mark.parametrize("ncat", [1, 2, 8])
+def test_gradients_other_category_counts_match_oracle(ncat):
+    """Every rate-category count the kernels are built for (4 is covered
+    above), on the three-reticulation sample."""
+    say("three_reticulations, categories", ncat)
+    net, parts = load_sample("three_reticulations", fixed_model(ncat=ncat))
+    for var in (0, 1):
+        ann = engine(net, parts, var)
+        got = computeLoglikelihood(ann, incremental=0)
+        want = O.loglikelihood(net, parts, var)[0]
+        assert abs(got - want) <= RTOL * abs(want), (ncat, var, got, want)
+        _, g, gp = ann.gradient()
+        _, og, ogp = O.gradient(net, parts, var)
+        assert np.allclose(g, og, rtol=1e-7, atol=1e-8 * max(1.0, np.abs(og).max())), (ncat, var)
+        assert np.allclose(gp, ogp, rtol=1e-7, atol=1e-8), (ncat, var)
+        ann.close()
+
+
 def test_celine_zero_branch_lengths_match_oracle():
     say("celine with clamped zero lengths")
     net, parts = load_sample("celine", fixed_model(), with_lengths=False)
