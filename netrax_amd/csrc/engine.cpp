@@ -217,26 +217,6 @@ static void build_schedule(Engine &e) {
   std::vector<int> root_of_slot(nv, -1);
   for (size_t r = 0; r < e.root_slots.size(); ++r) root_of_slot[e.root_slots[r]] = static_cast<int>(r);
 
-  // gradient rows: one per (op, child)
-  std::vector<std::array<int, 2>> gslot(nv, {-1, -1});
-  std::vector<std::vector<int>> by_edge(e.E);
-  e.n_uses = 0;
-  for (int s = 0; s < nv; ++s) {
-    if (!e.needed[s]) continue;
-    for (int k = 0; k < e.dag.var[s].nkids; ++k) {
-      gslot[s][k] = e.n_uses;
-      by_edge[e.dag.var[s].edge[k]].push_back(e.n_uses);
-      e.n_uses++;
-    }
-  }
-  e.edge_use_begin.assign(e.E + 1, 0);
-  e.edge_use_list.clear();
-  for (int ed = 0; ed < e.E; ++ed) {
-    e.edge_use_begin[ed] = static_cast<int>(e.edge_use_list.size());
-    for (int u : by_edge[ed]) e.edge_use_list.push_back(u);
-  }
-  e.edge_use_begin[e.E] = static_cast<int>(e.edge_use_list.size());
-
   auto make_op = [&](int s) {
     const Variant &v = e.dag.var[s];
     DevOp op{};
@@ -245,7 +225,7 @@ static void build_schedule(Engine &e) {
     for (int k = 0; k < 2; ++k) {
       op.kid[k] = k < v.nkids ? v.kid[k] : NO_CHILD;
       op.edge[k] = k < v.nkids ? v.edge[k] : 0;
-      op.gslot[k] = k < v.nkids ? gslot[s][k] : 0;
+      op.gslot[k] = 0;  // numbered below, once the adjoint order is known
     }
     op.root = root_of_slot[s];
     op.flags = 0;
@@ -353,6 +333,42 @@ static void build_schedule(Engine &e) {
     set_acc_early(e.ops_dfs, e.group_begin, nv);
   }
 
+  // gradient rows, one per (op, child), numbered in the order this engine's
+  // adjoint walks the ops (plain form: levels top down; FP32 form: groups
+  // backwards), so that a workgroup's consecutive ops write consecutive rows
+  {
+    std::vector<int> walk;  // indices into the form's op list
+    std::vector<DevOp> &ops = e.opt.precision == NRX_FP32 ? e.ops_dfs : e.ops_all;
+    if (e.opt.precision == NRX_FP32) {
+      for (int g = static_cast<int>(e.group_begin.size()) - 2; g >= 0; --g)
+        for (int i = e.group_begin[g]; i < e.group_begin[g + 1]; ++i) walk.push_back(i);
+    } else {
+      for (auto it = e.chunks.rbegin(); it != e.chunks.rend(); ++it)
+        for (int i = it->op_begin; i < it->op_end; ++i) walk.push_back(i);
+    }
+    std::vector<std::array<int32_t, 2>> gslot(nv, {0, 0});
+    std::vector<std::vector<int>> by_edge(e.E);
+    e.n_uses = 0;
+    for (int i : walk) {
+      const DevOp &op = ops[i];
+      for (int k = 0; k < op.nk; ++k) {
+        gslot[op.out][k] = e.n_uses;
+        by_edge[op.edge[k]].push_back(e.n_uses);
+        e.n_uses++;
+      }
+    }
+    for (std::vector<DevOp> *list : {&e.ops_all, &e.ops_dfs})
+      for (DevOp &op : *list)
+        for (int k = 0; k < op.nk; ++k) op.gslot[k] = gslot[op.out][k];
+    e.edge_use_begin.assign(e.E + 1, 0);
+    e.edge_use_list.clear();
+    for (int ed = 0; ed < e.E; ++ed) {
+      e.edge_use_begin[ed] = static_cast<int>(e.edge_use_list.size());
+      for (int u : by_edge[ed]) e.edge_use_list.push_back(u);
+    }
+    e.edge_use_begin[e.E] = static_cast<int>(e.edge_use_list.size());
+  }
+
   std::vector<DevClass> dc(nc);
   for (int k = 0; k < nc; ++k) {
     dc[k].logprob = e.class_logprob[k];
@@ -368,6 +384,19 @@ static void build_schedule(Engine &e) {
   upload(e.d_class_dlogprob, e.class_dlogprob, e.stream);
   std::vector<int32_t> rs(e.root_slots.begin(), e.root_slots.end());
   upload(e.d_root_slots, rs, e.stream);
+  // gradient partials per site block: plain form 64-site tiles of C waves;
+  // FP32 form its workgroups (tune_tile sites, or 32 for 20-state data)
+  if (e.opt.precision != NRX_FP32) {
+    e.gpw = TILE * e.C / 64;
+    e.gbpt = 1;
+  } else if (e.S == AA_STATES) {
+    e.gpw = e.C;
+    e.gbpt = TILE / AA_TILE;
+  } else {
+    e.gpw = e.tune_tile * e.C / 64;
+    e.gbpt = TILE / e.tune_tile;
+  }
+  if (e.gpw * e.gbpt != e.pw) throw std::logic_error("gradient partial layout does not cover the tile");
   const size_t n_wt = static_cast<size_t>(e.L.n_tiles) * e.pw;
   e.d_root_partials.alloc(sizeof(double) * std::max<size_t>(1, rs.size()) * n_wt);
   e.d_beta.alloc(sizeof(double) * std::max<size_t>(1, rs.size()) * e.P);
@@ -389,6 +418,7 @@ NetDesc make_netdesc(const Engine &e) {
   d.group_begin = e.d_group_begin.as<int32_t>();
   d.n_groups = static_cast<int32_t>(e.group_begin.size()) - 1;
   d.n_root = static_cast<int32_t>(e.root_slots.size());
+  d.n_uses = e.n_uses;
   d.clv = e.d_clv.as<float>();
   d.scl = e.d_scl.as<int32_t>();
   d.adj = e.d_adj.as<float>();
@@ -647,7 +677,7 @@ static int evaluate(Engine &e, int variant, bool incremental, bool update_pm, bo
         else
           launch_adjoint_persistent(e);
         if (timed) record_kernel_end(e, 1);
-        launch_grad_reduce(e, e.pw);
+        launch_grad_reduce(e);
       }
     };
     if (e.opt.use_graph) {
@@ -674,7 +704,7 @@ static int evaluate(Engine &e, int variant, bool incremental, bool update_pm, bo
         launch_adjoint_level(e, *it);
         record_kernel_end(e, 1);
       }
-      launch_grad_reduce(e, e.pw);
+      launch_grad_reduce(e);
     }
   }
   end_timing(e);
@@ -824,6 +854,8 @@ static SmallDesc make_smalldesc(const Engine &e) {
   d.rl_part_dbrlen = e.rl.part_dbrlen;
   d.rl_dprob = e.rl.dprob;
   d.pw = e.pw;
+  d.gpw = e.gpw;
+  d.gbpt = e.gbpt;
   const void *need[] = {d.V, d.Vinv, d.ev, d.rates, d.brlen, d.P, d.dP, d.d2P, d.Pf,
                         d.part_tile_begin, d.classes, d.class_dlogprob, d.root_partials,
                         d.root_sums, d.grad_partials, d.use_sums, d.beta, d.edge_use_begin,

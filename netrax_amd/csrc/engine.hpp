@@ -137,6 +137,8 @@ struct NetDesc {
   const int32_t *group_begin;
   int32_t n_groups;
   int32_t n_root;
+  int32_t n_uses;  // gradient rows
+  int32_t pad_;
   float *clv;
   int32_t *scl;
   float *adj;
@@ -175,7 +177,8 @@ struct SmallDesc {
   double *res;
   int32_t rl_total, rl_per_part, rl_root_logl, rl_tree_logl, rl_alpha, rl_dbrlen,
       rl_part_dbrlen, rl_dprob;
-  int32_t pw, pad;
+  int32_t pw;           // root partials per 64-site tile
+  int32_t gpw, gbpt, pad;  // gradient partials: waves per site block, blocks per tile
 };
 
 struct Engine;
@@ -199,7 +202,7 @@ void launch_forward_level(Engine &e, const DevOp *ops, int n_ops, const LevelChu
 void launch_root_partials(Engine &e, bool seeds);
 void launch_mixture(Engine &e, int variant, int partials_per_tile);
 void launch_adjoint_level(Engine &e, const LevelChunk &ch);
-void launch_grad_reduce(Engine &e, int partials_per_tile);
+void launch_grad_reduce(Engine &e);
 void launch_forward_persistent(Engine &e);
 void launch_adjoint_persistent(Engine &e);
 void launch_mfma_tables(Engine &e);

@@ -79,6 +79,8 @@ struct Engine : nrx_engine {
   std::vector<int32_t> edge_use_begin, edge_use_list;  // gradient rows per edge
   int n_uses = 0;
   int pw = 1;  // site-sum partials per tile (one per wave)
+  // gradient partials [site block][row][wave]: waves per block, blocks per tile
+  int gpw = 1, gbpt = 1;
   // FP32 form shape (nrx_autotune / nrx_set_tuning): ops per group of the
   // persistent kernels and sites per workgroup (4-state data; 32 for 20)
   int tune_group = 4;

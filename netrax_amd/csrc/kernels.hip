@@ -157,6 +157,15 @@ __device__ __forceinline__ double block_sum(double v, double *sh) {
   return r;
 }
 
+// Gradient partials: one value per (site block, gradient row, wave of the
+// block), stored [block][row][wave].  Rows are numbered in the order the
+// adjoint walks the ops, so a workgroup's waves fill whole cache lines of
+// their own block instead of scattering 8-byte writes over rows shared with
+// other workgroups (partial-line writes cost a read-modify-write in HBM).
+__host__ __device__ __forceinline__ size_t grad_index(int64_t block, int n_uses, int g, int pw, int w) {
+  return (static_cast<size_t>(block) * n_uses + g) * pw + w;
+}
+
 template <int S>
 __device__ __forceinline__ void load_clv(const double *__restrict__ p, double (&x)[S]) {
   if constexpr (S == 4) {
@@ -582,7 +591,7 @@ __global__ void __launch_bounds__(TILE *C)
                     const double *__restrict__ P, const double *__restrict__ Qd,
                     const double *__restrict__ freqs, const double *__restrict__ seed,
                     const int32_t *__restrict__ tile_part, const double *__restrict__ beta,
-                    int n_root, double *__restrict__ grad_partials) {
+                    int n_root, int n_uses, double *__restrict__ grad_partials) {
   // LDS: r_c Q of every category (4 states), then the staged P matrices
   double *sQ = dyn_lds<double>();
   double *sP = sQ + (S == 4 ? C * S * S : 0);
@@ -598,8 +607,7 @@ __global__ void __launch_bounds__(TILE *C)
   const int64_t site = static_cast<int64_t>(tile) * TILE + threadIdx.x / C;
   const int64_t n = L.n_sites_pad;
   const size_t row = static_cast<size_t>(C * S);
-  const size_t n_wt = static_cast<size_t>(L.n_tiles) * PW;
-  const int wt = tile * PW + threadIdx.x / 64;
+  const int wave = threadIdx.x / 64;
   const double *Qc = (S == 4 ? sQ : Qd + static_cast<size_t>(part) * C * S * S) + c * (S * S);
   // 20-state operands are too large to hold two ops' worth in registers
   constexpr bool PREFETCH = S == 4;
@@ -663,8 +671,7 @@ __global__ void __launch_bounds__(TILE *C)
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
       if (k < op.nk) {
-        if ((threadIdx.x & 63) == 0)
-          grad_partials[static_cast<size_t>(op.gslot[k]) * n_wt + wt] = g[k];
+        if ((threadIdx.x & 63) == 0) grad_partials[grad_index(tile, n_uses, op.gslot[k], PW, wave)] = g[k];
         const int32_t kid = op.kid[k];
         if (kid >= 0) {
           double *dst = adj + (static_cast<size_t>(kid) * n + site) * row + c * S;
@@ -1003,10 +1010,10 @@ __device__ __forceinline__ void adjoint_persistent_body(
     const GAS uint8_t *__restrict__ tips, const GAS float *__restrict__ Pf,
     const GAS float *__restrict__ Qf, const GAS int32_t *__restrict__ tile_part,
     const GAS double *__restrict__ freqs, const GAS double *__restrict__ weights,
-    const GAS double *__restrict__ beta, int n_root, GAS double *__restrict__ grad_partials,
+    const GAS double *__restrict__ beta, int n_root, int n_uses, GAS double *__restrict__ grad_partials,
     int tile) {
   float *sP = dyn_lds<float>();  // [edge][cat][16] then Q: [cat][16]
-  constexpr int PW = TILE * C / 64;
+  constexpr int PWB = TS * C / 64;  // waves of this site block
   const int64_t site0 = static_cast<int64_t>(tile) * TS;
   const int part = tile_part[site0 / TILE];
   const int np = L.n_edges * C * 16;
@@ -1018,8 +1025,7 @@ __device__ __forceinline__ void adjoint_persistent_body(
   const float *sQ = sP + np + c * 16;
   const int64_t site = site0 + threadIdx.x / C;
   const int64_t n = L.n_sites_pad;
-  const size_t n_wt = static_cast<size_t>(L.n_tiles) * PW;
-  const size_t wt = static_cast<size_t>(site0 * C / 64) + threadIdx.x / 64;  // global wave
+  const int wave = threadIdx.x / 64;
   const bool lane0 = c == 0;
   const double w = weights[site];
   double pi[4];
@@ -1135,8 +1141,7 @@ __device__ __forceinline__ void adjoint_persistent_body(
             mv4(sQ, y[kk], qx);
             const float gd = fmaf(z[3], qx[3], fmaf(z[2], qx[2], fmaf(z[1], qx[1], z[0] * qx[0])));
             double gv = wave_sum(ldexp(static_cast<double>(gd), ez));
-            if ((threadIdx.x & 63) == 0)
-              grad_partials[static_cast<size_t>(op.gslot[kk]) * n_wt + wt] = gv;
+            if ((threadIdx.x & 63) == 0) grad_partials[grad_index(tile, n_uses, op.gslot[kk], PWB, wave)] = gv;
             const int32_t kid = op.kid[kk];
             if (kid >= 0) {
               GAS float *dst_m = adj + (static_cast<size_t>(kid) * n + site) * row + c * 4;
@@ -1167,7 +1172,7 @@ __global__ void __launch_bounds__(TS *C)
                                     to_global(nd.clv), to_global(nd.scl), to_global(nd.adj),
                                     to_global(nd.adj_exp), to_global(nd.tips), to_global(nd.Pf),
                                     to_global(nd.Qf), to_global(nd.tile_part), to_global(nd.freqs),
-                                    to_global(nd.weights), to_global(nd.beta), nd.n_root,
+                                    to_global(nd.weights), to_global(nd.beta), nd.n_root, nd.n_uses,
                                     to_global(nd.grad_partials), wi.tile);
 }
 
@@ -1423,7 +1428,7 @@ __device__ __forceinline__ void adjoint_mfma_body(
     GAS float *__restrict__ adj, GAS int32_t *__restrict__ adj_exp,
     const GAS uint32_t *__restrict__ tips, const GAS double *__restrict__ freqs,
     const GAS int32_t *__restrict__ tile_part, const GAS float *__restrict__ Pa_all,
-    const GAS double *__restrict__ weights, const GAS double *__restrict__ beta, int n_root,
+    const GAS double *__restrict__ weights, const GAS double *__restrict__ beta, int n_root, int n_uses,
     GAS double *__restrict__ grad_partials, int tile) {
   __shared__ double rsum[2][C][AA_TILE];
   __shared__ float zb[C][AA_STATES * AA_TILE];
@@ -1440,8 +1445,6 @@ __device__ __forceinline__ void adjoint_mfma_body(
   const GAS float *__restrict__ Pa =
       Pa_all + static_cast<size_t>(part) * L.n_edges * C * AA_KINDS * AA_TABLE;
   const double w = weights[site];
-  const size_t n_wt = static_cast<size_t>(L.n_tiles) * 2 * C;
-  const size_t wt = static_cast<size_t>(tile) * C + c;
   float *zw = zb[c];
   int rb = 0;
   for (int gi = n_groups - 1; gi >= 0; --gi) {
@@ -1537,7 +1540,7 @@ __device__ __forceinline__ void adjoint_mfma_body(
 #pragma unroll
           for (int r = 0; r < AA_ROWS; ++r) gd = fmaf(z[r], dy[r], gd);
           const double gv = wave_sum(ldexp(static_cast<double>(gd), ez));
-          if (lane == 0) grad_partials[static_cast<size_t>(op.gslot[kk]) * n_wt + wt] = gv;
+          if (lane == 0) grad_partials[grad_index(tile, n_uses, op.gslot[kk], C, c)] = gv;
           const int32_t kid = op.kid[kk];
           if (kid >= 0) {
 #pragma unroll
@@ -1572,7 +1575,7 @@ __global__ void __launch_bounds__(64 * C)
                        to_global(nd.clv), to_global(nd.scl), to_global(nd.adj), to_global(nd.adj_exp),
                        to_global(reinterpret_cast<const uint32_t *>(nd.tips)), to_global(nd.freqs),
                        to_global(nd.tile_part), to_global(nd.Pa), to_global(nd.weights),
-                       to_global(nd.beta), nd.n_root, to_global(nd.grad_partials), wi.tile);
+                       to_global(nd.beta), nd.n_root, nd.n_uses, to_global(nd.grad_partials), wi.tile);
 }
 
 // ===================================================== branch derivatives
@@ -1783,12 +1786,34 @@ __global__ void __launch_bounds__(256) k_root_reduce_multi(const SmallDesc *__re
                        d.part_tile_begin, P, blockIdx.x / P, blockIdx.x % P, d.root_sums);
 }
 
+// out[g][p] = sum of the gradient partials of row g over the site blocks of
+// partition p and their waves (layout [block][row][wave], gbpt blocks per
+// 64-site tile, gpw waves per block), in a fixed order
+__device__ void reduce_grad_body(const double *__restrict__ in, int n_uses, int gpw, int gbpt,
+                                 const int32_t *__restrict__ ptb, int P, int g, int p,
+                                 double *__restrict__ out) {
+  __shared__ double sh[4];
+  const int64_t q1 = static_cast<int64_t>(ptb[p + 1] - ptb[p]) * gbpt * gpw;
+  const int64_t b0 = static_cast<int64_t>(ptb[p]) * gbpt;
+  double s = 0.0;
+  for (int64_t q = threadIdx.x; q < q1; q += 256)
+    s += in[grad_index(b0 + q / gpw, n_uses, g, gpw, static_cast<int>(q % gpw))];
+  s = block_sum<256>(s, sh);
+  if (threadIdx.x == 0) out[static_cast<size_t>(g) * P + p] = s;
+}
+
+__global__ void __launch_bounds__(256)
+    k_reduce_grad(const double *__restrict__ in, int n_uses, int gpw, int gbpt,
+                  const int32_t *__restrict__ ptb, int P, double *__restrict__ out) {
+  reduce_grad_body(in, n_uses, gpw, gbpt, ptb, P, blockIdx.x / P, blockIdx.x % P, out);
+}
+
 __global__ void __launch_bounds__(256) k_use_reduce_multi(const SmallDesc *__restrict__ descs) {
   const SmallDesc &d = descs[blockIdx.y];
   const int P = d.L.n_parts;
   if (static_cast<int>(blockIdx.x) >= d.n_uses * P) return;
-  reduce_segments_body(d.grad_partials, static_cast<size_t>(d.L.n_tiles) * d.pw, d.pw,
-                       d.part_tile_begin, P, blockIdx.x / P, blockIdx.x % P, d.use_sums);
+  reduce_grad_body(d.grad_partials, d.n_uses, d.gpw, d.gbpt, d.part_tile_begin, P, blockIdx.x / P,
+                   blockIdx.x % P, d.use_sums);
 }
 
 __global__ void __launch_bounds__(256) k_mixture_multi(const SmallDesc *__restrict__ descs,
@@ -1921,15 +1946,14 @@ void launch_adjoint_level(Engine &e, const LevelChunk &ch) {
                        e.d_scl.as<int32_t>(), e.d_adj.as<double>(), e.d_tips.as<TT>(),
                        e.d_P.as<double>(), e.d_Qd.as<double>(), e.d_freqs.as<double>(),
                        e.d_site_seed.as<double>(), e.d_tile_part.as<int32_t>(), e.d_beta.as<double>(),
-                       static_cast<int>(e.root_slots.size()), e.d_grad_partials.as<double>());
+                       static_cast<int>(e.root_slots.size()), e.n_uses, e.d_grad_partials.as<double>());
   });
   NRX_HIP(hipGetLastError());
 }
 
-void launch_grad_reduce(Engine &e, int pw) {
-  hipLaunchKernelGGL(k_reduce_segments, dim3(e.n_uses * e.P), dim3(256), 0, e.stream,
-                     e.d_grad_partials.as<double>(), static_cast<size_t>(e.L.n_tiles) * pw, pw,
-                     e.d_part_tile_begin.as<int32_t>(), e.P, e.d_use_sums.as<double>());
+void launch_grad_reduce(Engine &e) {
+  hipLaunchKernelGGL(k_reduce_grad, dim3(e.n_uses * e.P), dim3(256), 0, e.stream, e.d_grad_partials.as<double>(),
+                     e.n_uses, e.gpw, e.gbpt, e.d_part_tile_begin.as<int32_t>(), e.P, e.d_use_sums.as<double>());
   NRX_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_grad_total, dim3((e.E + 255) / 256), dim3(256), 0, e.stream, e.L,
                      e.d_edge_use_begin.as<int32_t>(), e.d_edge_use_list.as<int32_t>(),
