@@ -372,7 +372,7 @@ def main():
     for name, prec in (("plain", 0), ("batched", 1)):
         if args.only and args.only != name:
             continue
-        forms[name] = run_form(w, prec, args.steps, args.warmup, tune=prec == 1)
+        forms[name] = run_form(w, prec, args.steps, args.warmup, use_graph=prec == 1, tune=prec == 1)
         r = forms[name]
         log(f"{name}: {r['evals_per_s']:.1f} evals/s, device {r['device_ms_per_eval']:.3f} ms/eval "
             f"(fwd {r['fwd_ms_per_eval']:.3f} ms, adj {r['adj_ms_per_eval']:.3f} ms)")
@@ -382,7 +382,8 @@ def main():
     desc = {"plain": "plain FP64 (FP64 CLVs, one launch per pruning level, FP64 adjoint)",
             "batched": "batched FP32 (FP32 CLVs + per-site power-of-two exponents, one persistent "
                        "site-tile launch for all ops and trees, FP32 adjoint with per-site "
-                       "exponents, FP64 site sums / mixture / gradient reduction)"}
+                       "exponents, FP64 site sums / mixture / gradient reduction; each evaluation "
+                       "replays one captured HIP graph)"}
     out = {
         "metric": METRIC,
         "value": round(head["evals_per_s"], 3),

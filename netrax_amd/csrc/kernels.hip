@@ -437,6 +437,15 @@ __global__ void __launch_bounds__(TILE *C)
 // a max-shifted FP64 log-sum-exp stands in for the reference's MPFR sum.
 // Also produces the adjoint seeds beta[p][root CLV] = sum of the class
 // weights d lnL_p / d lnL_{p,t}, and d lnL / d p_r.
+// The inputs (root sums, class data, d log P / d p) and alpha are staged in
+// LDS when they fit in MIX_LDS_DOUBLES (the usual case), so the serial loops
+// over classes read LDS instead of making a memory round trip per class;
+// otherwise the same code runs on the global arrays.  The arithmetic and
+// its order are the same either way.
+constexpr int MIX_LDS_DOUBLES = 6144;  // 48 KiB
+__host__ __device__ inline int mixture_lds_doubles(int n_root, int nc, int R, int P) {
+  return n_root * P + 2 * nc + nc * P + nc * R;
+}
 __device__ void mixture_body(const DevLayout &L, int n_root, int nc, int R, int variant,
                              const DevClass *__restrict__ classes,
                              const double *__restrict__ class_dlogprob,
@@ -445,24 +454,52 @@ __device__ void mixture_body(const DevLayout &L, int n_root, int nc, int R, int 
   __shared__ double best[64], ssum[64];
   __shared__ int arg[64];
   const int P = L.n_parts;  // <= 64 (checked on the host)
+  const bool staged = mixture_lds_doubles(n_root, nc, R, P) <= MIX_LDS_DOUBLES;
+  // class k: log-probability lp[k], root index ri[k], use flag cu[k]
+  const double *rs = root_sums, *cdl = class_dlogprob;
   double *alpha = res + rl.alpha;
+  double *lp = nullptr;
+  int *ri = nullptr, *cu = nullptr;
+  if (staged) {
+    double *sm = dyn_lds<double>();
+    double *s_rs = sm, *s_lp = s_rs + n_root * P, *s_ri = s_lp + nc, *s_alpha = s_ri + nc, *s_cdl = s_alpha + nc * P;
+    for (int i = threadIdx.x; i < n_root * P; i += blockDim.x) s_rs[i] = root_sums[i];
+    for (int i = threadIdx.x; i < nc * R; i += blockDim.x) s_cdl[i] = class_dlogprob[i];
+    int *s_ri_int = reinterpret_cast<int *>(s_ri), *s_cu = s_ri_int + nc;
+    for (int k = threadIdx.x; k < nc; k += blockDim.x) {
+      const DevClass cl = classes[k];
+      s_lp[k] = cl.logprob;
+      s_ri_int[k] = cl.root_index;
+      s_cu[k] = cl.use;
+    }
+    rs = s_rs;
+    cdl = s_cdl;
+    alpha = s_alpha;
+    lp = s_lp;
+    ri = s_ri_int;
+    cu = s_cu;
+    __syncthreads();
+  }
+  const auto logprob = [&](int k) { return staged ? lp[k] : classes[k].logprob; };
+  const auto used = [&](int k) { return (staged ? cu[k] : classes[k].use) != 0; };
+  const auto root_of = [&](int k) { return staged ? ri[k] : classes[k].root_index; };
   // per-(root CLV, partition) tree lnL and per-class copies
   for (int idx = threadIdx.x; idx < n_root * P; idx += blockDim.x) {
     const int r = idx / P, p = idx % P;
-    res[rl.root_logl + p * n_root + r] = root_sums[idx];
+    res[rl.root_logl + p * n_root + r] = rs[idx];
   }
   for (int idx = threadIdx.x; idx < nc * P; idx += blockDim.x) {
     const int p = idx / nc, k = idx % nc;
-    const int ri = classes[k].root_index;
-    res[rl.tree_logl + p * nc + k] = ri >= 0 ? root_sums[ri * P + p] : 0.0;
+    const int r = root_of(k);
+    res[rl.tree_logl + p * nc + k] = r >= 0 ? rs[r * P + p] : 0.0;
   }
   // best class per partition (first maximum in class order)
   for (int p = threadIdx.x; p < P; p += blockDim.x) {
     double bv = -INFINITY;
     int ba = -1;
     for (int k = 0; k < nc; ++k) {
-      if (!classes[k].use) continue;
-      const double v = classes[k].logprob + root_sums[classes[k].root_index * P + p];
+      if (!used(k)) continue;
+      const double v = logprob(k) + rs[root_of(k) * P + p];
       if (v > bv) {
         bv = v;
         ba = k;
@@ -477,8 +514,8 @@ __device__ void mixture_body(const DevLayout &L, int n_root, int nc, int R, int 
     double a = 0.0;
     if (variant == 1)
       a = k == arg[p] ? 1.0 : 0.0;
-    else if (classes[k].use && best[p] > -INFINITY)
-      a = exp(classes[k].logprob + root_sums[classes[k].root_index * P + p] - best[p]);
+    else if (used(k) && best[p] > -INFINITY)
+      a = exp(logprob(k) + rs[root_of(k) * P + p] - best[p]);
     alpha[p * nc + k] = a;
   }
   __syncthreads();
@@ -495,18 +532,20 @@ __device__ void mixture_body(const DevLayout &L, int n_root, int nc, int R, int 
       alpha[idx] = ssum[p] > 0.0 ? alpha[idx] / ssum[p] : 0.0;
     }
   __syncthreads();
+  if (staged)
+    for (int idx = threadIdx.x; idx < nc * P; idx += blockDim.x) res[rl.alpha + idx] = alpha[idx];
   for (int idx = threadIdx.x; idx < n_root * P; idx += blockDim.x) {
     const int p = idx / n_root, r = idx % n_root;
     double b = 0.0;
     for (int k = 0; k < nc; ++k)
-      if (classes[k].root_index == r) b += alpha[p * nc + k];
+      if (root_of(k) == r) b += alpha[p * nc + k];
     beta[p * n_root + r] = b;
   }
   for (int r = threadIdx.x; r < R; r += blockDim.x) {
     double g = 0.0;
     for (int p = 0; p < P; ++p)
       for (int k = 0; k < nc; ++k)
-        g += alpha[p * nc + k] * class_dlogprob[static_cast<size_t>(k) * R + r];
+        g += alpha[p * nc + k] * cdl[static_cast<size_t>(k) * R + r];
     res[rl.dprob + r] = g;
   }
   if (threadIdx.x == 0) {
@@ -515,7 +554,6 @@ __device__ void mixture_body(const DevLayout &L, int n_root, int nc, int R, int 
     res[rl.total] = tot;
   }
 }
-
 
 __global__ void __launch_bounds__(256)
     k_mixture(DevLayout L, int n_root, int nc, int R, int variant,
@@ -1925,7 +1963,7 @@ void launch_mixture(Engine &e, int variant, int pw) {
                      e.d_root_partials.as<double>(), static_cast<size_t>(e.L.n_tiles) * pw, pw,
                      e.d_part_tile_begin.as<int32_t>(), e.P, e.d_root_sums.as<double>());
   NRX_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_mixture, dim3(1), dim3(256), 0, e.stream, e.L, nr,
+  hipLaunchKernelGGL(k_mixture, dim3(1), dim3(256), sizeof(double) * MIX_LDS_DOUBLES, e.stream, e.L, nr,
                      static_cast<int>(e.dag.classes.size()), e.R, variant,
                      e.d_classes.as<DevClass>(), e.d_class_dlogprob.as<double>(),
                      e.d_root_sums.as<double>(), e.d_beta.as<double>(), e.d_result.as<double>(),
@@ -2118,7 +2156,7 @@ void launch_root_reduce_multi(hipStream_t s, const SmallDesc *d, int n, int max_
 }
 
 void launch_mixture_multi(hipStream_t s, const SmallDesc *d, int n, int variant) {
-  hipLaunchKernelGGL(k_mixture_multi, dim3(n), dim3(256), 0, s, d, variant);
+  hipLaunchKernelGGL(k_mixture_multi, dim3(n), dim3(256), sizeof(double) * MIX_LDS_DOUBLES, s, d, variant);
   NRX_HIP(hipGetLastError());
 }
 
