@@ -19,6 +19,7 @@ import ctypes as C
 import json
 import os
 import statistics
+import subprocess
 import sys
 import time
 
@@ -305,6 +306,38 @@ def oracle_check(w, lnl, g, gp, threads):
     }
 
 
+def stream_reference(nbytes=1 << 30):
+    """Practical HBM rates on this GPU for plain streams (torch elementwise
+    kernels over 1 GiB float64 buffers, best of 5, HIP events): a read-only
+    sum, a copy (1 read + 1 write) and an add (2 reads + 1 write).  Reported
+    beside the published 8 TB/s peak that `roofline.frac` uses."""
+    import torch
+
+    n = nbytes // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.ones_like(a)
+    c = torch.empty_like(a)
+    out = {}
+    for name, moved, fn in (("read_sum", 1, lambda: a.sum()),
+                            ("copy", 2, lambda: c.copy_(a)),
+                            ("add_2r1w", 3, lambda: torch.add(a, b, out=c))):
+        fn()
+        best = float("inf")
+        for _ in range(5):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            e1.synchronize()
+            best = min(best, e0.elapsed_time(e1))
+        out[name + "_gbps"] = round(moved * nbytes / (best * 1e-3) / 1e9, 1)
+    del a, b, c
+    torch.cuda.empty_cache()
+    out["note"] = ("torch elementwise kernels, 1 GiB float64 buffers: what plain streams reach on this "
+                   "GPU, beside the 8000 GB/s peak used for roofline.frac")
+    return out
+
+
 def copy_calibration(nbytes):
     """A copy kernel of known size (reads and writes nbytes each), three
     times, for scaling the PMC byte counters of a profiled run."""
@@ -336,6 +369,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the protein and candidate-batch sections")
     ap.add_argument("--candidates", type=int, default=256)
+    ap.add_argument("--stream-reference", action="store_true",
+                    help="print the plain-stream HBM rates of this GPU (torch kernels) and exit")
     ap.add_argument("--shape", default=None, metavar="GxTS",
                     help="fixed FP32 kernel shape (ops per group x sites per workgroup) "
                          "instead of autotuning, e.g. 2x64 (profiling runs)")
@@ -348,11 +383,24 @@ def main():
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
         return 0
+    if args.stream_reference:
+        print(json.dumps(stream_reference()), flush=True)
+        return 0
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; the site-sharded multi-GPU form is "
               "design only (DESIGN.md). Run with --gpus 1.", flush=True)
         return 1
     threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    # torch's HIP runtime and this library's do not share one process, so the
+    # stream reference runs in a child process, before this one touches the GPU
+    stream_ref = None
+    if not args.only:
+        try:
+            r = subprocess.run([sys.executable, os.path.abspath(__file__), "--stream-reference"],
+                               capture_output=True, text=True, timeout=300)
+            stream_ref = json.loads(r.stdout.strip().splitlines()[-1]) if r.returncode == 0 else None
+        except (subprocess.TimeoutExpired, ValueError, IndexError):
+            stream_ref = None
     if args.only == "protein":
         print(json.dumps({"protein": bench_protein(args.steps, args.warmup)}), flush=True)
         return 0
@@ -407,6 +455,7 @@ def main():
             "tuning": head["tuning"],
         },
         "roofline": roofline(head),
+        "stream_reference": stream_ref,
         "device_ms_per_eval": round(head["device_ms_per_eval"], 4),
         "timing_note": "value/ms_per_step: engine without timing events; device_ms and roofline "
                        "kernel durations: same-shape engine with HIP events around every launch",
