@@ -451,7 +451,7 @@ def main():
             "form": desc[head_name],
             "patterns": n_pat, "sites": 50000, "edges": int(w.network.n_edges),
             "clv_ops": st.n_ops_needed, "levels": st.n_levels, "displayed_trees": st.n_classes,
-            "adjoint_uses": st.n_uses, "parallelism": "single GPU",
+            "adjoint_uses": st.n_uses, "adjoint_lds_handoffs": st.n_lds_handoffs, "parallelism": "single GPU",
             "tuning": head["tuning"],
         },
         "roofline": roofline(head),

@@ -50,7 +50,7 @@ class Options(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("n_variants", C.c_int32), ("n_ops_needed", C.c_int32), ("n_levels", C.c_int32),
                 ("n_classes", C.c_int32), ("n_root_clvs", C.c_int32), ("n_uses", C.c_int32),
-                ("n_inner_uses", C.c_int32), ("n_sites_padded", C.c_int64), ("fwd_op_bytes", C.c_double),
+                ("n_inner_uses", C.c_int32), ("n_lds_handoffs", C.c_int32), ("n_sites_padded", C.c_int64), ("fwd_op_bytes", C.c_double),
                 ("root_bytes", C.c_double), ("adj_bytes", C.c_double), ("fwd_bytes", C.c_double),
                 ("grad_bytes", C.c_double), ("clv_bytes", C.c_double)]
 

@@ -182,6 +182,44 @@ static void set_acc_early(std::vector<DevOp> &ops, const std::vector<int32_t> &g
   }
 }
 
+// OP_PASS_* for the FP32 4-state reverse sweep (groups backwards): a
+// delivery to a child whose only consumer is this op goes to one of
+// PASS_SLOTS per-thread LDS slots, which the child's op reads in its group's
+// load phase.  A slot is reusable from the group in which its child loads
+// it (that load precedes the group's deliveries).  Greedy in walk order.
+static void set_pass_slots(std::vector<DevOp> &ops, const std::vector<int32_t> &group_begin, int nv) {
+  std::vector<int> consumers(nv, 0), op_of(nv, -1), group_of(ops.size(), -1);
+  const int ng = static_cast<int>(group_begin.size()) - 1;
+  for (int g = 0; g < ng; ++g)
+    for (int i = group_begin[g]; i < group_begin[g + 1]; ++i) {
+      group_of[i] = ng - 1 - g;  // position of the group in the reverse sweep
+      op_of[ops[i].out] = i;
+    }
+  for (const DevOp &op : ops)
+    for (int k = 0; k < op.nk; ++k)
+      if (op.kid[k] >= 0) consumers[op.kid[k]]++;
+  for (DevOp &op : ops) op.flags &= ~(OP_PASS_KID0 | OP_PASS_KID1 | OP_A_PASS | (0x3F << PASS_SHIFT));
+  std::array<int, PASS_SLOTS> busy_until;  // sweep position of the group that reads the slot
+  busy_until.fill(-1);
+  for (int g = ng - 1, t = 0; g >= 0; --g, ++t)
+    for (int i = group_begin[g]; i < group_begin[g + 1]; ++i) {
+      DevOp &op = ops[i];
+      for (int k = 0; k < op.nk; ++k) {
+        const int32_t kid = op.kid[k];
+        if (kid < 0 || consumers[kid] != 1 || op_of[kid] < 0) continue;
+        const int tc = group_of[op_of[kid]];
+        if (tc <= t) continue;  // not later in the sweep (cannot happen for a child)
+        for (int q = 0; q < PASS_SLOTS; ++q) {
+          if (busy_until[q] > t) continue;
+          busy_until[q] = tc;
+          op.flags |= (k == 0 ? OP_PASS_KID0 : OP_PASS_KID1) | (q << (PASS_SHIFT + 2 * k));
+          ops[op_of[kid]].flags |= OP_A_PASS | (q << (PASS_SHIFT + 4));
+          break;
+        }
+      }
+    }
+}
+
 // Builds the level schedule, the batched order, root CLV set and gradient
 // rows for the classes that pass the log-probability cut, and uploads them.
 static void build_schedule(Engine &e) {
@@ -331,6 +369,7 @@ static void build_schedule(Engine &e) {
     for (size_t g = 0; g < group_indep.size(); ++g)
       if (group_indep[g]) e.ops_dfs[e.group_begin[g]].flags |= OP_GROUP_INDEP;
     set_acc_early(e.ops_dfs, e.group_begin, nv);
+    if (e.S == 4) set_pass_slots(e.ops_dfs, e.group_begin, nv);
   }
 
   // gradient rows, one per (op, child), numbered in the order this engine's
@@ -894,7 +933,7 @@ static void build_batch(Batch &b) {
     b.max_edges = std::max(b.max_edges, e.E);
     b.max_res = std::max(b.max_res, e.rl.size);
     b.lds_fwd = std::max(b.lds_fwd, sizeof(float) * e.E * e.C * 16);
-    b.lds_adj = std::max(b.lds_adj, sizeof(float) * (e.E * e.C * 16 + e.C * 16));
+    b.lds_adj = std::max(b.lds_adj, adjoint_persistent_lds(e.E, e.C, b.TS));
   }
   std::stable_sort(order.begin(), order.end(),
                    [](const std::pair<int, int> &x, const std::pair<int, int> &y) {
@@ -1481,23 +1520,33 @@ int nrx_get_stats(nrx_engine *eng, nrx_stats *out) {
     }
     const double root = e.root_slots.size() * (row + scl + 8.0 * sites);
     const double pm = static_cast<double>(e.P) * e.E * e.C * e.S * e.S * b;
-    // push-form adjoint: per op read its adjoint (when accumulated), its
-    // scaler, its children (and, for root CLVs, itself for the seed); per
-    // inner child write the delivered adjoint, and read it back first when
-    // another consumer delivered before
-    // adjoint exponents: per site, or per (site, category) for 20-state FP32
-    const double aexp = e.opt.precision == NRX_FP32 ? (e.S == AA_STATES ? scl * e.C : scl) : 0.0;
+    // push-form adjoint, over the op list and flags of the engine's form:
+    // per op read its scaler, its adjoint (when accumulated) and its children
+    // with their scalers; per inner child write the delivered adjoint, and
+    // read it back first when another consumer delivered before.  A root op
+    // reads the seed factor (plain form, 8 bytes per site) or its own CLV
+    // (FP32 form).  Adjoints handed over through LDS (FP32, 4 states) move no
+    // HBM bytes.  Adjoint exponents: per site, or per (site, category) for
+    // 20-state FP32.
+    const bool fp32 = e.opt.precision == NRX_FP32;
+    const double aexp = fp32 ? (e.S == AA_STATES ? scl * e.C : scl) : 0.0;
+    const bool pass = fp32 && e.S == 4;
+    out->n_lds_handoffs = 0;
+    if (pass)
+      for (const DevOp &op : e.ops_dfs)
+        out->n_lds_handoffs += ((op.flags & OP_PASS_KID0) != 0) + ((op.flags & OP_PASS_KID1) != 0);
     double adj = 0.0;
-    for (const DevOp &op : e.ops_all) {
+    for (const DevOp &op : fp32 ? e.ops_dfs : e.ops_all) {
       adj += scl;
-      if (op.flags & OP_HAS_CONSUMERS) adj += row + aexp;
-      if (op.root >= 0) adj += row;
+      if ((op.flags & OP_HAS_CONSUMERS) && !(pass && (op.flags & OP_A_PASS))) adj += row + aexp;
+      if (op.root >= 0) adj += fp32 ? row : 8.0 * sites;
       for (int k = 0; k < op.nk; ++k) {
         if (op.kid[k] < 0) {
           adj += tip_b;
           continue;
         }
         adj += row + scl;  // child CLV and scaler
+        if (pass && (op.flags & (k == 0 ? OP_PASS_KID0 : OP_PASS_KID1))) continue;
         adj += row + aexp;  // delivered adjoint
         if (!(op.flags & (k == 0 ? OP_FIRST_KID0 : OP_FIRST_KID1))) adj += row + aexp;
       }

@@ -94,6 +94,16 @@ constexpr int32_t OP_GROUP_INDEP = 8;
 // value may be loaded with the group's other inputs
 constexpr int32_t OP_ACC_EARLY0 = 16;
 constexpr int32_t OP_ACC_EARLY1 = 32;
+// FP32 form, 4 states: a child with this op as its only consumer receives
+// its adjoint through a per-thread LDS slot instead of HBM.  OP_PASS_KIDk:
+// deliver child k's adjoint to slot pass_slot(flags, k); OP_A_PASS: this
+// op's own adjoint is in slot pass_slot(flags, 2)
+constexpr int32_t OP_PASS_KID0 = 64;
+constexpr int32_t OP_PASS_KID1 = 128;
+constexpr int32_t OP_A_PASS = 256;
+constexpr int PASS_SLOTS = 4;
+constexpr int PASS_SHIFT = 12;  // slot of use u (0, 1: children, 2: own) in bits 12 + 2u
+inline __host__ __device__ int pass_slot(int32_t flags, int u) { return (flags >> (PASS_SHIFT + 2 * u)) & 3; }
 
 // A launch of the plain form: the ops [op_begin, op_end) of one pruning
 // level.  The forward runs one workgroup per (tile, PLAIN_G consecutive
@@ -195,6 +205,8 @@ void launch_forward_persistent_multi(hipStream_t stream, int C, int G, int TS, c
                                      const WorkItem *work, int n_work, size_t lds);
 void launch_adjoint_persistent_multi(hipStream_t stream, int C, int G, int TS, const NetDesc *nets,
                                      const WorkItem *work, int n_work, size_t lds);
+// dynamic LDS of the FP32 4-state adjoint (P/Q tables and pass slots)
+size_t adjoint_persistent_lds(int E, int C, int TS);
 
 // Kernel launchers (kernels.hip)
 void launch_pmatrices(Engine &e);

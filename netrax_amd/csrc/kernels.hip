@@ -1040,6 +1040,26 @@ __device__ __forceinline__ void finish_delivery(GAS float *__restrict__ dst_m, G
   if (lane0) *dst_e = eb;
 }
 
+// A child's only delivery handed over through LDS (see OP_PASS_KID0):
+// normalised as finish_delivery does for a first delivery, then kept in the
+// thread's pass slot (mantissas, and the site exponent in every lane).
+template <int C>
+__device__ __forceinline__ void pass_delivery(float *__restrict__ slot_m, int *__restrict__ slot_e, float (&b)[4],
+                                              int eb) {
+  const float mb = cat_maxf<C>(fmaxf(fmaxf(fabsf(b[0]), fabsf(b[1])), fmaxf(fabsf(b[2]), fabsf(b[3]))));
+  if (mb > 0.f) {
+    int e;
+    (void)frexpf(mb, &e);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) b[i] = ldexpf(b[i], -e);
+    eb += e;
+  } else {
+    eb = EXP_EMPTY;
+  }
+  *reinterpret_cast<float4 *>(slot_m) = make_float4(b[0], b[1], b[2], b[3]);
+  *slot_e = eb;
+}
+
 template <int C, int G, int TS>
 __device__ __forceinline__ void adjoint_persistent_body(
     const CAS DevOp *__restrict__ ops, const CAS int32_t *__restrict__ group_begin, int n_groups,
@@ -1050,8 +1070,9 @@ __device__ __forceinline__ void adjoint_persistent_body(
     const GAS double *__restrict__ freqs, const GAS double *__restrict__ weights,
     const GAS double *__restrict__ beta, int n_root, int n_uses, GAS double *__restrict__ grad_partials,
     int tile) {
-  float *sP = dyn_lds<float>();  // [edge][cat][16] then Q: [cat][16]
+  float *sP = dyn_lds<float>();  // [edge][cat][16] then Q: [cat][16], then the pass slots
   constexpr int PWB = TS * C / 64;  // waves of this site block
+  constexpr int NT = TS * C;
   const int64_t site0 = static_cast<int64_t>(tile) * TS;
   const int part = tile_part[site0 / TILE];
   const int np = L.n_edges * C * 16;
@@ -1061,6 +1082,9 @@ __device__ __forceinline__ void adjoint_persistent_body(
   __syncthreads();
   const int c = threadIdx.x % C;
   const float *sQ = sP + np + c * 16;
+  // pass slots: mantissas [slot][thread][4], exponents [slot][thread]
+  float *pass_m = sP + ((np + C * 16 + 3) & ~3);
+  int *pass_e = reinterpret_cast<int *>(pass_m + PASS_SLOTS * NT * 4);
   const int64_t site = site0 + threadIdx.x / C;
   const int64_t n = L.n_sites_pad;
   const int wave = threadIdx.x / 64;
@@ -1084,8 +1108,18 @@ __device__ __forceinline__ void adjoint_persistent_body(
         const DevOp op = ops[g0 + j];
         kq[j] = scl[static_cast<size_t>(op.out) * n + site];
         if (op.flags & OP_HAS_CONSUMERS) {
-          load4(adj + (static_cast<size_t>(op.out) * n + site) * row + c * 4, aq[j]);
-          gq[j] = adj_exp[static_cast<size_t>(op.out) * n + site];
+          if (op.flags & OP_A_PASS) {
+            const int q = pass_slot(op.flags, 2) * NT + threadIdx.x;
+            const float4 v = reinterpret_cast<const float4 *>(pass_m)[q];
+            aq[j][0] = v.x;
+            aq[j][1] = v.y;
+            aq[j][2] = v.z;
+            aq[j][3] = v.w;
+            gq[j] = pass_e[q];
+          } else {
+            load4(adj + (static_cast<size_t>(op.out) * n + site) * row + c * 4, aq[j]);
+            gq[j] = adj_exp[static_cast<size_t>(op.out) * n + site];
+          }
         }
         if (op.root >= 0) load4(clv + (static_cast<size_t>(op.out) * n + site) * row + c * 4, xq[j]);
       }
@@ -1181,7 +1215,10 @@ __device__ __forceinline__ void adjoint_persistent_body(
             double gv = wave_sum(ldexp(static_cast<double>(gd), ez));
             if ((threadIdx.x & 63) == 0) grad_partials[grad_index(tile, n_uses, op.gslot[kk], PWB, wave)] = gv;
             const int32_t kid = op.kid[kk];
-            if (kid >= 0) {
+            if (kid >= 0 && (op.flags & (kk == 0 ? OP_PASS_KID0 : OP_PASS_KID1))) {
+              const int q = pass_slot(op.flags, kk) * NT + threadIdx.x;
+              pass_delivery<C>(pass_m + 4 * q, pass_e + q, back, ez);
+            } else if (kid >= 0) {
               GAS float *dst_m = adj + (static_cast<size_t>(kid) * n + site) * row + c * 4;
               GAS int32_t *dst_e = adj_exp + static_cast<size_t>(kid) * n + site;
               const bool first = (op.flags & (kk == 0 ? OP_FIRST_KID0 : OP_FIRST_KID1)) != 0;
@@ -2045,7 +2082,13 @@ void launch_adjoint_persistent(Engine &e) {
   launch_adjoint_persistent_multi(e.stream, e.C, e.tune_group, e.tune_tile,
                                   e.d_netdesc.as<NetDesc>(), e.d_work.as<WorkItem>(),
                                   static_cast<int>(e.L.n_sites_pad / e.tune_tile),
-                                  sizeof(float) * (e.E * e.C * 16 + e.C * 16));
+                                  adjoint_persistent_lds(e.E, e.C, e.tune_tile));
+}
+
+size_t adjoint_persistent_lds(int E, int C, int TS) {
+  // P and Q tables, padded to 16 bytes, then PASS_SLOTS slots of 16 + 4 bytes per thread
+  const size_t tables = (static_cast<size_t>(E) * C * 16 + C * 16 + 3) & ~size_t(3);
+  return sizeof(float) * tables + static_cast<size_t>(PASS_SLOTS) * TS * C * (4 * sizeof(float) + sizeof(int));
 }
 
 void launch_forward_persistent_multi(hipStream_t stream, int C, int G, int TS, const NetDesc *nets,
