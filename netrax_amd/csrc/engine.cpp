@@ -183,11 +183,12 @@ static void set_acc_early(std::vector<DevOp> &ops, const std::vector<int32_t> &g
 }
 
 // OP_PASS_* for the FP32 4-state reverse sweep (groups backwards): a
-// delivery to a child whose only consumer is this op goes to one of
-// PASS_SLOTS per-thread LDS slots, which the child's op reads in its group's
-// load phase.  A slot is reusable from the group in which its child loads
+// delivery to a child whose only consumer is this op goes to one of n_slots
+// (at most PASS_SLOTS) per-thread LDS slots, which the child's op reads in
+// its group's load phase.  A slot is reusable from the group in which its child loads
 // it (that load precedes the group's deliveries).  Greedy in walk order.
-static void set_pass_slots(std::vector<DevOp> &ops, const std::vector<int32_t> &group_begin, int nv) {
+static void set_pass_slots(std::vector<DevOp> &ops, const std::vector<int32_t> &group_begin, int nv,
+                           int n_slots) {
   std::vector<int> consumers(nv, 0), op_of(nv, -1), group_of(ops.size(), -1);
   const int ng = static_cast<int>(group_begin.size()) - 1;
   for (int g = 0; g < ng; ++g)
@@ -199,8 +200,7 @@ static void set_pass_slots(std::vector<DevOp> &ops, const std::vector<int32_t> &
     for (int k = 0; k < op.nk; ++k)
       if (op.kid[k] >= 0) consumers[op.kid[k]]++;
   for (DevOp &op : ops) op.flags &= ~(OP_PASS_KID0 | OP_PASS_KID1 | OP_A_PASS | (0x3F << PASS_SHIFT));
-  std::array<int, PASS_SLOTS> busy_until;  // sweep position of the group that reads the slot
-  busy_until.fill(-1);
+  std::vector<int> busy_until(n_slots, -1);  // sweep position of the group that reads the slot
   for (int g = ng - 1, t = 0; g >= 0; --g, ++t)
     for (int i = group_begin[g]; i < group_begin[g + 1]; ++i) {
       DevOp &op = ops[i];
@@ -209,7 +209,7 @@ static void set_pass_slots(std::vector<DevOp> &ops, const std::vector<int32_t> &
         if (kid < 0 || consumers[kid] != 1 || op_of[kid] < 0) continue;
         const int tc = group_of[op_of[kid]];
         if (tc <= t) continue;  // not later in the sweep (cannot happen for a child)
-        for (int q = 0; q < PASS_SLOTS; ++q) {
+        for (int q = 0; q < n_slots; ++q) {
           if (busy_until[q] > t) continue;
           busy_until[q] = tc;
           op.flags |= (k == 0 ? OP_PASS_KID0 : OP_PASS_KID1) | (q << (PASS_SHIFT + 2 * k));
@@ -369,7 +369,10 @@ static void build_schedule(Engine &e) {
     for (size_t g = 0; g < group_indep.size(); ++g)
       if (group_indep[g]) e.ops_dfs[e.group_begin[g]].flags |= OP_GROUP_INDEP;
     set_acc_early(e.ops_dfs, e.group_begin, nv);
-    if (e.S == 4) set_pass_slots(e.ops_dfs, e.group_begin, nv);
+    e.pass_slots = 0;
+    if (e.S == 4 && e.opt.precision == NRX_FP32)
+      e.pass_slots = adjoint_pass_slots(e.C, e.tune_group, e.tune_tile, e.E, e.L.n_sites_pad / e.tune_tile);
+    set_pass_slots(e.ops_dfs, e.group_begin, nv, e.pass_slots);
   }
 
   // gradient rows, one per (op, child), numbered in the order this engine's
@@ -458,6 +461,7 @@ NetDesc make_netdesc(const Engine &e) {
   d.n_groups = static_cast<int32_t>(e.group_begin.size()) - 1;
   d.n_root = static_cast<int32_t>(e.root_slots.size());
   d.n_uses = e.n_uses;
+  d.n_pass = e.pass_slots;
   d.clv = e.d_clv.as<float>();
   d.scl = e.d_scl.as<int32_t>();
   d.adj = e.d_adj.as<float>();
@@ -918,6 +922,23 @@ static void build_batch(Batch &b) {
   b.versions.resize(n);
   b.G = b.engines[0]->tune_group;
   b.TS = b.engines[0]->tune_tile;
+  // One batch launch holds far more workgroups than one engine's, so each
+  // engine keeps only as many LDS pass slots as leave the launch's
+  // workgroups per CU unchanged (a slot given up moves its deliveries back
+  // to HBM); its own graphs then re-capture.
+  int64_t total_work = 0;
+  for (const Engine *e : b.engines) total_work += e->L.n_sites_pad / b.TS;
+  for (Engine *e : b.engines) {
+    if (e->S != 4 || e->tune_group != b.G || e->tune_tile != b.TS) continue;
+    const int k = adjoint_pass_slots(e->C, b.G, b.TS, e->E, total_work);
+    if (k >= e->pass_slots) continue;
+    e->pass_slots = k;
+    set_pass_slots(e->ops_dfs, e->group_begin, static_cast<int>(e->dag.var.size()), k);
+    upload(e->d_ops_dfs, e->ops_dfs, e->stream);
+    upload_netdesc(*e);
+    NRX_HIP(hipStreamSynchronize(e->stream));
+    e->schedule_version++;
+  }
   for (int i = 0; i < n; ++i) {
     const Engine &e = *b.engines[i];
     if (e.tune_group != b.G || e.tune_tile != b.TS)
@@ -933,7 +954,7 @@ static void build_batch(Batch &b) {
     b.max_edges = std::max(b.max_edges, e.E);
     b.max_res = std::max(b.max_res, e.rl.size);
     b.lds_fwd = std::max(b.lds_fwd, sizeof(float) * e.E * e.C * 16);
-    b.lds_adj = std::max(b.lds_adj, adjoint_persistent_lds(e.E, e.C, b.TS));
+    b.lds_adj = std::max(b.lds_adj, adjoint_persistent_lds(e.E, e.C, b.TS, e.pass_slots));
   }
   std::stable_sort(order.begin(), order.end(),
                    [](const std::pair<int, int> &x, const std::pair<int, int> &y) {

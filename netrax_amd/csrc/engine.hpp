@@ -101,7 +101,7 @@ constexpr int32_t OP_ACC_EARLY1 = 32;
 constexpr int32_t OP_PASS_KID0 = 64;
 constexpr int32_t OP_PASS_KID1 = 128;
 constexpr int32_t OP_A_PASS = 256;
-constexpr int PASS_SLOTS = 4;
+constexpr int PASS_SLOTS = 4;  // at most; fewer where LDS would cost residency
 constexpr int PASS_SHIFT = 12;  // slot of use u (0, 1: children, 2: own) in bits 12 + 2u
 inline __host__ __device__ int pass_slot(int32_t flags, int u) { return (flags >> (PASS_SHIFT + 2 * u)) & 3; }
 
@@ -148,7 +148,7 @@ struct NetDesc {
   int32_t n_groups;
   int32_t n_root;
   int32_t n_uses;  // gradient rows
-  int32_t pad_;
+  int32_t n_pass;  // LDS pass slots of the 4-state reverse sweep (layout of its dynamic LDS)
   float *clv;
   int32_t *scl;
   float *adj;
@@ -206,7 +206,8 @@ void launch_forward_persistent_multi(hipStream_t stream, int C, int G, int TS, c
 void launch_adjoint_persistent_multi(hipStream_t stream, int C, int G, int TS, const NetDesc *nets,
                                      const WorkItem *work, int n_work, size_t lds);
 // dynamic LDS of the FP32 4-state adjoint (P/Q tables and pass slots)
-size_t adjoint_persistent_lds(int E, int C, int TS);
+size_t adjoint_persistent_lds(int E, int C, int TS, int n_pass);
+int adjoint_pass_slots(int C, int G, int TS, int E, int64_t n_work);
 
 // Kernel launchers (kernels.hip)
 void launch_pmatrices(Engine &e);

@@ -86,6 +86,7 @@ struct Engine : nrx_engine {
   int tune_group = 4;
   int tune_tile = TILE;
   int schedule_version = 0;  // bumped by every schedule rebuild
+  int pass_slots = 0;        // LDS pass slots the FP32 4-state reverse sweep uses
 
   // branch derivatives: sumtables of one branch for every displayed tree
   // containing it (valid while nothing but that branch's length changes)

@@ -1007,35 +1007,47 @@ __global__ void __launch_bounds__(TS *C)
 // mantissas with a per-site exponent g: dJ/d(stored CLV) = stored * 2^g,
 // renormalised at every write; contributions are aligned to the larger
 // exponent before they are added, so no intermediate can overflow.
+// Site maximum of |x| over the C category lanes of a site.
+template <int C>
+__device__ __forceinline__ float site_absmax(const float (&x)[4]) {
+  return cat_maxf<C>(fmaxf(fmaxf(fabsf(x[0]), fabsf(x[1])), fmaxf(fabsf(x[2]), fabsf(x[3]))));
+}
+
+// x 2^ex + y 2^ey with both aligned to the larger exponent, which is
+// returned.  Branch-free (one side shifts by zero): the comparison differs
+// between the sites of a wave.
+__device__ __forceinline__ int align_add(float (&x)[4], int ex, const float (&y)[4], int ey) {
+  const int e = max(ex, ey);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) x[i] = ldexpf(x[i], ex - e) + ldexpf(y[i], ey - e);
+  return e;
+}
+
+// Scales x so that its site maximum m of |x| lands in [0.5, 1) and returns
+// the new exponent; EXP_EMPTY for an all-zero site (frexp of 0 gives 0, so
+// the scaling is then the identity).
+__device__ __forceinline__ int renorm(float (&x)[4], int ex, float m) {
+  int e = 0;
+  (void)frexpf(m, &e);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) x[i] = ldexpf(x[i], -e);
+  return m > 0.f ? ex + e : EXP_EMPTY;
+}
+
 // Adds delivered vector b (adjoint exponent eb) to the accumulated value
 // (acc, ea) unless this is the child's first delivery, renormalises and
-// stores.
+// stores.  An all-zero delivery takes exponent EXP_EMPTY so that it cannot
+// scale the accumulated value away.
 template <int C>
 __device__ __forceinline__ void finish_delivery(GAS float *__restrict__ dst_m, GAS int32_t *__restrict__ dst_e,
                                                 bool first, bool lane0, float (&b)[4], int eb,
                                                 const float (&acc)[4], int ea) {
-  float mb = cat_maxf<C>(fmaxf(fmaxf(fabsf(b[0]), fabsf(b[1])), fmaxf(fabsf(b[2]), fabsf(b[3]))));
-  if (!(mb > 0.f)) eb = EXP_EMPTY;
-  if (!first) {
-    if (ea > eb) {
-#pragma unroll
-      for (int i = 0; i < 4; ++i) b[i] = acc[i] + ldexpf(b[i], eb - ea);
-      eb = ea;
-    } else {
-#pragma unroll
-      for (int i = 0; i < 4; ++i) b[i] += ldexpf(acc[i], ea - eb);
-    }
-    mb = cat_maxf<C>(fmaxf(fmaxf(fabsf(b[0]), fabsf(b[1])), fmaxf(fabsf(b[2]), fabsf(b[3]))));
+  float mb = site_absmax<C>(b);
+  if (!first) {  // uniform: a flag of the op
+    eb = align_add(b, mb > 0.f ? eb : EXP_EMPTY, acc, ea);
+    mb = site_absmax<C>(b);
   }
-  if (mb > 0.f) {
-    int e;
-    (void)frexpf(mb, &e);
-#pragma unroll
-    for (int i = 0; i < 4; ++i) b[i] = ldexpf(b[i], -e);
-    eb += e;
-  } else {
-    eb = EXP_EMPTY;
-  }
+  eb = renorm(b, eb, mb);
   store4(dst_m, b);
   if (lane0) *dst_e = eb;
 }
@@ -1046,16 +1058,7 @@ __device__ __forceinline__ void finish_delivery(GAS float *__restrict__ dst_m, G
 template <int C>
 __device__ __forceinline__ void pass_delivery(float *__restrict__ slot_m, int *__restrict__ slot_e, float (&b)[4],
                                               int eb) {
-  const float mb = cat_maxf<C>(fmaxf(fmaxf(fabsf(b[0]), fabsf(b[1])), fmaxf(fabsf(b[2]), fabsf(b[3]))));
-  if (mb > 0.f) {
-    int e;
-    (void)frexpf(mb, &e);
-#pragma unroll
-    for (int i = 0; i < 4; ++i) b[i] = ldexpf(b[i], -e);
-    eb += e;
-  } else {
-    eb = EXP_EMPTY;
-  }
+  eb = renorm(b, eb, site_absmax<C>(b));
   *reinterpret_cast<float4 *>(slot_m) = make_float4(b[0], b[1], b[2], b[3]);
   *slot_e = eb;
 }
@@ -1068,8 +1071,8 @@ __device__ __forceinline__ void adjoint_persistent_body(
     const GAS uint8_t *__restrict__ tips, const GAS float *__restrict__ Pf,
     const GAS float *__restrict__ Qf, const GAS int32_t *__restrict__ tile_part,
     const GAS double *__restrict__ freqs, const GAS double *__restrict__ weights,
-    const GAS double *__restrict__ beta, int n_root, int n_uses, GAS double *__restrict__ grad_partials,
-    int tile) {
+    const GAS double *__restrict__ beta, int n_root, int n_uses, int n_pass,
+    GAS double *__restrict__ grad_partials, int tile) {
   float *sP = dyn_lds<float>();  // [edge][cat][16] then Q: [cat][16], then the pass slots
   constexpr int PWB = TS * C / 64;  // waves of this site block
   constexpr int NT = TS * C;
@@ -1084,7 +1087,7 @@ __device__ __forceinline__ void adjoint_persistent_body(
   const float *sQ = sP + np + c * 16;
   // pass slots: mantissas [slot][thread][4], exponents [slot][thread]
   float *pass_m = sP + ((np + C * 16 + 3) & ~3);
-  int *pass_e = reinterpret_cast<int *>(pass_m + PASS_SLOTS * NT * 4);
+  int *pass_e = reinterpret_cast<int *>(pass_m + n_pass * NT * 4);
   const int64_t site = site0 + threadIdx.x / C;
   const int64_t n = L.n_sites_pad;
   const int wave = threadIdx.x / 64;
@@ -1188,14 +1191,7 @@ __device__ __forceinline__ void adjoint_persistent_body(
 #pragma unroll
             for (int i = 0; i < 4; ++i) sd[i] = static_cast<float>(mf * pi[i]);
           }
-          if (ef > ea) {
-#pragma unroll
-            for (int i = 0; i < 4; ++i) a[i] = sd[i] + ldexpf(a[i], ea - ef);
-            ea = ef;
-          } else {
-#pragma unroll
-            for (int i = 0; i < 4; ++i) a[i] += ldexpf(sd[i], ef - ea);
-          }
+          ea = align_add(a, ea, sd, ef);
         }
         float y[2][4];
 #pragma unroll
@@ -1248,7 +1244,7 @@ __global__ void __launch_bounds__(TS *C)
                                     to_global(nd.adj_exp), to_global(nd.tips), to_global(nd.Pf),
                                     to_global(nd.Qf), to_global(nd.tile_part), to_global(nd.freqs),
                                     to_global(nd.weights), to_global(nd.beta), nd.n_root, nd.n_uses,
-                                    to_global(nd.grad_partials), wi.tile);
+                                    nd.n_pass, to_global(nd.grad_partials), wi.tile);
 }
 
 // ============================================= FP32 form, 20 states (MFMA)
@@ -2082,13 +2078,43 @@ void launch_adjoint_persistent(Engine &e) {
   launch_adjoint_persistent_multi(e.stream, e.C, e.tune_group, e.tune_tile,
                                   e.d_netdesc.as<NetDesc>(), e.d_work.as<WorkItem>(),
                                   static_cast<int>(e.L.n_sites_pad / e.tune_tile),
-                                  adjoint_persistent_lds(e.E, e.C, e.tune_tile));
+                                  adjoint_persistent_lds(e.E, e.C, e.tune_tile, e.pass_slots));
 }
 
-size_t adjoint_persistent_lds(int E, int C, int TS) {
-  // P and Q tables, padded to 16 bytes, then PASS_SLOTS slots of 16 + 4 bytes per thread
+size_t adjoint_persistent_lds(int E, int C, int TS, int n_pass) {
+  // P and Q tables, padded to 16 bytes, then n_pass slots of 16 + 4 bytes per thread
   const size_t tables = (static_cast<size_t>(E) * C * 16 + C * 16 + 3) & ~size_t(3);
-  return sizeof(float) * tables + static_cast<size_t>(PASS_SLOTS) * TS * C * (4 * sizeof(float) + sizeof(int));
+  return sizeof(float) * tables + static_cast<size_t>(n_pass) * TS * C * (4 * sizeof(float) + sizeof(int));
+}
+
+// Pass slots for a reverse-sweep launch of n_work workgroups on the current
+// device: the most slots (at most PASS_SLOTS) whose LDS still lets as many
+// workgroups per CU be resident as the launch can use (its registers and
+// tables allow, and its size fills); 0 when the tables alone do not fit.
+int adjoint_pass_slots(int C, int G, int TS, int E, int64_t n_work) {
+  int dev = 0, n_cu = 0;
+  NRX_HIP(hipGetDevice(&dev));
+  NRX_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
+  int best = 0;
+  dispatch_shape(C, G, TS, [&](auto c, auto g, auto ts) {
+    constexpr int CC = decltype(c)::value, GG = decltype(g)::value, T = decltype(ts)::value;
+    auto resident = [&](int k) {
+      int nb = 0;
+      NRX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &nb, reinterpret_cast<const void *>(&k_adjoint_persistent<CC, GG, T>), T * CC,
+          adjoint_persistent_lds(E, C, TS, k)));
+      return nb;
+    };
+    const int r0 = resident(0);
+    if (r0 == 0) return;
+    const int64_t want = std::min<int64_t>(r0, (n_work + n_cu - 1) / std::max(1, n_cu));
+    for (int k = PASS_SLOTS; k > 0; --k)
+      if (resident(k) >= want) {
+        best = k;
+        return;
+      }
+  });
+  return best;
 }
 
 void launch_forward_persistent_multi(hipStream_t stream, int C, int G, int TS, const NetDesc *nets,

@@ -58,3 +58,35 @@ def test_batch_runs_with_a_tuned_shape():
     with pytest.raises(Exception):
         NetworkBatch(anns)
     batch.close()
+
+
+def test_batch_gives_up_lds_pass_slots_and_results_stay_the_same():
+    """A batch launch holds many more workgroups than one engine's, so each
+    engine keeps only as many LDS pass slots as leave the workgroups
+    resident per CU unchanged.  Deliveries moved back to HBM are normalised
+    the same way, so the batch and the engines alone (before and after)
+    agree to the batch tolerance."""
+    print("batch pass slots: 4 networks, 50-60 taxa, 30k sites", flush=True)
+    cands = candidate_batch(1, 4, seed=21, taxa=(50, 60), retic=(2, 4), sites=(30000, 30000))
+    anns = [AnnotatedNetwork(n, p, NetraxOptions(precision=Precision.FP32)) for n, p in cands]
+    for a in anns:
+        a.set_tuning(2, 64)
+    solo = [a.gradient() for a in anns]
+    before = [a.stats().n_lds_handoffs for a in anns]
+    assert min(before) > 0
+    batch = NetworkBatch(anns)
+    after = [a.stats().n_lds_handoffs for a in anns]
+    print("hand-offs before", before, "after", after, flush=True)
+    assert all(x <= y for x, y in zip(after, before))
+    assert sum(after) < sum(before)  # over 768 tiles fill every CU: fewer slots
+    lnl, dbs, _, _ = batch.evaluate(0)
+    for i, a in enumerate(anns):
+        l0, g0, _ = solo[i]
+        assert abs(lnl[i] - l0) <= 1e-9 * abs(l0), (i, lnl[i], l0)
+        assert np.allclose(dbs[i], g0, rtol=1e-6, atol=1e-6 * max(1.0, np.abs(g0).max()))
+        l1, g1, _ = a.gradient()
+        assert abs(l1 - l0) <= 1e-9 * abs(l0), (i, l1, l0)
+        assert np.allclose(g1, g0, rtol=1e-6, atol=1e-6 * max(1.0, np.abs(g0).max()))
+    batch.close()
+    for a in anns:
+        a.close()
