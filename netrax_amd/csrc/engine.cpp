@@ -182,42 +182,50 @@ static void set_acc_early(std::vector<DevOp> &ops, const std::vector<int32_t> &g
   }
 }
 
-// OP_PASS_* for the FP32 4-state reverse sweep (groups backwards): a
-// delivery to a child whose only consumer is this op goes to one of n_slots
-// (at most PASS_SLOTS) per-thread LDS slots, which the child's op reads in
-// its group's load phase.  A slot is reusable from the group in which its child loads
-// it (that load precedes the group's deliveries).  Greedy in walk order.
+// OP_PASS_* for the FP32 4-state reverse sweep (groups backwards): every
+// delivery to a child held in one of n_slots (at most PASS_SLOTS) per-thread
+// LDS slots goes there instead of HBM.  The child's first delivery writes
+// the slot, later ones add to it, and the child's op reads it in its group's
+// load phase.  A slot is held from the group of the child's first delivery
+// to the group of the child's op; that op's load precedes the group's
+// deliveries, so the slot can take a new child in that same group.
+// Children are taken by earliest release (interval scheduling, which keeps
+// the most children for a number of slots), each in the free slot released
+// last.
 static void set_pass_slots(std::vector<DevOp> &ops, const std::vector<int32_t> &group_begin, int nv,
                            int n_slots) {
-  std::vector<int> consumers(nv, 0), op_of(nv, -1), group_of(ops.size(), -1);
   const int ng = static_cast<int>(group_begin.size()) - 1;
-  for (int g = 0; g < ng; ++g)
-    for (int i = group_begin[g]; i < group_begin[g + 1]; ++i) {
-      group_of[i] = ng - 1 - g;  // position of the group in the reverse sweep
-      op_of[ops[i].out] = i;
-    }
-  for (const DevOp &op : ops)
-    for (int k = 0; k < op.nk; ++k)
-      if (op.kid[k] >= 0) consumers[op.kid[k]]++;
-  for (DevOp &op : ops) op.flags &= ~(OP_PASS_KID0 | OP_PASS_KID1 | OP_A_PASS | (0x3F << PASS_SHIFT));
-  std::vector<int> busy_until(n_slots, -1);  // sweep position of the group that reads the slot
+  std::vector<int> op_of(nv, -1), release(nv, -1), start(nv, -1), slot_of(nv, -1);
   for (int g = ng - 1, t = 0; g >= 0; --g, ++t)
     for (int i = group_begin[g]; i < group_begin[g + 1]; ++i) {
-      DevOp &op = ops[i];
-      for (int k = 0; k < op.nk; ++k) {
-        const int32_t kid = op.kid[k];
-        if (kid < 0 || consumers[kid] != 1 || op_of[kid] < 0) continue;
-        const int tc = group_of[op_of[kid]];
-        if (tc <= t) continue;  // not later in the sweep (cannot happen for a child)
-        for (int q = 0; q < n_slots; ++q) {
-          if (busy_until[q] > t) continue;
-          busy_until[q] = tc;
-          op.flags |= (k == 0 ? OP_PASS_KID0 : OP_PASS_KID1) | (q << (PASS_SHIFT + 2 * k));
-          ops[op_of[kid]].flags |= OP_A_PASS | (q << (PASS_SHIFT + 4));
-          break;
-        }
-      }
+      op_of[ops[i].out] = i;
+      release[ops[i].out] = t;  // sweep position of the child's own op
+      for (int k = 0; k < ops[i].nk; ++k)
+        if (ops[i].kid[k] >= 0 && start[ops[i].kid[k]] < 0) start[ops[i].kid[k]] = t;
     }
+  for (DevOp &op : ops) op.flags &= ~(OP_PASS_KID0 | OP_PASS_KID1 | OP_A_PASS | (0x3F << PASS_SHIFT));
+  std::vector<int> cand;
+  for (int v = 0; v < nv; ++v)
+    if (start[v] >= 0 && op_of[v] >= 0 && release[v] > start[v]) cand.push_back(v);
+  std::stable_sort(cand.begin(), cand.end(), [&](int x, int y) { return release[x] < release[y]; });
+  std::vector<int> busy_until(n_slots, -1);  // sweep position of the group that reads the slot
+  for (int v : cand) {
+    int best = -1;
+    for (int q = 0; q < n_slots; ++q)
+      if (busy_until[q] <= start[v] && (best < 0 || busy_until[q] > busy_until[best])) best = q;
+    if (best < 0) continue;
+    busy_until[best] = release[v];
+    slot_of[v] = best;
+  }
+  for (DevOp &op : ops) {
+    for (int k = 0; k < op.nk; ++k) {
+      const int32_t kid = op.kid[k];
+      if (kid < 0 || slot_of[kid] < 0) continue;
+      op.flags |= (k == 0 ? OP_PASS_KID0 : OP_PASS_KID1) | (slot_of[kid] << (PASS_SHIFT + 2 * k));
+      op.flags &= ~(k == 0 ? OP_ACC_EARLY0 : OP_ACC_EARLY1);  // no HBM read-back
+    }
+    if (slot_of[op.out] >= 0) op.flags |= OP_A_PASS | (slot_of[op.out] << (PASS_SHIFT + 4));
+  }
 }
 
 // Builds the level schedule, the batched order, root CLV set and gradient

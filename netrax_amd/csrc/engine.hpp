@@ -94,10 +94,10 @@ constexpr int32_t OP_GROUP_INDEP = 8;
 // value may be loaded with the group's other inputs
 constexpr int32_t OP_ACC_EARLY0 = 16;
 constexpr int32_t OP_ACC_EARLY1 = 32;
-// FP32 form, 4 states: a child with this op as its only consumer receives
-// its adjoint through a per-thread LDS slot instead of HBM.  OP_PASS_KIDk:
-// deliver child k's adjoint to slot pass_slot(flags, k); OP_A_PASS: this
-// op's own adjoint is in slot pass_slot(flags, 2)
+// FP32 form, 4 states: a child whose adjoint lives in a per-thread LDS slot
+// instead of HBM.  OP_PASS_KIDk: deliver child k's adjoint to slot
+// pass_slot(flags, k) (written by the first delivery, added to by later
+// ones); OP_A_PASS: this op's own adjoint is in slot pass_slot(flags, 2)
 constexpr int32_t OP_PASS_KID0 = 64;
 constexpr int32_t OP_PASS_KID1 = 128;
 constexpr int32_t OP_A_PASS = 256;

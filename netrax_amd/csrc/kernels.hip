@@ -1052,13 +1052,20 @@ __device__ __forceinline__ void finish_delivery(GAS float *__restrict__ dst_m, G
   if (lane0) *dst_e = eb;
 }
 
-// A child's only delivery handed over through LDS (see OP_PASS_KID0):
-// normalised as finish_delivery does for a first delivery, then kept in the
-// thread's pass slot (mantissas, and the site exponent in every lane).
+// A delivery to a child whose adjoint lives in an LDS pass slot (see
+// OP_PASS_KID0): the same arithmetic as finish_delivery, on the thread's
+// slot (mantissas, and the site exponent in every lane) instead of HBM.
 template <int C>
-__device__ __forceinline__ void pass_delivery(float *__restrict__ slot_m, int *__restrict__ slot_e, float (&b)[4],
-                                              int eb) {
-  eb = renorm(b, eb, site_absmax<C>(b));
+__device__ __forceinline__ void pass_delivery(float *__restrict__ slot_m, int *__restrict__ slot_e, bool first,
+                                              float (&b)[4], int eb) {
+  float mb = site_absmax<C>(b);
+  if (!first) {  // uniform: a flag of the op
+    const float4 v = *reinterpret_cast<const float4 *>(slot_m);
+    const float acc[4] = {v.x, v.y, v.z, v.w};
+    eb = align_add(b, mb > 0.f ? eb : EXP_EMPTY, acc, *slot_e);
+    mb = site_absmax<C>(b);
+  }
+  eb = renorm(b, eb, mb);
   *reinterpret_cast<float4 *>(slot_m) = make_float4(b[0], b[1], b[2], b[3]);
   *slot_e = eb;
 }
@@ -1213,7 +1220,8 @@ __device__ __forceinline__ void adjoint_persistent_body(
             const int32_t kid = op.kid[kk];
             if (kid >= 0 && (op.flags & (kk == 0 ? OP_PASS_KID0 : OP_PASS_KID1))) {
               const int q = pass_slot(op.flags, kk) * NT + threadIdx.x;
-              pass_delivery<C>(pass_m + 4 * q, pass_e + q, back, ez);
+              pass_delivery<C>(pass_m + 4 * q, pass_e + q, (op.flags & (kk == 0 ? OP_FIRST_KID0 : OP_FIRST_KID1)) != 0,
+                               back, ez);
             } else if (kid >= 0) {
               GAS float *dst_m = adj + (static_cast<size_t>(kid) * n + site) * row + c * 4;
               GAS int32_t *dst_e = adj_exp + static_cast<size_t>(kid) * n + site;
