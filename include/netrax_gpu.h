@@ -113,8 +113,8 @@ typedef struct {
   int32_t n_root_clvs;     /* distinct CLVs the root kernel reads */
   int32_t n_uses;          /* (parent op, child) pairs walked by the adjoint */
   int32_t n_inner_uses;    /* those pairs whose child is a CLV (not a tip) */
-  int32_t n_lds_handoffs;  /* FP32 4-state adjoint: deliveries to a single-consumer
-                              child handed over through LDS instead of HBM */
+  int32_t n_lds_handoffs;  /* FP32 4-state adjoint: deliveries into a child's LDS
+                              pass slot instead of HBM */
   int64_t n_sites_padded;  /* sites over all partitions after tile padding */
   /* algorithmic HBM bytes over the real (unpadded) sites: each CLV, scaler
    * and tip row counted once per read or write an op makes */
