@@ -88,6 +88,8 @@ Engine::~Engine() {
   if (stream) (void)hipStreamSynchronize(stream);
   destroy_graphs(graph_exec);
   if (h_result) (void)hipHostFree(h_result);
+  if (h_brlen) (void)hipHostFree(h_brlen);
+  if (ev_brlen) (void)hipEventDestroy(ev_brlen);
   if (ev_begin) (void)hipEventDestroy(ev_begin);
   if (ev_end) (void)hipEventDestroy(ev_end);
   for (auto &v : ev_k)
@@ -594,6 +596,16 @@ static void upload_models(Engine &e) {
 }
 
 // P matrices for changed edges / models; marks dependent CLVs invalid.
+// Branch lengths to the device through the engine's pinned staging buffer,
+// asynchronously on stream s (no host round trip per evaluation).  The
+// buffer is rewritten only once the previous copy out of it has finished.
+static void upload_brlen(Engine &e, hipStream_t s) {
+  NRX_HIP(hipEventSynchronize(e.ev_brlen));
+  std::copy(e.brlen.begin(), e.brlen.end(), e.h_brlen);
+  NRX_HIP(hipMemcpyAsync(e.d_brlen.p, e.h_brlen, sizeof(double) * e.E, hipMemcpyHostToDevice, s));
+  NRX_HIP(hipEventRecord(e.ev_brlen, s));
+}
+
 static void update_pmatrices(Engine &e, bool do_update) {
   const bool any_edge = std::any_of(e.edge_dirty.begin(), e.edge_dirty.end(), [](char c) { return c; });
   if (!(e.model_dirty || any_edge)) return;
@@ -601,7 +613,7 @@ static void update_pmatrices(Engine &e, bool do_update) {
   mark_clv_dirty_for_edges(e);
   if (!do_update) return;
   if (e.model_dirty) upload_models(e);
-  upload(e.d_brlen, e.brlen, e.stream);
+  upload_brlen(e, e.stream);
   launch_pmatrices(e);
   if (e.opt.precision == NRX_FP32 && e.S == AA_STATES) launch_mfma_tables(e);
   e.model_dirty = false;
@@ -1036,7 +1048,7 @@ int nrx_batch_evaluate(nrx_batch *batch, int32_t variant, double *lnl, double *d
       check_ready(e);
       // models and branch lengths of every engine, as update_pmatrices does
       if (e.model_dirty) upload_models(e);
-      upload(e.d_brlen, e.brlen, e.stream);
+      upload_brlen(e, b.stream);  // ordered before the batch's kernels
       e.model_dirty = false;
       std::fill(e.edge_dirty.begin(), e.edge_dirty.end(), 0);
       rebuild |= b.versions[i] != e.schedule_version;
@@ -1201,6 +1213,7 @@ int nrx_create(const nrx_network_desc *net, const nrx_partition_desc *parts, int
     NRX_HIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
     NRX_HIP(hipEventCreate(&e->ev_begin));
     NRX_HIP(hipEventCreate(&e->ev_end));
+    NRX_HIP(hipEventCreateWithFlags(&e->ev_brlen, hipEventDisableTiming));
 
     // tips, weights, tile -> partition
     const int64_t n = L.n_sites_pad;
@@ -1252,6 +1265,8 @@ int nrx_create(const nrx_network_desc *net, const nrx_partition_desc *parts, int
     e->d_rates.alloc(sizeof(double) * n_parts * e->C);
     e->d_freqs.alloc(sizeof(double) * n_parts * e->S);
     e->d_brlen.alloc(sizeof(double) * e->E);
+    NRX_HIP(hipHostMalloc(reinterpret_cast<void **>(&e->h_brlen), sizeof(double) * std::max(1, e->E),
+                          hipHostMallocDefault));
     e->d_Qd.alloc(sizeof(double) * ss * e->C);
     if (e->opt.precision == NRX_FP32) e->d_Qf.alloc(sizeof(float) * ss * e->C);
     const size_t pm = static_cast<size_t>(n_parts) * e->E * e->C * e->S * e->S;

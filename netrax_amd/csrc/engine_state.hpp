@@ -124,6 +124,8 @@ struct Engine : nrx_engine {
   DevBuf d_site_seed;  // plain form: adjoint seed factor w / (C Ltilde) [root][site]
   DevBuf d_result;
   double *h_result = nullptr;  // pinned
+  double *h_brlen = nullptr;   // pinned staging of the branch lengths
+  hipEvent_t ev_brlen = nullptr;  // the last copy out of h_brlen
   ResultLayout rl{};
   size_t clv_elems_per_slot = 0;  // n_sites_pad * C * S
 
