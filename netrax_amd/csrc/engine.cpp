@@ -64,6 +64,10 @@ static void destroy_graphs(hipGraphExec_t (&g)[2][2]) {
     }
 }
 
+static void destroy_graphs(hipGraphExec_t (&g)[2][2][2]) {
+  for (auto &plane : g) destroy_graphs(plane);
+}
+
 // Captures the work `enqueue` puts on `stream` into an instantiated graph.
 template <typename F>
 static hipGraphExec_t capture_graph(hipStream_t stream, F &&enqueue) {
@@ -606,18 +610,25 @@ static void upload_brlen(Engine &e, hipStream_t s) {
   NRX_HIP(hipEventRecord(e.ev_brlen, s));
 }
 
-static void update_pmatrices(Engine &e, bool do_update) {
+// Host side of a P-matrix update: marks the CLVs the changed edges and
+// models touch, and (do_update) stages the models and branch lengths on the
+// stream.  Returns whether the P kernels (enqueue_pmatrices) must run.
+static bool stage_pmatrices(Engine &e, bool do_update) {
   const bool any_edge = std::any_of(e.edge_dirty.begin(), e.edge_dirty.end(), [](char c) { return c; });
-  if (!(e.model_dirty || any_edge)) return;
+  if (!(e.model_dirty || any_edge)) return false;
   if (e.model_dirty) std::fill(e.clv_dirty.begin(), e.clv_dirty.end(), 1);
   mark_clv_dirty_for_edges(e);
-  if (!do_update) return;
+  if (!do_update) return false;
   if (e.model_dirty) upload_models(e);
   upload_brlen(e, e.stream);
-  launch_pmatrices(e);
-  if (e.opt.precision == NRX_FP32 && e.S == AA_STATES) launch_mfma_tables(e);
   e.model_dirty = false;
   std::fill(e.edge_dirty.begin(), e.edge_dirty.end(), 0);
+  return true;
+}
+
+static void enqueue_pmatrices(Engine &e) {
+  launch_pmatrices(e);
+  if (e.opt.precision == NRX_FP32 && e.S == AA_STATES) launch_mfma_tables(e);
 }
 
 static void record_kernel_begin(Engine &e, int kind) {
@@ -714,18 +725,24 @@ static int evaluate(Engine &e, int variant, bool incremental, bool update_pm, bo
   if (variant != NRX_AVERAGE_DISPLAYED_TREES && variant != NRX_BEST_DISPLAYED_TREE)
     throw std::invalid_argument("unknown likelihood variant");
   begin_timing(e);
-  update_pmatrices(e, update_pm);
+  const bool pm = stage_pmatrices(e, update_pm);
+  // an FP32 evaluation replayed from a graph runs the P kernels inside it
+  const bool pm_in_graph = pm && e.opt.precision == NRX_FP32 && e.opt.use_graph;
+  if (pm && !pm_in_graph) enqueue_pmatrices(e);
   const bool any_dirty = !incremental || std::any_of(e.ops_all.begin(), e.ops_all.end(),
                                                      [&](const DevOp &o) { return e.clv_dirty[o.out]; });
   if (!gradient && !any_dirty && !e.probs_dirty && e.cache_valid && e.cache_variant == variant) {
+    if (pm_in_graph) enqueue_pmatrices(e);  // staged above: the P matrices must follow
     end_timing(e);
     return NRX_OK;  // nothing changed since the last evaluation
   }
+  bool results_copied = false;
   if (e.opt.precision == NRX_FP32) {
     // batched form: the whole op list in one persistent launch, root site
     // sums fused into it; every CLV is recomputed
     const bool aa32 = e.S == AA_STATES;
-    auto enqueue = [&e, aa32, variant, gradient](bool timed) {
+    auto enqueue = [&e, aa32, variant, gradient, pm_in_graph](bool timed) {
+      if (pm_in_graph) enqueue_pmatrices(e);
       if (timed) record_kernel_begin(e, 0);
       if (aa32)
         launch_forward_mfma(e);
@@ -749,9 +766,16 @@ static int evaluate(Engine &e, int variant, bool incremental, bool update_pm, bo
         destroy_graphs(e.graph_exec);
         e.graph_version = e.schedule_version;
       }
-      hipGraphExec_t &ge = e.graph_exec[variant][gradient ? 1 : 0];
-      if (!ge) ge = capture_graph(e.stream, [&]() { enqueue(false); });
+      hipGraphExec_t &ge = e.graph_exec[variant][gradient ? 1 : 0][pm_in_graph ? 1 : 0];
+      if (!ge)
+        ge = capture_graph(e.stream, [&]() {
+          enqueue(false);
+          // the result block comes back inside the graph (a copy node)
+          NRX_HIP(hipMemcpyAsync(e.h_result, e.d_result.p, sizeof(double) * e.rl.size, hipMemcpyDeviceToHost,
+                                 e.stream));
+        });
       NRX_HIP(hipGraphLaunch(ge, e.stream));
+      results_copied = true;
     } else {
       enqueue(true);
     }
@@ -771,7 +795,10 @@ static int evaluate(Engine &e, int variant, bool incremental, bool update_pm, bo
     }
   }
   end_timing(e);
-  fetch_results(e);
+  if (results_copied)
+    NRX_HIP(hipStreamSynchronize(e.stream));
+  else
+    fetch_results(e);
   e.probs_dirty = false;
   e.cache_valid = true;
   e.cache_variant = variant;

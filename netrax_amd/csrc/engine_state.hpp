@@ -131,7 +131,7 @@ struct Engine : nrx_engine {
 
   // FP32 form: instantiated graphs of one evaluation, [variant][gradient],
   // valid for schedule `graph_version`
-  hipGraphExec_t graph_exec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+  hipGraphExec_t graph_exec[2][2][2] = {};  // [variant][gradient][P matrices updated in the graph]
   int graph_version = -1;
 
   // timing
