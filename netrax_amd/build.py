@@ -17,7 +17,7 @@ BUILD_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libnetrax_gpu.so")
 EXAMPLE = os.path.join(BUILD_DIR, "mirror_example")
 
-SOURCES = ["host_model.cpp", "dag.cpp", "engine.cpp", "kernels.hip"]
+SOURCES = ["host_model.cpp", "dag.cpp", "schedule.cpp", "engine.cpp", "kernels.hip"]
 
 COMMON = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
