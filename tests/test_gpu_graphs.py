@@ -31,6 +31,14 @@ def _replays(w):
         assert lg == le and np.array_equal(gg, ge) and np.array_equal(pg, pe), (it, lg, le)
         # lnL only (no adjoint): a second captured graph
         assert g.loglikelihood(incremental=0) == e.loglikelihood(incremental=0)
+        # moved branches evaluated with the P matrices left as they are (a
+        # graph without the P kernels), then with them updated
+        for ann in (g, e):
+            ann.set_branch_lengths(b * 1.1)
+        stale = g.loglikelihood(incremental=0, update_pmatrices=0)
+        assert stale == e.loglikelihood(incremental=0, update_pmatrices=0)
+        fresh = g.loglikelihood(incremental=0)
+        assert fresh == e.loglikelihood(incremental=0) and fresh != stale
     if w.network.n_retic:
         # a reticulation probability that drops trees below the threshold
         # changes the schedule: the graphs are captured again
