@@ -205,7 +205,7 @@ int nrx_last_timing(nrx_engine *eng, nrx_timing *out);
 
 /* Shape of the FP32 form's persistent kernels.  No counterpart in the
  * reference (libpll's kernels have no launch shape); the autotuner picks
- * it per engine.  4-state data: group_width (2, 4 or 8) independent ops
+ * it per engine.  4-state data: group_width (2, 3, 4 or 8) independent ops
  * per group, whose CLV loads are issued together, and tile_sites (16, 32
  * or 64, with tile_sites * rate_cats a multiple of 64) sites per
  * workgroup, i.e. tile_sites * rate_cats / 64 wavefronts per site block.

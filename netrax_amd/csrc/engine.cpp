@@ -1488,7 +1488,7 @@ static std::vector<std::pair<int, int>> shape_candidates(const Engine &e) {
   } else {
     for (int ts : {64, 32, 16})
       if (ts * e.C % 64 == 0)
-        for (int g : {4, 2, 8}) out.push_back({g, ts});
+        for (int g : {4, 2, 3, 8}) out.push_back({g, ts});
   }
   return out;
 }

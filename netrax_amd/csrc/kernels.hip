@@ -2052,7 +2052,7 @@ void dispatch_c(int C, F &&f) {
   }
 }
 
-// Shapes of the 4-state persistent kernels: group width G in {2, 4, 8}
+// Shapes of the 4-state persistent kernels: group width G in {2, 3, 4, 8}
 // and TS in {16, 32, 64} sites per workgroup with TS C a multiple of 64.
 template <typename F>
 void dispatch_shape(int C, int G, int TS, F &&f) {
@@ -2068,6 +2068,7 @@ void dispatch_shape(int C, int G, int TS, F &&f) {
   dispatch_c(C, [&](auto c) {
     switch (G) {
       case 2: with_ts(c, std::integral_constant<int, 2>{}); break;
+      case 3: with_ts(c, std::integral_constant<int, 3>{}); break;
       case 4: with_ts(c, std::integral_constant<int, 4>{}); break;
       case 8: with_ts(c, std::integral_constant<int, 8>{}); break;
       default: throw std::invalid_argument("unsupported group width");

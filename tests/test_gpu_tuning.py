@@ -27,7 +27,7 @@ def _check_all_shapes(w):
         assert np.abs(g32 - g64).max() <= 1e-4 * scale, (gw, ts)
         assert np.abs(p32 - p64).max(initial=0.0) <= 1e-4 * max([1.0, *np.abs(p64)])
     with pytest.raises(Exception):
-        a32.set_tuning(3, 64)
+        a32.set_tuning(5, 64)
     a32.close()
     a64.close()
 
