@@ -101,9 +101,10 @@ constexpr int32_t OP_ACC_EARLY1 = 32;
 constexpr int32_t OP_PASS_KID0 = 64;
 constexpr int32_t OP_PASS_KID1 = 128;
 constexpr int32_t OP_A_PASS = 256;
-constexpr int PASS_SLOTS = 4;  // at most; fewer where LDS would cost residency
-constexpr int PASS_SHIFT = 12;  // slot of use u (0, 1: children, 2: own) in bits 12 + 2u
-inline __host__ __device__ int pass_slot(int32_t flags, int u) { return (flags >> (PASS_SHIFT + 2 * u)) & 3; }
+constexpr int PASS_SLOTS = 5;  // at most; fewer where LDS would cost residency
+constexpr int PASS_SHIFT = 12;  // slot of use u (0, 1: children, 2: own) in bits 12 + 3u
+constexpr int32_t PASS_BITS = 0x1FF << PASS_SHIFT;
+inline __host__ __device__ int pass_slot(int32_t flags, int u) { return (flags >> (PASS_SHIFT + 3 * u)) & 7; }
 
 // A launch of the plain form: the ops [op_begin, op_end) of one pruning
 // level.  The forward runs one workgroup per (tile, PLAIN_G consecutive

@@ -1054,7 +1054,8 @@ __device__ __forceinline__ void finish_delivery(GAS float *__restrict__ dst_m, G
 
 // A delivery to a child whose adjoint lives in an LDS pass slot (see
 // OP_PASS_KID0): the same arithmetic as finish_delivery, on the thread's
-// slot (mantissas, and the site exponent in every lane) instead of HBM.
+// slot instead of HBM.  slot_e is the site's exponent, shared by the C
+// lanes of the site (each writes the same value).
 template <int C>
 __device__ __forceinline__ void pass_delivery(float *__restrict__ slot_m, int *__restrict__ slot_e, bool first,
                                               float (&b)[4], int eb) {
@@ -1092,9 +1093,10 @@ __device__ __forceinline__ void adjoint_persistent_body(
   __syncthreads();
   const int c = threadIdx.x % C;
   const float *sQ = sP + np + c * 16;
-  // pass slots: mantissas [slot][thread][4], exponents [slot][thread]
+  // pass slots: mantissas [slot][thread][4], site exponents [slot][site]
   float *pass_m = sP + ((np + C * 16 + 3) & ~3);
   int *pass_e = reinterpret_cast<int *>(pass_m + n_pass * NT * 4);
+  const int site_in_tile = threadIdx.x / C;
   const int64_t site = site0 + threadIdx.x / C;
   const int64_t n = L.n_sites_pad;
   const int wave = threadIdx.x / 64;
@@ -1119,13 +1121,13 @@ __device__ __forceinline__ void adjoint_persistent_body(
         kq[j] = scl[static_cast<size_t>(op.out) * n + site];
         if (op.flags & OP_HAS_CONSUMERS) {
           if (op.flags & OP_A_PASS) {
-            const int q = pass_slot(op.flags, 2) * NT + threadIdx.x;
-            const float4 v = reinterpret_cast<const float4 *>(pass_m)[q];
+            const int slot = pass_slot(op.flags, 2);
+            const float4 v = reinterpret_cast<const float4 *>(pass_m)[slot * NT + threadIdx.x];
             aq[j][0] = v.x;
             aq[j][1] = v.y;
             aq[j][2] = v.z;
             aq[j][3] = v.w;
-            gq[j] = pass_e[q];
+            gq[j] = pass_e[slot * TS + site_in_tile];
           } else {
             load4(adj + (static_cast<size_t>(op.out) * n + site) * row + c * 4, aq[j]);
             gq[j] = adj_exp[static_cast<size_t>(op.out) * n + site];
@@ -1219,9 +1221,9 @@ __device__ __forceinline__ void adjoint_persistent_body(
             if ((threadIdx.x & 63) == 0) grad_partials[grad_index(tile, n_uses, op.gslot[kk], PWB, wave)] = gv;
             const int32_t kid = op.kid[kk];
             if (kid >= 0 && (op.flags & (kk == 0 ? OP_PASS_KID0 : OP_PASS_KID1))) {
-              const int q = pass_slot(op.flags, kk) * NT + threadIdx.x;
-              pass_delivery<C>(pass_m + 4 * q, pass_e + q, (op.flags & (kk == 0 ? OP_FIRST_KID0 : OP_FIRST_KID1)) != 0,
-                               back, ez);
+              const int slot = pass_slot(op.flags, kk);
+              pass_delivery<C>(pass_m + 4 * (slot * NT + threadIdx.x), pass_e + slot * TS + site_in_tile,
+                               (op.flags & (kk == 0 ? OP_FIRST_KID0 : OP_FIRST_KID1)) != 0, back, ez);
             } else if (kid >= 0) {
               GAS float *dst_m = adj + (static_cast<size_t>(kid) * n + site) * row + c * 4;
               GAS int32_t *dst_e = adj_exp + static_cast<size_t>(kid) * n + site;
@@ -2091,9 +2093,10 @@ void launch_adjoint_persistent(Engine &e) {
 }
 
 size_t adjoint_persistent_lds(int E, int C, int TS, int n_pass) {
-  // P and Q tables, padded to 16 bytes, then n_pass slots of 16 + 4 bytes per thread
+  // P and Q tables, padded to 16 bytes, then n_pass slots of 16 bytes per
+  // thread and 4 per site
   const size_t tables = (static_cast<size_t>(E) * C * 16 + C * 16 + 3) & ~size_t(3);
-  return sizeof(float) * tables + static_cast<size_t>(n_pass) * TS * C * (4 * sizeof(float) + sizeof(int));
+  return sizeof(float) * tables + static_cast<size_t>(n_pass) * TS * (C * 4 * sizeof(float) + sizeof(int));
 }
 
 // Pass slots for a reverse-sweep launch of n_work workgroups on the current

@@ -76,7 +76,7 @@ void set_pass_slots(std::vector<DevOp> &ops, const std::vector<int32_t> &group_b
       for (int k = 0; k < ops[i].nk; ++k)
         if (ops[i].kid[k] >= 0 && start[ops[i].kid[k]] < 0) start[ops[i].kid[k]] = t;
     }
-  for (DevOp &op : ops) op.flags &= ~(OP_PASS_KID0 | OP_PASS_KID1 | OP_A_PASS | (0x3F << PASS_SHIFT));
+  for (DevOp &op : ops) op.flags &= ~(OP_PASS_KID0 | OP_PASS_KID1 | OP_A_PASS | PASS_BITS);
   std::vector<int> cand;
   for (int v = 0; v < nv; ++v)
     if (start[v] >= 0 && op_of[v] >= 0 && release[v] > start[v]) cand.push_back(v);
@@ -94,10 +94,10 @@ void set_pass_slots(std::vector<DevOp> &ops, const std::vector<int32_t> &group_b
     for (int k = 0; k < op.nk; ++k) {
       const int32_t kid = op.kid[k];
       if (kid < 0 || slot_of[kid] < 0) continue;
-      op.flags |= (k == 0 ? OP_PASS_KID0 : OP_PASS_KID1) | (slot_of[kid] << (PASS_SHIFT + 2 * k));
+      op.flags |= (k == 0 ? OP_PASS_KID0 : OP_PASS_KID1) | (slot_of[kid] << (PASS_SHIFT + 3 * k));
       op.flags &= ~(k == 0 ? OP_ACC_EARLY0 : OP_ACC_EARLY1);  // no HBM read-back
     }
-    if (slot_of[op.out] >= 0) op.flags |= OP_A_PASS | (slot_of[op.out] << (PASS_SHIFT + 4));
+    if (slot_of[op.out] >= 0) op.flags |= OP_A_PASS | (slot_of[op.out] << (PASS_SHIFT + 6));
   }
 }
 

@@ -22,7 +22,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 SAMPLES = os.path.join(HERE, "golden", "sample_networks")
 HIP_INCLUDE = "/opt/rocm/include"
-PAIRS = [(g, s) for g in (2, 3, 4, 8) for s in (0, 1, 2, 4)]
+PAIRS = [(g, s) for g in (2, 3, 4, 8) for s in (0, 1, 2, 4, 5)]
 
 
 @pytest.fixture(scope="module")
