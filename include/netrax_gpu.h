@@ -169,7 +169,8 @@ int nrx_gradient(nrx_engine *eng, int32_t variant, double *total, double *dbrlen
                  double *dprob, double *per_partition_dbrlen);
 
 /* First and second derivative of lnL w.r.t. the length of one branch,
- * evaluated at branch length `brlen` with every other parameter fixed.
+ * evaluated at branch length `brlen` (clamped to [brlen_min, brlen_max] as
+ * nrx_set_branch_lengths clamps) with every other parameter fixed.
  * mode: NRX_DERIV_REFERENCE or NRX_DERIV_TRUE.  part_d1/part_d2 may be NULL. */
 int nrx_brlen_derivatives(nrx_engine *eng, int32_t variant, int32_t edge,
                           int32_t mode, double brlen, double *d1, double *d2,
@@ -185,7 +186,8 @@ int nrx_get_stats(nrx_engine *eng, nrx_stats *out);
  * of a search step (src/search/Filtering.cpp:236-270 scores candidates one
  * at a time): every engine of the batch is scored by one launch per kernel.
  * All engines must use NRX_FP32, 4-state data and the same number of rate
- * categories on the same device, and must outlive the batch.  With
+ * categories on the same device, must outlive the batch and may appear
+ * only once in it (a repeated engine is refused: NRX_ERR_ARG).  With
  * use_graph set on the first engine, the whole batch evaluation (P matrices,
  * persistent forward, reductions, mixture, persistent adjoint, gradient
  * totals, result gather) is one hipGraph launch. */

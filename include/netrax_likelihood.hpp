@@ -46,15 +46,21 @@ inline void nrx_check(int rc) {
 struct GpuAnnotatedNetwork {
   nrx_engine *engine = nullptr;
   LikelihoodVariant likelihood_variant = LikelihoodVariant::AVERAGE_DISPLAYED_TREES;
-  std::vector<double> linked_branch_lengths;  // [n_edges]
+  std::vector<double> linked_branch_lengths;  // [n_edges], clamped as the engine clamps
   std::vector<double> partition_loglh;        // [n_parts]
   double cached_logl = 0.0;
   bool cached_logl_valid = false;
+  double brlen_min = 1e-6, brlen_max = 100.0;
 
   GpuAnnotatedNetwork() = default;
   GpuAnnotatedNetwork(const nrx_network_desc &net, const std::vector<nrx_partition_desc> &parts,
                       const nrx_options *opt = nullptr) {
-    nrx_check(nrx_create(&net, parts.data(), static_cast<int32_t>(parts.size()), opt, &engine));
+    nrx_options o;
+    nrx_default_options(&o);
+    if (opt) o = *opt;
+    brlen_min = o.brlen_min;
+    brlen_max = o.brlen_max;
+    nrx_check(nrx_create(&net, parts.data(), static_cast<int32_t>(parts.size()), &o, &engine));
     partition_loglh.assign(parts.size(), 0.0);
     linked_branch_lengths.assign(net.n_edges, 0.1);
   }
@@ -64,9 +70,12 @@ struct GpuAnnotatedNetwork {
     if (engine) nrx_destroy(engine);
   }
 
+  // createNetworkPllTreeinfo clamps to [brlen_min, brlen_max]
+  // (RaxmlWrapper.cpp:652-663); the stored lengths are the clamped ones
   void setBranchLengths(const std::vector<double> &b) {
     nrx_check(nrx_set_branch_lengths(engine, b.data()));
     linked_branch_lengths = b;
+    for (double &x : linked_branch_lengths) x = x < brlen_min ? brlen_min : (x > brlen_max ? brlen_max : x);
     cached_logl_valid = false;
   }
   void setReticulationProbs(const std::vector<double> &first_parent_prob) {

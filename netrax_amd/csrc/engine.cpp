@@ -867,6 +867,10 @@ int nrx_batch_create(nrx_engine *const *engines, int32_t n_engines, nrx_batch **
     for (int i = 0; i < n_engines; ++i) {
       Engine *e = static_cast<Engine *>(engines[i]);
       if (!e) throw std::invalid_argument("null engine in batch");
+      // an engine listed twice would have its tiles worked by two
+      // workgroups at once, racing on its adjoint rows
+      if (std::find(b->engines.begin(), b->engines.end(), e) != b->engines.end())
+        throw std::invalid_argument("engine " + std::to_string(i) + " appears twice in the batch");
       if (e->opt.precision != NRX_FP32 || e->S != 4)
         throw std::invalid_argument("batches take FP32-form engines with 4-state data");
       if (i > 0 && (e->C != b->engines[0]->C || e->opt.device != b->engines[0]->opt.device))
@@ -1292,6 +1296,9 @@ int nrx_brlen_derivatives(nrx_engine *eng, int32_t variant, int32_t edge, int32_
     if (variant != NRX_AVERAGE_DISPLAYED_TREES && variant != NRX_BEST_DISPLAYED_TREE)
       throw std::invalid_argument("unknown likelihood variant");
     if (!(brlen >= 0.0)) throw std::invalid_argument("branch length must be >= 0");
+    // the same clamp as nrx_set_branch_lengths (RaxmlWrapper.cpp:652-663):
+    // derivatives at a length the engine would never evaluate are not given
+    brlen = std::min(std::max(brlen, e.opt.brlen_min), e.opt.brlen_max);
     if (e.opt.precision == NRX_FP32 && e.S == AA_STATES)
       return fail(NRX_ERR_UNSUPPORTED,
                   "per-branch sumtables of 20-state data need the FP64 form (the FP32 form "

@@ -287,14 +287,25 @@ class NetworkBatch:
     def __init__(self, networks: Sequence[AnnotatedNetwork]):
         self.lib = _lib.load()
         self.networks = list(networks)
+        if len({id(a) for a in self.networks}) != len(self.networks):
+            raise ValueError("a network appears twice in the batch")
+        self._check_open()
         arr = (C.c_void_p * len(self.networks))(*[a.handle.value for a in self.networks])
         h = C.c_void_p()
         _lib.check(self.lib.nrx_batch_create(arr, len(self.networks), C.byref(h)))
         self.handle = h
 
+    def _check_open(self) -> None:
+        for i, a in enumerate(self.networks):
+            if not getattr(a, "handle", None):
+                raise RuntimeError(f"network {i} of the batch is closed")
+
     def evaluate(self, variant: int = 0, gradient: bool = True):
         """(lnL [n], per-network dlnL/dbrlen list, per-network dlnL/dprob
         list, device ms); the lists are empty when gradient is False."""
+        if not getattr(self, "handle", None):
+            raise RuntimeError("the batch is closed")
+        self._check_open()
         n = len(self.networks)
         lnl = np.zeros(n)
         E = [a.network.n_edges for a in self.networks]

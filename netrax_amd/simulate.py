@@ -145,9 +145,11 @@ def simulate_states(net: Network, model: Model, n_sites: int, rng: np.random.Gen
 
 def compress_patterns(masks: np.ndarray):
     """Unique site columns with integer weights (RaxmlWrapper.cpp:330 turns
-    pattern compression on).  ``masks`` is [n_tips, n_sites]."""
-    cols, counts = np.unique(masks, axis=1, return_counts=True)
-    return np.ascontiguousarray(cols.astype(np.uint32)), counts.astype(np.uint32)
+    pattern compression on).  ``masks`` is [n_tips, n_sites]; patterns in
+    lexicographic column order, which fixes the seeded workloads."""
+    from .alignment import compress_patterns as _compress
+
+    return _compress(masks, order="sorted")
 
 
 @dataclasses.dataclass

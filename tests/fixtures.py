@@ -7,6 +7,7 @@ import os
 
 import numpy as np
 
+from netrax_amd.alignment import read_fasta
 from netrax_amd.network import parse_newick
 from netrax_amd.simulate import Model, Partition, dna_masks
 
@@ -37,18 +38,8 @@ DEAD_RETIC_NEWICK = "((A:0.2,((B:0.1)#H2:0.15::0.4)#H1:0.1::0.7):0.1,(#H1:0.3::0
 
 
 def read_alignment(fname: str) -> dict:
-    aln, lab = {}, None
-    with open(os.path.join(SAMPLES, fname)) as fh:
-        for line in fh:
-            line = line.strip()
-            if not line:
-                continue
-            if line.startswith(">"):
-                lab = line[1:].strip()
-                aln[lab] = ""
-            else:
-                aln[lab] += line
-    return aln
+    """One of the reference's sample alignments (FASTA), by file name."""
+    return read_fasta(os.path.join(SAMPLES, fname))
 
 
 def fixed_model(seed: int = 7, ncat: int = 4) -> Model:
