@@ -61,12 +61,12 @@ def run_form(w, precision, steps, warmup, use_graph=False, tune=False):
     tuning = None
     if tune and SHAPE:
         shape = SHAPE
-        tuning = {"group_width": SHAPE[0], "tile_sites": SHAPE[1], "fixed": True}
+        tuning = {"group_width": SHAPE[0], "tile_sites": SHAPE[1], "row_lanes": SHAPE[2], "fixed": True}
     elif tune:
         best, cands = tim.autotune(reps=3)
-        shape = best[:2]
-        tuning = {"group_width": best[0], "tile_sites": best[1],
-                  "candidates_ms": {f"{g}x{t}": round(ms, 4) for g, t, ms in cands}}
+        shape = best[:3]
+        tuning = {"group_width": best[0], "tile_sites": best[1], "row_lanes": best[2],
+                  "candidates_ms": {f"{g}x{t}x{h}": round(ms, 4) for g, t, h, ms in cands}}
         log(f"autotune: {best} of {cands}")
     if tune:
         ann.set_tuning(*shape)
@@ -136,7 +136,9 @@ def roofline(res, ncat=4):
     t = res.get("tuning")
     step = "forward" if name.startswith("forward") else "adjoint"
     if t:  # persistent form: one launch per evaluation
-        kname = f"k_{step}_persistent<{ncat}, {t['group_width']}, {t['tile_sites']}>"
+        split = step == "adjoint" and t.get("row_lanes", 1) == 2
+        kname = (f"k_adjoint_split<{ncat}, {t['group_width']}, {t['tile_sites']}>" if split else
+                 f"k_{step}_persistent<{ncat}, {t['group_width']}, {t['tile_sites']}>")
     else:  # plain form: one launch per pruning level
         kname = f"k_{step}_level<4, {ncat}>" if step == "adjoint" else f"k_forward_level<4, {ncat}, 4>"
     traffic = pmc_traffic(kname)
@@ -218,7 +220,7 @@ def bench_candidates(n_cand, steps, warmup):
     big = max(range(len(anns)), key=lambda i: stats[i].grad_bytes)
     best, _ = anns[big].autotune(reps=2)
     for a in anns:
-        a.set_tuning(best[0], best[1])
+        a.set_tuning(*best[:3])
     batch = NetworkBatch(anns)
     for _ in range(warmup):
         batch.evaluate(0)
@@ -254,7 +256,7 @@ def bench_candidates(n_cand, steps, warmup):
            "batch_speedup_device": round(seq_dev / (dev / steps), 3),
            "algorithmic_gbps": round(gb / (dev / steps * 1e-3) / 1e9, 1),
            "patterns_total": int(sum(s.n_sites_padded for s in stats)),
-           "tuning": {"group_width": best[0], "tile_sites": best[1]},
+           "tuning": {"group_width": best[0], "tile_sites": best[1], "row_lanes": best[2]},
            "max_rel_lnl_diff_vs_fp64_sampled": max(errs)}
     batch.close()
     for a in anns:
@@ -371,15 +373,18 @@ def main():
     ap.add_argument("--candidates", type=int, default=256)
     ap.add_argument("--stream-reference", action="store_true",
                     help="print the plain-stream HBM rates of this GPU (torch kernels) and exit")
-    ap.add_argument("--shape", default=None, metavar="GxTS",
-                    help="fixed FP32 kernel shape (ops per group x sites per workgroup) "
-                         "instead of autotuning, e.g. 2x64 (profiling runs)")
+    ap.add_argument("--shape", default=None, metavar="GxTSxH",
+                    help="fixed FP32 kernel shape (ops per group x sites per workgroup x lanes "
+                         "per row in the reverse sweep) instead of autotuning, e.g. 3x64x2 "
+                         "(profiling runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device copy of BYTES (PMC counter calibration)")
     args = ap.parse_args()
     global SHAPE
     if args.shape:
         SHAPE = tuple(int(v) for v in args.shape.lower().split("x"))
+        if len(SHAPE) == 2:
+            SHAPE = SHAPE + (1,)
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
         return 0

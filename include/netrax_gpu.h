@@ -208,14 +208,17 @@ int nrx_last_timing(nrx_engine *eng, nrx_timing *out);
 /* Shape of the FP32 form's persistent kernels.  No counterpart in the
  * reference (libpll's kernels have no launch shape); the autotuner picks
  * it per engine.  4-state data: group_width (2, 3, 4 or 8) independent ops
- * per group, whose CLV loads are issued together, and tile_sites (16, 32
+ * per group, whose CLV loads are issued together, tile_sites (16, 32
  * or 64, with tile_sites * rate_cats a multiple of 64) sites per
- * workgroup, i.e. tile_sites * rate_cats / 64 wavefronts per site block.
- * 20-state data: group_width 1, 2 or 4 and 32 sites per workgroup (one
- * matrix-instruction column block). */
+ * workgroup, i.e. tile_sites * rate_cats / 64 wavefronts per site block,
+ * and row_lanes (1 or 2) lanes per (site, category) row of 4 states in the
+ * reverse sweep (2: each lane holds two states, twice the wavefronts).
+ * 20-state data: group_width 1, 2 or 4, 32 sites per workgroup (one
+ * matrix-instruction column block) and row_lanes 1. */
 typedef struct {
   int32_t group_width;
   int32_t tile_sites;
+  int32_t row_lanes;
   double ms; /* device time of one lnL + gradient evaluation (autotune) */
 } nrx_tuning;
 
@@ -227,7 +230,7 @@ int nrx_autotune(nrx_engine *eng, int32_t variant, int32_t reps, nrx_tuning *bes
                  nrx_tuning *results, int32_t max_results, int32_t *n_results);
 /* Sets a shape (e.g. one found by an earlier nrx_autotune); NRX_ERR_ARG if
  * the engine's data cannot take it.  Engines of one batch must share it. */
-int nrx_set_tuning(nrx_engine *eng, int32_t group_width, int32_t tile_sites);
+int nrx_set_tuning(nrx_engine *eng, int32_t group_width, int32_t tile_sites, int32_t row_lanes);
 int nrx_get_tuning(nrx_engine *eng, nrx_tuning *out);
 
 #ifdef __cplusplus

@@ -61,7 +61,8 @@ class Timing(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("group_width", C.c_int32), ("tile_sites", C.c_int32), ("ms", C.c_double)]
+    _fields_ = [("group_width", C.c_int32), ("tile_sites", C.c_int32), ("row_lanes", C.c_int32),
+                ("ms", C.c_double)]
 
 
 _lib = None
@@ -101,7 +102,7 @@ def load(build_if_missing: bool = True):
     lib.nrx_batch_evaluate.argtypes = [C.c_void_p, C.c_int32, pd, pd, pd, pd]
     lib.nrx_autotune.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(Tuning),
                                  C.POINTER(Tuning), C.c_int32, C.POINTER(C.c_int32)]
-    lib.nrx_set_tuning.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
+    lib.nrx_set_tuning.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]
     lib.nrx_get_tuning.argtypes = [C.c_void_p, C.POINTER(Tuning)]
     _lib = lib
     return lib

@@ -218,7 +218,8 @@ static void build_schedule(Engine &e) {
     build_batched_order(e.dag, e.root_slots, proto, e.tune_group, e.ops_dfs, e.group_begin);
     e.pass_slots = 0;
     if (e.S == 4 && e.opt.precision == NRX_FP32)
-      e.pass_slots = adjoint_pass_slots(e.C, e.tune_group, e.tune_tile, e.E, e.L.n_sites_pad / e.tune_tile);
+      e.pass_slots = adjoint_pass_slots(e.C, e.tune_group, e.tune_tile, e.tune_lanes, e.E,
+                                        e.L.n_sites_pad / e.tune_tile);
     set_pass_slots(e.ops_dfs, e.group_begin, nv, e.pass_slots);
   }
 
@@ -274,7 +275,8 @@ static void build_schedule(Engine &e) {
   std::vector<int32_t> rs(e.root_slots.begin(), e.root_slots.end());
   upload(e.d_root_slots, rs, e.stream);
   // gradient partials per site block: plain form 64-site tiles of C waves;
-  // FP32 form its workgroups (tune_tile sites, or 32 for 20-state data)
+  // FP32 form its reverse-sweep workgroups (tune_tile sites of tune_lanes
+  // lanes per row, or 32 sites for 20-state data)
   if (e.opt.precision != NRX_FP32) {
     e.gpw = TILE * e.C / 64;
     e.gbpt = 1;
@@ -282,14 +284,15 @@ static void build_schedule(Engine &e) {
     e.gpw = e.C;
     e.gbpt = TILE / AA_TILE;
   } else {
-    e.gpw = e.tune_tile * e.C / 64;
+    e.gpw = e.tune_tile * e.C * e.tune_lanes / 64;
     e.gbpt = TILE / e.tune_tile;
   }
-  if (e.gpw * e.gbpt != e.pw) throw std::logic_error("gradient partial layout does not cover the tile");
+  if (e.gpw * e.gbpt < e.pw) throw std::logic_error("gradient partial layout does not cover the tile");
   const size_t n_wt = static_cast<size_t>(e.L.n_tiles) * e.pw;
+  const size_t n_gwt = static_cast<size_t>(e.L.n_tiles) * e.gpw * e.gbpt;
   e.d_root_partials.alloc(sizeof(double) * std::max<size_t>(1, rs.size()) * n_wt);
   e.d_beta.alloc(sizeof(double) * std::max<size_t>(1, rs.size()) * e.P);
-  e.d_grad_partials.alloc(sizeof(double) * std::max(1, e.n_uses) * n_wt);
+  e.d_grad_partials.alloc(sizeof(double) * std::max(1, e.n_uses) * n_gwt);
   e.d_site_seed.alloc(sizeof(double) * std::max<size_t>(1, rs.size()) * e.L.n_sites_pad);
   e.d_root_sums.alloc(sizeof(double) * std::max<size_t>(1, rs.size()) * e.P);
   e.d_use_sums.alloc(sizeof(double) * std::max(1, e.n_uses) * static_cast<size_t>(e.P));
@@ -729,7 +732,7 @@ struct Batch {
       max_res = 0;
   size_t lds_fwd = 0, lds_adj = 0;
   bool use_graph = false;
-  int G = 4, TS = TILE;  // shape of the persistent kernels, shared by all engines
+  int G = 4, TS = TILE, H = 1;  // shape of the persistent kernels, shared by all engines
   hipGraphExec_t graph_exec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [variant][gradient]
   ~Batch() {
     if (stream) (void)hipStreamSynchronize(stream);
@@ -802,6 +805,7 @@ static void build_batch(Batch &b) {
   b.versions.resize(n);
   b.G = b.engines[0]->tune_group;
   b.TS = b.engines[0]->tune_tile;
+  b.H = b.engines[0]->tune_lanes;
   // One batch launch holds far more workgroups than one engine's, so each
   // engine keeps only as many LDS pass slots as leave the launch's
   // workgroups per CU unchanged (a slot given up moves its deliveries back
@@ -809,8 +813,8 @@ static void build_batch(Batch &b) {
   int64_t total_work = 0;
   for (const Engine *e : b.engines) total_work += e->L.n_sites_pad / b.TS;
   for (Engine *e : b.engines) {
-    if (e->S != 4 || e->tune_group != b.G || e->tune_tile != b.TS) continue;
-    const int k = adjoint_pass_slots(e->C, b.G, b.TS, e->E, total_work);
+    if (e->S != 4 || e->tune_group != b.G || e->tune_tile != b.TS || e->tune_lanes != b.H) continue;
+    const int k = adjoint_pass_slots(e->C, b.G, b.TS, b.H, e->E, total_work);
     if (k >= e->pass_slots) continue;
     e->pass_slots = k;
     set_pass_slots(e->ops_dfs, e->group_begin, static_cast<int>(e->dag.var.size()), k);
@@ -821,7 +825,7 @@ static void build_batch(Batch &b) {
   }
   for (int i = 0; i < n; ++i) {
     const Engine &e = *b.engines[i];
-    if (e.tune_group != b.G || e.tune_tile != b.TS)
+    if (e.tune_group != b.G || e.tune_tile != b.TS || e.tune_lanes != b.H)
       throw std::invalid_argument("batch engines must share the FP32 kernel shape (nrx_set_tuning)");
     nets[i] = make_netdesc(e);
     small[i] = make_smalldesc(e);
@@ -927,7 +931,7 @@ int nrx_batch_evaluate(nrx_batch *batch, int32_t variant, double *lnl, double *d
       launch_root_reduce_multi(b.stream, sd, n, b.max_root_rows);
       launch_mixture_multi(b.stream, sd, n, variant);
       if (grad) {
-        launch_adjoint_persistent_multi(b.stream, b.C, b.G, b.TS, b.d_nets.as<NetDesc>(),
+        launch_adjoint_persistent_multi(b.stream, b.C, b.G, b.TS, b.H, b.d_nets.as<NetDesc>(),
                                         b.d_work.as<WorkItem>(), b.n_work, b.lds_adj);
         launch_grad_reduce_multi(b.stream, sd, n, b.max_use_rows, b.max_edges);
       }
@@ -1486,24 +1490,32 @@ int nrx_last_timing(nrx_engine *eng, nrx_timing *out) {
 namespace nrx {
 
 // Shapes of the FP32 form's persistent kernels: 4-state data takes G ops
-// per group and TS sites per workgroup (TS C threads, a multiple of 64);
-// 20-state data has 32 sites per workgroup and tunes G only.
-static std::vector<std::pair<int, int>> shape_candidates(const Engine &e) {
-  std::vector<std::pair<int, int>> out;
+// per group, TS sites per workgroup (TS C threads, a multiple of 64) and H
+// lanes per (site, category) row in the reverse sweep; 20-state data has 32
+// sites per workgroup and tunes G only.
+struct Shape {
+  int g, ts, h;
+  bool operator==(const Shape &o) const { return g == o.g && ts == o.ts && h == o.h; }
+};
+
+static std::vector<Shape> shape_candidates(const Engine &e) {
+  std::vector<Shape> out;
   if (e.S == AA_STATES) {
-    for (int g : {1, 2, 4}) out.push_back({g, AA_TILE});
+    for (int g : {1, 2, 4}) out.push_back({g, AA_TILE, 1});
   } else {
-    for (int ts : {64, 32, 16})
-      if (ts * e.C % 64 == 0)
-        for (int g : {4, 2, 3, 8}) out.push_back({g, ts});
+    for (int h : {1, 2})
+      for (int ts : {64, 32, 16})
+        if (ts * e.C % 64 == 0)
+          for (int g : {4, 2, 3, 8}) out.push_back({g, ts, h});
   }
   return out;
 }
 
-static void apply_shape(Engine &e, int g, int ts) {
-  if (e.tune_group == g && e.tune_tile == ts) return;
-  e.tune_group = g;
-  e.tune_tile = ts;
+static void apply_shape(Engine &e, const Shape &s) {
+  if (e.tune_group == s.g && e.tune_tile == s.ts && e.tune_lanes == s.h) return;
+  e.tune_group = s.g;
+  e.tune_tile = s.ts;
+  e.tune_lanes = s.h;
   build_schedule(e);
   std::fill(e.clv_dirty.begin(), e.clv_dirty.end(), 1);
   e.cache_valid = false;
@@ -1513,16 +1525,17 @@ static void apply_shape(Engine &e, int g, int ts) {
 
 extern "C" {
 
-int nrx_set_tuning(nrx_engine *eng, int32_t group_width, int32_t tile_sites) {
+int nrx_set_tuning(nrx_engine *eng, int32_t group_width, int32_t tile_sites, int32_t row_lanes) {
   return guarded([&]() -> int {
     if (!eng) throw std::invalid_argument("null argument");
     Engine &e = *static_cast<Engine *>(eng);
     if (e.opt.precision != NRX_FP32)
       return fail(NRX_ERR_UNSUPPORTED, "kernel shapes are tuned for the FP32 form only");
     const auto cands = shape_candidates(e);
-    if (std::find(cands.begin(), cands.end(), std::make_pair(group_width, tile_sites)) == cands.end())
+    const Shape s{group_width, tile_sites, row_lanes};
+    if (std::find(cands.begin(), cands.end(), s) == cands.end())
       throw std::invalid_argument("unsupported kernel shape for this engine");
-    apply_shape(e, group_width, tile_sites);
+    apply_shape(e, s);
     return NRX_OK;
   });
 }
@@ -1533,6 +1546,7 @@ int nrx_get_tuning(nrx_engine *eng, nrx_tuning *out) {
     const Engine &e = *static_cast<Engine *>(eng);
     out->group_width = e.tune_group;
     out->tile_sites = e.S == AA_STATES ? AA_TILE : e.tune_tile;
+    out->row_lanes = e.tune_lanes;
     out->ms = 0.0;
     return NRX_OK;
   });
@@ -1552,7 +1566,7 @@ int nrx_autotune(nrx_engine *eng, int32_t variant, int32_t reps, nrx_tuning *bes
     int rc = NRX_OK;
     try {
       for (size_t i = 0; i < cands.size() && rc == NRX_OK; ++i) {
-        apply_shape(e, cands[i].first, cands[i].second);
+        apply_shape(e, cands[i]);
         rc = evaluate(e, variant, false, true, true);  // warm-up (and graph capture)
         for (int r = 0; r < reps && rc == NRX_OK; ++r) {
           rc = evaluate(e, variant, false, true, true);
@@ -1566,12 +1580,10 @@ int nrx_autotune(nrx_engine *eng, int32_t variant, int32_t reps, nrx_tuning *bes
     e.opt.timing = saved_timing;
     if (rc != NRX_OK) return rc;
     const size_t bi = static_cast<size_t>(std::min_element(ms.begin(), ms.end()) - ms.begin());
-    apply_shape(e, cands[bi].first, cands[bi].second);
-    best->group_width = cands[bi].first;
-    best->tile_sites = cands[bi].second;
-    best->ms = ms[bi];
+    apply_shape(e, cands[bi]);
+    *best = nrx_tuning{cands[bi].g, cands[bi].ts, cands[bi].h, ms[bi]};
     for (size_t i = 0; results && i < cands.size() && static_cast<int32_t>(i) < max_results; ++i)
-      results[i] = nrx_tuning{cands[i].first, cands[i].second, ms[i]};
+      results[i] = nrx_tuning{cands[i].g, cands[i].ts, cands[i].h, ms[i]};
     if (n_results) *n_results = static_cast<int32_t>(cands.size());
     return NRX_OK;
   });

@@ -89,9 +89,10 @@ constexpr int32_t OP_FIRST_KID1 = 4;
 // previous group's stores (and, walking backwards, load the previous
 // group's inputs before this group delivers)
 constexpr int32_t OP_GROUP_INDEP = 8;
-// FP32 form: child k's adjoint already holds deliveries made at least two
-// groups earlier in the reverse sweep and none since, so the accumulated
-// value may be loaded with the group's other inputs
+// FP32 form: child k's adjoint already holds deliveries made in earlier
+// groups of the reverse sweep and none in this group, so the accumulated
+// value may be loaded with the group's other inputs (the kernels issue a
+// group's loads after the previous group's stores)
 constexpr int32_t OP_ACC_EARLY0 = 16;
 constexpr int32_t OP_ACC_EARLY1 = 32;
 // FP32 form, 4 states: a child whose adjoint lives in a per-thread LDS slot
@@ -204,11 +205,12 @@ void launch_gather_results(hipStream_t s, const SmallDesc *d, int n, const int64
 // G = ops per group, TS = sites per workgroup (the autotuned shape)
 void launch_forward_persistent_multi(hipStream_t stream, int C, int G, int TS, const NetDesc *nets,
                                      const WorkItem *work, int n_work, size_t lds);
-void launch_adjoint_persistent_multi(hipStream_t stream, int C, int G, int TS, const NetDesc *nets,
+// H = lanes per (site, category) row of the 4-state reverse sweep (1 or 2)
+void launch_adjoint_persistent_multi(hipStream_t stream, int C, int G, int TS, int H, const NetDesc *nets,
                                      const WorkItem *work, int n_work, size_t lds);
 // dynamic LDS of the FP32 4-state adjoint (P/Q tables and pass slots)
 size_t adjoint_persistent_lds(int E, int C, int TS, int n_pass);
-int adjoint_pass_slots(int C, int G, int TS, int E, int64_t n_work);
+int adjoint_pass_slots(int C, int G, int TS, int H, int E, int64_t n_work);
 
 // Kernel launchers (kernels.hip)
 void launch_pmatrices(Engine &e);

@@ -85,6 +85,7 @@ struct Engine : nrx_engine {
   // persistent kernels and sites per workgroup (4-state data; 32 for 20)
   int tune_group = 4;
   int tune_tile = TILE;
+  int tune_lanes = 1;  // lanes per (site, category) row of the 4-state reverse sweep
   int schedule_version = 0;  // bumped by every schedule rebuild
   int pass_slots = 0;        // LDS pass slots the FP32 4-state reverse sweep uses
 

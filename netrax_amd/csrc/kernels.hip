@@ -622,7 +622,7 @@ __device__ __forceinline__ void adjoint_inputs(const DevOp &op, int64_t site, in
 }
 
 template <int S, int C>
-__global__ void __launch_bounds__(TILE *C)
+__global__ void __launch_bounds__(TILE *C, 3)
     k_adjoint_level(const DevOp *__restrict__ ops, int n_ops, int n_staged, DevLayout L,
                     const double *__restrict__ clv, const int32_t *__restrict__ scl,
                     double *__restrict__ adj, const typename TipType<S>::type *__restrict__ tips,
@@ -647,8 +647,15 @@ __global__ void __launch_bounds__(TILE *C)
   const size_t row = static_cast<size_t>(C * S);
   const int wave = threadIdx.x / 64;
   const double *Qc = (S == 4 ? sQ : Qd + static_cast<size_t>(part) * C * S * S) + c * (S * S);
-  // 20-state operands are too large to hold two ops' worth in registers
-  constexpr bool PREFETCH = S == 4;
+  // 4 states: r_c Q of this lane's category in registers (every op reads it;
+  // from LDS it would cost as many LDS bytes as a P matrix per child)
+  double Qr[S == 4 ? 16 : 1];
+  if constexpr (S == 4)
+#pragma unroll
+    for (int i = 0; i < 16; ++i) Qr[i] = Qc[i];
+  // no prefetch of the next op's inputs: it bought nothing measurable and
+  // its registers hold Q instead
+  constexpr bool PREFETCH = false;
   AdjIn<S> cur, nxt;
   if constexpr (PREFETCH) {
     adjoint_inputs<S, C>(ops[0], site, n, c, clv, scl, adj, tips, seed, cur);
@@ -682,25 +689,70 @@ __global__ void __launch_bounds__(TILE *C)
       for (int i = 0; i < S; ++i) a[i] = fma(f, freqs[part * S + i], a[i]);
     }
     double y[2][S];
-#pragma unroll
-    for (int k = 0; k < 2; ++k)
-      if (k < op.nk) matvec<S>(op_p<S, C>(sP, n_staged, P, part, L, j, k, op.edge[k], c), cur.x[k], y[k]);
     const double F = (cur.kq - ks) > 0 ? SCALE_FACTOR : 1.0;
     // arithmetic of both children first, memory traffic after
     double back[2][S], g[2];
+    if constexpr (S == 4) {
+      // y_k = P_k x_k, z_0 = a F y_1 (or a F), z_1 = a F y_0, back_k = P_k^T
+      // z_k: P_0 is read once (row i gives y_0[i], z_0[i] and row i's share
+      // of back_0), P_1 twice -- three matrix reads per op instead of four
+      const double *P0 = op_p<S, C>(sP, n_staged, P, part, L, j, 0, op.edge[0], c);
+      // (selects, not op.edge[op.nk - 1]: a runtime index would put op on the stack)
+      const double *P1 = op.nk == 2 ? op_p<S, C>(sP, n_staged, P, part, L, j, 1, op.edge[1], c) : P0;
+      double z[2][S];
+      if (op.nk == 2) matvec<S>(P1, cur.x[1], y[1]);
 #pragma unroll
-    for (int k = 0; k < 2; ++k) {
-      if (k < op.nk) {
-        double z[S], dx[S];
+      for (int i = 0; i < S; ++i) {
+        const double2 r01 = reinterpret_cast<const double2 *>(P0 + i * S)[0];
+        const double2 r23 = reinterpret_cast<const double2 *>(P0 + i * S)[1];
+        y[0][i] = fma(r23.y, cur.x[0][3], fma(r23.x, cur.x[0][2], fma(r01.y, cur.x[0][1], r01.x * cur.x[0][0])));
+        z[0][i] = a[i] * F * (op.nk == 2 ? y[1][i] : 1.0);
+        if (i == 0) {
+          back[0][0] = r01.x * z[0][0];
+          back[0][1] = r01.y * z[0][0];
+          back[0][2] = r23.x * z[0][0];
+          back[0][3] = r23.y * z[0][0];
+        } else {
+          back[0][0] = fma(r01.x, z[0][i], back[0][0]);
+          back[0][1] = fma(r01.y, z[0][i], back[0][1]);
+          back[0][2] = fma(r23.x, z[0][i], back[0][2]);
+          back[0][3] = fma(r23.y, z[0][i], back[0][3]);
+        }
+      }
+      if (op.nk == 2) {
 #pragma unroll
-        for (int i = 0; i < S; ++i) z[i] = a[i] * F * (op.nk == 2 ? y[1 - k][i] : 1.0);
-        matvec<S>(Qc, y[k], dx);
-        double gk = 0.0;
+        for (int i = 0; i < S; ++i) z[1][i] = a[i] * F * y[0][i];
+        if (op.kid[1] >= 0) matvec_t<S>(P1, z[1], back[1]);
+      }
 #pragma unroll
-        for (int i = 0; i < S; ++i) gk = fma(z[i], dx[i], gk);
-        g[k] = wave_sum(gk);
-        if (op.kid[k] >= 0)
-          matvec_t<S>(op_p<S, C>(sP, n_staged, P, part, L, j, k, op.edge[k], c), z, back[k]);
+      for (int k = 0; k < 2; ++k) {
+        if (k < op.nk) {
+          double dx[S];
+          matvec<S>(Qr, y[k], dx);
+          double gk = 0.0;
+#pragma unroll
+          for (int i = 0; i < S; ++i) gk = fma(z[k][i], dx[i], gk);
+          g[k] = wave_sum(gk);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < 2; ++k)
+        if (k < op.nk) matvec<S>(op_p<S, C>(sP, n_staged, P, part, L, j, k, op.edge[k], c), cur.x[k], y[k]);
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        if (k < op.nk) {
+          double z[S], dx[S];
+#pragma unroll
+          for (int i = 0; i < S; ++i) z[i] = a[i] * F * (op.nk == 2 ? y[1 - k][i] : 1.0);
+          matvec<S>(Qc, y[k], dx);
+          double gk = 0.0;
+#pragma unroll
+          for (int i = 0; i < S; ++i) gk = fma(z[i], dx[i], gk);
+          g[k] = wave_sum(gk);
+          if (op.kid[k] >= 0)
+            matvec_t<S>(op_p<S, C>(sP, n_staged, P, part, L, j, k, op.edge[k], c), z, back[k]);
+        }
       }
     }
     // op j + 1's loads have had op j's arithmetic to arrive; waiting here,
@@ -1092,7 +1144,10 @@ __device__ __forceinline__ void adjoint_persistent_body(
   for (int i = threadIdx.x; i < C * 16; i += blockDim.x) sP[np + i] = Qf[part * C * 16 + i];
   __syncthreads();
   const int c = threadIdx.x % C;
-  const float *sQ = sP + np + c * 16;
+  // r_c Q of this lane's category, kept in registers (every op reads it)
+  float Qr[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) Qr[i] = sP[np + c * 16 + i];
   // pass slots: mantissas [slot][thread][4], site exponents [slot][site]
   float *pass_m = sP + ((np + C * 16 + 3) & ~3);
   int *pass_e = reinterpret_cast<int *>(pass_m + n_pass * NT * 4);
@@ -1202,39 +1257,64 @@ __device__ __forceinline__ void adjoint_persistent_body(
           }
           ea = align_add(a, ea, sd, ef);
         }
-        float y[2][4];
+        // y_k = P_k x_k, z_0 = a y_1 (or a), z_1 = a y_0 and back_k = P_k^T z_k
+        // read P_0 once (row i gives y_0[i], then z_0[i] and row i's share of
+        // back_0) and P_1 twice: three matrix reads from LDS per op, not four
+        float y[2][4], z[2][4], back[2][4];
+        const float *P0 = sP + (op.edge[0] * C + c) * 16;
+        // (a select, not op.edge[op.nk - 1]: a runtime index would put op on the stack)
+        const float *P1 = sP + ((op.nk == 2 ? op.edge[1] : op.edge[0]) * C + c) * 16;
+        if (op.nk == 2) mv4(P1, x[j][1], y[1]);
 #pragma unroll
-        for (int kk = 0; kk < 2; ++kk)
-          if (kk < op.nk) mv4(sP + (op.edge[kk] * C + c) * 16, x[j][kk], y[kk]);
+        for (int i = 0; i < 4; ++i) {
+          const float4 r = reinterpret_cast<const float4 *>(P0)[i];
+          y[0][i] = fmaf(r.w, x[j][0][3], fmaf(r.z, x[j][0][2], fmaf(r.y, x[j][0][1], r.x * x[j][0][0])));
+          z[0][i] = op.nk == 2 ? a[i] * y[1][i] : a[i];
+          if (i == 0) {
+            back[0][0] = r.x * z[0][0];
+            back[0][1] = r.y * z[0][0];
+            back[0][2] = r.z * z[0][0];
+            back[0][3] = r.w * z[0][0];
+          } else {
+            back[0][0] = fmaf(r.x, z[0][i], back[0][0]);
+            back[0][1] = fmaf(r.y, z[0][i], back[0][1]);
+            back[0][2] = fmaf(r.z, z[0][i], back[0][2]);
+            back[0][3] = fmaf(r.w, z[0][i], back[0][3]);
+          }
+        }
+        if (op.nk == 2) {
+#pragma unroll
+          for (int i = 0; i < 4; ++i) z[1][i] = a[i] * y[0][i];
+          mv4t(P1, z[1], back[1]);
+        }
         const int ez = ea + kq[j] - ks[j];
 #pragma unroll
         for (int kk = 0; kk < 2; ++kk) {
           if (kk < op.nk) {
-            float z[4], back[4], qx[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) z[i] = op.nk == 2 ? a[i] * y[1 - kk][i] : a[i];
-            const float *Pe = sP + (op.edge[kk] * C + c) * 16;
-            mv4t(Pe, z, back);
-            mv4(sQ, y[kk], qx);
-            const float gd = fmaf(z[3], qx[3], fmaf(z[2], qx[2], fmaf(z[1], qx[1], z[0] * qx[0])));
+            float qx[4];
+            mv4(Qr, y[kk], qx);
+            const float gd =
+                fmaf(z[kk][3], qx[3], fmaf(z[kk][2], qx[2], fmaf(z[kk][1], qx[1], z[kk][0] * qx[0])));
             double gv = wave_sum(ldexp(static_cast<double>(gd), ez));
             if ((threadIdx.x & 63) == 0) grad_partials[grad_index(tile, n_uses, op.gslot[kk], PWB, wave)] = gv;
             const int32_t kid = op.kid[kk];
             if (kid >= 0 && (op.flags & (kk == 0 ? OP_PASS_KID0 : OP_PASS_KID1))) {
               const int slot = pass_slot(op.flags, kk);
               pass_delivery<C>(pass_m + 4 * (slot * NT + threadIdx.x), pass_e + slot * TS + site_in_tile,
-                               (op.flags & (kk == 0 ? OP_FIRST_KID0 : OP_FIRST_KID1)) != 0, back, ez);
+                               (op.flags & (kk == 0 ? OP_FIRST_KID0 : OP_FIRST_KID1)) != 0, back[kk], ez);
             } else if (kid >= 0) {
               GAS float *dst_m = adj + (static_cast<size_t>(kid) * n + site) * row + c * 4;
               GAS int32_t *dst_e = adj_exp + static_cast<size_t>(kid) * n + site;
               const bool first = (op.flags & (kk == 0 ? OP_FIRST_KID0 : OP_FIRST_KID1)) != 0;
               if (first || (op.flags & (kk == 0 ? OP_ACC_EARLY0 : OP_ACC_EARLY1))) {
-                finish_delivery<C>(dst_m, dst_e, first, lane0, back, ez, acc[j][kk], eacc[j][kk]);
+                finish_delivery<C>(dst_m, dst_e, first, lane0, back[kk], ez, acc[j][kk], eacc[j][kk]);
               } else {
-                // an earlier op of this group or the previous group wrote it: read it now
+                // the schedule keeps two deliveries to one child out of a
+                // group, so this read-back of the adjoint written by an
+                // earlier op of this group is never taken; kept for any schedule
                 float acc_now[4];
                 load4(dst_m, acc_now);
-                finish_delivery<C>(dst_m, dst_e, false, lane0, back, ez, acc_now, *dst_e);
+                finish_delivery<C>(dst_m, dst_e, false, lane0, back[kk], ez, acc_now, *dst_e);
               }
             }
           }
@@ -1245,7 +1325,7 @@ __device__ __forceinline__ void adjoint_persistent_body(
 }
 
 template <int C, int G, int TS>
-__global__ void __launch_bounds__(TS *C)
+__global__ void __launch_bounds__(TS *C, 3)
     k_adjoint_persistent(const NetDesc *__restrict__ nets, const WorkItem *__restrict__ work) {
   const WorkItem wi = work[blockIdx.x];
   const NetDesc &nd = nets[wi.net];
@@ -1255,6 +1335,307 @@ __global__ void __launch_bounds__(TS *C)
                                     to_global(nd.Qf), to_global(nd.tile_part), to_global(nd.freqs),
                                     to_global(nd.weights), to_global(nd.beta), nd.n_root, nd.n_uses,
                                     nd.n_pass, to_global(nd.grad_partials), wi.tile);
+}
+
+// ------------------------------------------- reverse sweep, two lanes per row
+// The same sweep (ops, groups, flags, LDS pass slots, arithmetic) with each
+// (site, category) row of 4 states held by two adjacent lanes: lane h = 0
+// holds states 0, 1 and lane h = 1 states 2, 3 (threadIdx.x = (site C +
+// category) 2 + h).  A tile then has twice the waves, each carrying half the
+// registers per op, so more groups' loads are in flight per SIMD.  Products
+// with a 4 x 4 matrix need the partner's two values (one DPP move each):
+//   y = P x      my rows of P times (mine, partner's) x
+//   back = P^T z partial sums over my two rows for all four columns; the
+//                partner's partials for my columns arrive by DPP
+//   Q y          as P x, on the exchanged y
+// The P and Q tables in LDS are stored per lane half: block h of a (edge,
+// category) holds rows 2h, 2h+1 with the columns reordered (2h, 2h+1, then
+// the partner's two), so every lane reads its operands in one fixed order.
+__device__ __forceinline__ float pair_swap(float v) { return dpp<DPP_XOR1>(v); }
+
+// maximum / sum over the 2C lanes of a site (halves at lane bit 0,
+// categories at bits 1 ..)
+template <int C>
+__device__ __forceinline__ float site_max2(float m) {
+  m = fmaxf(m, pair_swap(m));
+  if constexpr (C > 1) m = fmaxf(m, dpp<DPP_XOR2>(m));
+  if constexpr (C > 2) m = fmaxf(m, __shfl_xor(m, 4));
+  if constexpr (C > 4) m = fmaxf(m, __shfl_xor(m, 8));
+  return m;
+}
+template <int C>
+__device__ __forceinline__ double site_sum2(double m) {
+  m += dpp<DPP_XOR1>(m);
+  if constexpr (C > 1) m += dpp<DPP_XOR2>(m);
+  if constexpr (C > 2) m += __shfl_xor(m, 4);
+  if constexpr (C > 4) m += __shfl_xor(m, 8);
+  return m;
+}
+
+// source index in a row-major 4 x 4 matrix of entry r (0..15) of lane half
+// h's block: rows 2h + r / 4 % 2, columns (2h, 2h+1, partner's two)
+__device__ __forceinline__ int half_block_src(int r) {
+  const int h = r >> 3, k = (r >> 2) & 1, q = r & 3;
+  const int row = 2 * h + k;
+  const int col = q < 2 ? 2 * h + q : 2 * (1 - h) + (q - 2);
+  return row * 4 + col;
+}
+
+// y (my two rows) = M (x, partner's x); M = the lane's 8-entry block
+__device__ __forceinline__ void mv2(const float *__restrict__ M, float x0, float x1, float o0, float o1,
+                                    float &y0, float &y1) {
+  y0 = fmaf(M[3], o1, fmaf(M[2], o0, fmaf(M[1], x1, M[0] * x0)));
+  y1 = fmaf(M[7], o1, fmaf(M[6], o0, fmaf(M[5], x1, M[4] * x0)));
+}
+
+// (P^T z) for my two states from my block and my two z values
+__device__ __forceinline__ void mv2t(const float *__restrict__ M, float z0, float z1, float &b0, float &b1) {
+  const float p0 = fmaf(M[4], z1, M[0] * z0);  // columns 2h, 2h+1: my states
+  const float p1 = fmaf(M[5], z1, M[1] * z0);
+  const float q0 = fmaf(M[6], z1, M[2] * z0);  // the partner's states
+  const float q1 = fmaf(M[7], z1, M[3] * z0);
+  b0 = p0 + pair_swap(q0);
+  b1 = p1 + pair_swap(q1);
+}
+
+// x 2^ex + y 2^ey (two values), aligned to the larger exponent
+__device__ __forceinline__ int align_add2(float &x0, float &x1, int ex, float y0, float y1, int ey) {
+  const int e = max(ex, ey);
+  x0 = ldexpf(x0, ex - e) + ldexpf(y0, ey - e);
+  x1 = ldexpf(x1, ex - e) + ldexpf(y1, ey - e);
+  return e;
+}
+
+__device__ __forceinline__ int renorm2(float &x0, float &x1, int ex, float m) {
+  int e = 0;
+  (void)frexpf(m, &e);
+  x0 = ldexpf(x0, -e);
+  x1 = ldexpf(x1, -e);
+  return m > 0.f ? ex + e : EXP_EMPTY;
+}
+
+__device__ __forceinline__ void load2(const GAS float *__restrict__ p, float &a, float &b) {
+  const float2 v = *reinterpret_cast<const GAS float2 *>(p);
+  a = v.x;
+  b = v.y;
+}
+__device__ __forceinline__ void store2(GAS float *__restrict__ p, float a, float b) {
+  *reinterpret_cast<GAS float2 *>(p) = make_float2(a, b);
+}
+
+// delivered (b, eb) plus the accumulated (acc, ea) unless first, then
+// renormalised (as finish_delivery)
+template <int C>
+__device__ __forceinline__ void finish_delivery2(float &b0, float &b1, int &eb, bool first, float a0, float a1,
+                                                 int ea) {
+  float mb = site_max2<C>(fmaxf(fabsf(b0), fabsf(b1)));
+  if (!first) {
+    eb = align_add2(b0, b1, mb > 0.f ? eb : EXP_EMPTY, a0, a1, ea);
+    mb = site_max2<C>(fmaxf(fabsf(b0), fabsf(b1)));
+  }
+  eb = renorm2(b0, b1, eb, mb);
+}
+
+template <int C, int G, int TS>
+__device__ __forceinline__ void adjoint_split_body(
+    const CAS DevOp *__restrict__ ops, const CAS int32_t *__restrict__ group_begin, int n_groups,
+    const DevLayout L, const GAS float *__restrict__ clv, const GAS int32_t *__restrict__ scl,
+    GAS float *__restrict__ adj, GAS int32_t *__restrict__ adj_exp,
+    const GAS uint8_t *__restrict__ tips, const GAS float *__restrict__ Pf,
+    const GAS float *__restrict__ Qf, const GAS int32_t *__restrict__ tile_part,
+    const GAS double *__restrict__ freqs, const GAS double *__restrict__ weights,
+    const GAS double *__restrict__ beta, int n_root, int n_uses, int n_pass,
+    GAS double *__restrict__ grad_partials, int tile) {
+  float *sP = dyn_lds<float>();  // [edge][cat][half][8] then Q [cat][half][8], then pass slots
+  constexpr int NT = TS * C * 2;
+  constexpr int PWB = NT / 64;  // waves of this site block
+  const int64_t site0 = static_cast<int64_t>(tile) * TS;
+  const int part = tile_part[site0 / TILE];
+  const int np = L.n_edges * C * 16;
+  const GAS float *gP = Pf + static_cast<size_t>(part) * np;
+  for (int i = threadIdx.x; i < np; i += NT) sP[i] = gP[(i & ~15) + half_block_src(i & 15)];
+  for (int i = threadIdx.x; i < C * 16; i += NT) sP[np + i] = Qf[part * C * 16 + (i & ~15) + half_block_src(i & 15)];
+  __syncthreads();
+  const int h = threadIdx.x & 1;
+  const int c = (threadIdx.x >> 1) % C;
+  float Qr[8];  // this lane's half block of r_c Q, kept in registers
+#pragma unroll
+  for (int i = 0; i < 8; ++i) Qr[i] = sP[np + c * 16 + h * 8 + i];
+  // pass slots: mantissas [slot][thread][2], site exponents [slot][site]
+  float *pass_m = sP + ((np + C * 16 + 3) & ~3);
+  int *pass_e = reinterpret_cast<int *>(pass_m + n_pass * NT * 2);
+  const int site_in_tile = threadIdx.x / (2 * C);
+  const int64_t site = site0 + site_in_tile;
+  const int64_t n = L.n_sites_pad;
+  const int wave = threadIdx.x / 64;
+  const bool lead = (threadIdx.x & (2 * C - 1)) == 0;  // writes the site's exponent
+  const double w = weights[site];
+  const double pi0 = freqs[part * 4 + 2 * h], pi1 = freqs[part * 4 + 2 * h + 1];
+  const size_t row = C * 4;
+  const int off = c * 4 + 2 * h;  // my two states within a site's row
+  for (int gi = n_groups - 1; gi >= 0; --gi) {
+    const int g0 = group_begin[gi];
+    const int nb = group_begin[gi + 1] - g0;
+    float aq[G][2], x[G][2][2], xq[G][2], acc[G][2][2];
+    int gq[G], kq[G], sk[G][2], eacc[G][2];
+    uint32_t tm[G][2];
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+      if (j < nb) {
+        const DevOp op = ops[g0 + j];
+        kq[j] = scl[static_cast<size_t>(op.out) * n + site];
+        if (op.flags & OP_HAS_CONSUMERS) {
+          if (op.flags & OP_A_PASS) {
+            const int slot = pass_slot(op.flags, 2);
+            const float2 v = reinterpret_cast<const float2 *>(pass_m)[slot * NT + threadIdx.x];
+            aq[j][0] = v.x;
+            aq[j][1] = v.y;
+            gq[j] = pass_e[slot * TS + site_in_tile];
+          } else {
+            load2(adj + (static_cast<size_t>(op.out) * n + site) * row + off, aq[j][0], aq[j][1]);
+            gq[j] = adj_exp[static_cast<size_t>(op.out) * n + site];
+          }
+        }
+        if (op.root >= 0) load2(clv + (static_cast<size_t>(op.out) * n + site) * row + off, xq[j][0], xq[j][1]);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+#pragma unroll
+      for (int kk = 0; kk < 2; ++kk) {
+        if (j < nb) {
+          const DevOp op = ops[g0 + j];
+          const int32_t kid = op.kid[kk];
+          if (kk < op.nk && kid >= 0) {
+            load2(clv + (static_cast<size_t>(kid) * n + site) * row + off, x[j][kk][0], x[j][kk][1]);
+            sk[j][kk] = scl[static_cast<size_t>(kid) * n + site];
+            if (op.flags & (kk == 0 ? OP_ACC_EARLY0 : OP_ACC_EARLY1)) {
+              load2(adj + (static_cast<size_t>(kid) * n + site) * row + off, acc[j][kk][0], acc[j][kk][1]);
+              eacc[j][kk] = adj_exp[static_cast<size_t>(kid) * n + site];
+            }
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+#pragma unroll
+      for (int kk = 0; kk < 2; ++kk) {
+        if (j < nb) {
+          const DevOp op = ops[g0 + j];
+          const int32_t kid = op.kid[kk];
+          if (kk < op.nk && kid < 0) tm[j][kk] = tips[static_cast<size_t>(-kid - 1) * n + site];
+        }
+      }
+    }
+    wait_loads();
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+      if (j < nb) {
+        const DevOp op = ops[g0 + j];
+        int ks = 0;
+#pragma unroll
+        for (int kk = 0; kk < 2; ++kk) {
+          if (kk < op.nk) {
+            if (op.kid[kk] < 0) {
+              const uint32_t m = tm[j][kk] >> (2 * h);
+              x[j][kk][0] = (m & 1u) ? 1.f : 0.f;
+              x[j][kk][1] = (m & 2u) ? 1.f : 0.f;
+            } else {
+              ks += sk[j][kk];
+            }
+          }
+        }
+        const bool has_adj = (op.flags & OP_HAS_CONSUMERS) != 0;
+        float a0 = has_adj ? aq[j][0] : 0.f, a1 = has_adj ? aq[j][1] : 0.f;
+        int ea = has_adj ? gq[j] : EXP_EMPTY;
+        if (op.root >= 0) {
+          double t = site_sum2<C>(fma(pi0, static_cast<double>(xq[j][0]), pi1 * static_cast<double>(xq[j][1])));
+          t *= 1.0 / C;
+          const double b = beta[part * n_root + op.root] * w;
+          int ef = EXP_EMPTY;
+          float sd0 = 0.f, sd1 = 0.f;
+          if (b != 0.0) {
+            const double mf = frexp(b / (C * t), &ef);
+            sd0 = static_cast<float>(mf * pi0);
+            sd1 = static_cast<float>(mf * pi1);
+          }
+          ea = align_add2(a0, a1, ea, sd0, sd1, ef);
+        }
+        float y[2][2];
+#pragma unroll
+        for (int kk = 0; kk < 2; ++kk)
+          if (kk < op.nk)
+            mv2(sP + (op.edge[kk] * C + c) * 16 + h * 8, x[j][kk][0], x[j][kk][1], pair_swap(x[j][kk][0]),
+                pair_swap(x[j][kk][1]), y[kk][0], y[kk][1]);
+        const int ez = ea + kq[j] - ks;
+#pragma unroll
+        for (int kk = 0; kk < 2; ++kk) {
+          if (kk < op.nk) {
+            const float z0 = op.nk == 2 ? a0 * y[1 - kk][0] : a0;
+            const float z1 = op.nk == 2 ? a1 * y[1 - kk][1] : a1;
+            float q0, q1, b0, b1;
+            mv2(Qr, y[kk][0], y[kk][1], pair_swap(y[kk][0]), pair_swap(y[kk][1]), q0, q1);
+            mv2t(sP + (op.edge[kk] * C + c) * 16 + h * 8, z0, z1, b0, b1);
+            const float gd = fmaf(z1, q1, z0 * q0);
+            const double gv = wave_sum(ldexp(static_cast<double>(gd), ez));
+            if ((threadIdx.x & 63) == 0) grad_partials[grad_index(tile, n_uses, op.gslot[kk], PWB, wave)] = gv;
+            const int32_t kid = op.kid[kk];
+            if (kid < 0) continue;
+            const bool first = (op.flags & (kk == 0 ? OP_FIRST_KID0 : OP_FIRST_KID1)) != 0;
+            int eb = ez;
+            if (op.flags & (kk == 0 ? OP_PASS_KID0 : OP_PASS_KID1)) {
+              const int slot = pass_slot(op.flags, kk);
+              float2 *sm = reinterpret_cast<float2 *>(pass_m) + slot * NT + threadIdx.x;
+              int *se = pass_e + slot * TS + site_in_tile;
+              float p0 = 0.f, p1 = 0.f;
+              int pe = EXP_EMPTY;
+              if (!first) {
+                const float2 v = *sm;
+                p0 = v.x;
+                p1 = v.y;
+                pe = *se;
+              }
+              finish_delivery2<C>(b0, b1, eb, first, p0, p1, pe);
+              *sm = make_float2(b0, b1);
+              *se = eb;  // every lane of the site writes the same value
+            } else {
+              GAS float *dst_m = adj + (static_cast<size_t>(kid) * n + site) * row + off;
+              GAS int32_t *dst_e = adj_exp + static_cast<size_t>(kid) * n + site;
+              float p0 = 0.f, p1 = 0.f;
+              int pe = EXP_EMPTY;
+              if (!first) {
+                if (op.flags & (kk == 0 ? OP_ACC_EARLY0 : OP_ACC_EARLY1)) {
+                  p0 = acc[j][kk][0];
+                  p1 = acc[j][kk][1];
+                  pe = eacc[j][kk];
+                } else {
+                  // an earlier op of this group or the previous group wrote it: read it now
+                  load2(dst_m, p0, p1);
+                  pe = *dst_e;
+                }
+              }
+              finish_delivery2<C>(b0, b1, eb, first, p0, p1, pe);
+              store2(dst_m, b0, b1);
+              if (lead) *dst_e = eb;
+            }
+          }
+        }
+      }
+    }
+  }
+}
+
+template <int C, int G, int TS>
+__global__ void __launch_bounds__(TS *C * 2)
+    k_adjoint_split(const NetDesc *__restrict__ nets, const WorkItem *__restrict__ work) {
+  const WorkItem wi = work[blockIdx.x];
+  const NetDesc &nd = nets[wi.net];
+  adjoint_split_body<C, G, TS>(to_constant(nd.ops), to_constant(nd.group_begin), nd.n_groups, nd.L,
+                               to_global(nd.clv), to_global(nd.scl), to_global(nd.adj), to_global(nd.adj_exp),
+                               to_global(nd.tips), to_global(nd.Pf), to_global(nd.Qf), to_global(nd.tile_part),
+                               to_global(nd.freqs), to_global(nd.weights), to_global(nd.beta), nd.n_root,
+                               nd.n_uses, nd.n_pass, to_global(nd.grad_partials), wi.tile);
 }
 
 // ============================================= FP32 form, 20 states (MFMA)
@@ -2086,7 +2467,7 @@ void launch_forward_persistent(Engine &e) {
 }
 
 void launch_adjoint_persistent(Engine &e) {
-  launch_adjoint_persistent_multi(e.stream, e.C, e.tune_group, e.tune_tile,
+  launch_adjoint_persistent_multi(e.stream, e.C, e.tune_group, e.tune_tile, e.tune_lanes,
                                   e.d_netdesc.as<NetDesc>(), e.d_work.as<WorkItem>(),
                                   static_cast<int>(e.L.n_sites_pad / e.tune_tile),
                                   adjoint_persistent_lds(e.E, e.C, e.tune_tile, e.pass_slots));
@@ -2103,18 +2484,19 @@ size_t adjoint_persistent_lds(int E, int C, int TS, int n_pass) {
 // device: the most slots (at most PASS_SLOTS) whose LDS still lets as many
 // workgroups per CU be resident as the launch can use (its registers and
 // tables allow, and its size fills); 0 when the tables alone do not fit.
-int adjoint_pass_slots(int C, int G, int TS, int E, int64_t n_work) {
+int adjoint_pass_slots(int C, int G, int TS, int H, int E, int64_t n_work) {
   int dev = 0, n_cu = 0;
   NRX_HIP(hipGetDevice(&dev));
   NRX_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
   int best = 0;
   dispatch_shape(C, G, TS, [&](auto c, auto g, auto ts) {
     constexpr int CC = decltype(c)::value, GG = decltype(g)::value, T = decltype(ts)::value;
+    const void *kern = H == 2 ? reinterpret_cast<const void *>(&k_adjoint_split<CC, GG, T>)
+                              : reinterpret_cast<const void *>(&k_adjoint_persistent<CC, GG, T>);
     auto resident = [&](int k) {
       int nb = 0;
-      NRX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-          &nb, reinterpret_cast<const void *>(&k_adjoint_persistent<CC, GG, T>), T * CC,
-          adjoint_persistent_lds(E, C, TS, k)));
+      NRX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, T * CC * H,
+                                                           adjoint_persistent_lds(E, C, TS, k)));
       return nb;
     };
     const int r0 = resident(0);
@@ -2139,12 +2521,17 @@ void launch_forward_persistent_multi(hipStream_t stream, int C, int G, int TS, c
   NRX_HIP(hipGetLastError());
 }
 
-void launch_adjoint_persistent_multi(hipStream_t stream, int C, int G, int TS, const NetDesc *nets,
+void launch_adjoint_persistent_multi(hipStream_t stream, int C, int G, int TS, int H, const NetDesc *nets,
                                      const WorkItem *work, int n_work, size_t lds) {
+  if (H != 1 && H != 2) throw std::invalid_argument("lanes per row must be 1 or 2");
   dispatch_shape(C, G, TS, [&](auto c, auto g, auto ts) {
     constexpr int CC = decltype(c)::value, GG = decltype(g)::value, T = decltype(ts)::value;
-    hipLaunchKernelGGL((k_adjoint_persistent<CC, GG, T>), dim3(n_work), dim3(T * CC), lds, stream,
-                       nets, work);
+    if (H == 2)
+      hipLaunchKernelGGL((k_adjoint_split<CC, GG, T>), dim3(n_work), dim3(T * CC * 2), lds, stream, nets,
+                         work);
+    else
+      hipLaunchKernelGGL((k_adjoint_persistent<CC, GG, T>), dim3(n_work), dim3(T * CC), lds, stream,
+                         nets, work);
   });
   NRX_HIP(hipGetLastError());
 }

@@ -29,9 +29,10 @@ void set_adjoint_flags(std::vector<DevOp> &ops, const std::vector<int> &adj_orde
 
 // OP_ACC_EARLY*: for the FP32 form's reverse sweep (groups backwards, ops in
 // index order), a delivery that accumulates into a child adjoint last
-// written at least two groups earlier (and not earlier in its own group), so
-// the read-back can be issued with the group's other loads even while the
-// group before it is still delivering.
+// written in an earlier group (not earlier in its own group), so the
+// read-back is issued with the group's other loads.  Those loads follow the
+// earlier group's stores in the wave's program order, and a wave's loads
+// see its own earlier stores to the same address.
 void set_acc_early(std::vector<DevOp> &ops, const std::vector<int32_t> &group_begin, int nv) {
   std::vector<int> last(nv, -1);  // sweep position of the group that last delivered to a slot
   const int ng = static_cast<int>(group_begin.size()) - 1;
@@ -46,7 +47,7 @@ void set_acc_early(std::vector<DevOp> &ops, const std::vector<int32_t> &group_be
         if (kid < 0) continue;
         const bool first = (op.flags & (k == 0 ? OP_FIRST_KID0 : OP_FIRST_KID1)) != 0;
         const bool in_group = std::find(mine.begin(), mine.end(), kid) != mine.end();
-        if (!first && !in_group && last[kid] >= 0 && t - last[kid] >= 2)
+        if (!first && !in_group && last[kid] >= 0 && t - last[kid] >= 1)
           op.flags |= k == 0 ? OP_ACC_EARLY0 : OP_ACC_EARLY1;
         mine.push_back(kid);
       }
@@ -154,9 +155,26 @@ void build_batched_order(const Dag &dag, const std::vector<int> &root_slots, con
     while (!ready.empty() || !fresh.empty()) {
       const bool indep = !ready.empty();
       std::vector<int> &src = indep ? ready : fresh;
-      const int take = std::min<int>(group, static_cast<int>(src.size()));
-      std::vector<int> grp(src.begin(), src.begin() + take);
-      src.erase(src.begin(), src.begin() + take);
+      // up to `group` ops in post-order, skipping an op that shares a child
+      // CLV with one already taken: two deliveries to one child inside a
+      // group would make the second read back the first's store at once
+      std::vector<int> grp;
+      std::vector<int32_t> kids_taken;
+      for (size_t i = 0; i < src.size() && static_cast<int>(grp.size()) < group;) {
+        const Variant &v = dag.var[src[i]];
+        bool clash = false;
+        for (int k = 0; k < v.nkids; ++k)
+          if (v.kid[k] >= 0 && std::find(kids_taken.begin(), kids_taken.end(), v.kid[k]) != kids_taken.end())
+            clash = true;
+        if (clash && !grp.empty()) {
+          ++i;
+          continue;
+        }
+        for (int k = 0; k < v.nkids; ++k)
+          if (v.kid[k] >= 0) kids_taken.push_back(v.kid[k]);
+        grp.push_back(src[i]);
+        src.erase(src.begin() + static_cast<std::ptrdiff_t>(i));
+      }
       // what the previous group made ready no longer depends on the group
       // after this one
       for (int s : fresh) ready.insert(std::upper_bound(ready.begin(), ready.end(), s, by_pos), s);

@@ -217,23 +217,25 @@ class AnnotatedNetwork:
 
     def autotune(self, variant: Optional[int] = None, reps: int = 3):
         """FP32 form: times every kernel shape (ops per group, sites per
-        workgroup), keeps the fastest; returns (best, [all candidates]) as
-        (group_width, tile_sites, ms) tuples."""
+        workgroup, lanes per row in the reverse sweep), keeps the fastest;
+        returns (best, [all candidates]) as (group_width, tile_sites,
+        row_lanes, ms) tuples."""
         v = int(self.options.likelihood_variant if variant is None else variant)
         best = _lib.Tuning()
-        res = (_lib.Tuning * 16)()
+        res = (_lib.Tuning * 64)()
         n = C.c_int32()
-        _lib.check(self.lib.nrx_autotune(self.handle, v, int(reps), C.byref(best), res, 16, C.byref(n)))
-        return ((best.group_width, best.tile_sites, best.ms),
-                [(r.group_width, r.tile_sites, r.ms) for r in res[:n.value]])
+        _lib.check(self.lib.nrx_autotune(self.handle, v, int(reps), C.byref(best), res, 64, C.byref(n)))
+        return ((best.group_width, best.tile_sites, best.row_lanes, best.ms),
+                [(r.group_width, r.tile_sites, r.row_lanes, r.ms) for r in res[:n.value]])
 
-    def set_tuning(self, group_width: int, tile_sites: int) -> None:
-        _lib.check(self.lib.nrx_set_tuning(self.handle, int(group_width), int(tile_sites)))
+    def set_tuning(self, group_width: int, tile_sites: int, row_lanes: int = 1) -> None:
+        _lib.check(self.lib.nrx_set_tuning(self.handle, int(group_width), int(tile_sites), int(row_lanes)))
 
     def tuning(self):
+        """(group_width, tile_sites, row_lanes) of the FP32 form's kernels."""
         t = _lib.Tuning()
         _lib.check(self.lib.nrx_get_tuning(self.handle, C.byref(t)))
-        return t.group_width, t.tile_sites
+        return t.group_width, t.tile_sites, t.row_lanes
 
     def close(self) -> None:
         if getattr(self, "handle", None):

@@ -95,7 +95,7 @@ static int check(const Dag &dag, int group, int n_slots) {
         if (first != (delivered[kid] == 0)) fail("OP_FIRST_KID");
         const bool early = (op.flags & (k == 0 ? OP_ACC_EARLY0 : OP_ACC_EARLY1)) != 0;
         const bool in_group = std::find(mine.begin(), mine.end(), kid) != mine.end();
-        if (early && (first || in_group || last_group[kid] < 0 || t - last_group[kid] < 2)) fail("OP_ACC_EARLY");
+        if (early && (first || in_group || last_group[kid] < 0 || t - last_group[kid] < 1)) fail("OP_ACC_EARLY");
         if (op.flags & (k == 0 ? OP_PASS_KID0 : OP_PASS_KID1)) {
           const int q = pass_slot(op.flags, k);
           n_pass++;

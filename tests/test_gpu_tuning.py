@@ -17,14 +17,14 @@ def _check_all_shapes(w):
     best, cands = a32.autotune(reps=2)
     print("autotune", best, cands, flush=True)
     assert len(cands) >= 3
-    assert best[2] == min(c[2] for c in cands)
-    assert a32.tuning() == best[:2]
-    for gw, ts, ms in cands:
-        a32.set_tuning(gw, ts)
+    assert best[3] == min(c[3] for c in cands)
+    assert a32.tuning() == best[:3]
+    for gw, ts, h, ms in cands:
+        a32.set_tuning(gw, ts, h)
         l32, g32, p32 = a32.gradient()
-        assert abs(l32 - l64) <= 1e-6 * abs(l64), (gw, ts, l32, l64)
+        assert abs(l32 - l64) <= 1e-6 * abs(l64), (gw, ts, h, l32, l64)
         scale = max(1.0, np.abs(g64).max())
-        assert np.abs(g32 - g64).max() <= 1e-4 * scale, (gw, ts)
+        assert np.abs(g32 - g64).max() <= 1e-4 * scale, (gw, ts, h)
         assert np.abs(p32 - p64).max(initial=0.0) <= 1e-4 * max([1.0, *np.abs(p64)])
     with pytest.raises(Exception):
         a32.set_tuning(5, 64)
@@ -43,21 +43,22 @@ def test_protein_fp32_every_shape_matches_fp64():
 
 
 def test_batch_runs_with_a_tuned_shape():
-    print("batch with shape (2, 32)", flush=True)
+    print("batch with shapes (2, 32, 1) and (3, 32, 2)", flush=True)
     cands = candidate_batch(2, 2, seed=10, sites=(300, 2000))
     anns = [AnnotatedNetwork(n, p, NetraxOptions(precision=Precision.FP32)) for n, p in cands]
-    for a in anns:
-        a.set_tuning(2, 32)
-    batch = NetworkBatch(anns)
-    lnl, dbs, dps, _ = batch.evaluate(0)
-    for i, a in enumerate(anns):
-        l, g, p = a.gradient()
-        assert abs(l - lnl[i]) <= 1e-9 * abs(l)
-        assert np.allclose(g, dbs[i], rtol=1e-6, atol=1e-6 * max(1.0, np.abs(g).max()))
-    anns[0].set_tuning(4, 64)
+    for shape in ((2, 32, 1), (3, 32, 2)):
+        for a in anns:
+            a.set_tuning(*shape)
+        batch = NetworkBatch(anns)
+        lnl, dbs, dps, _ = batch.evaluate(0)
+        for i, a in enumerate(anns):
+            l, g, p = a.gradient()
+            assert abs(l - lnl[i]) <= 1e-9 * abs(l), shape
+            assert np.allclose(g, dbs[i], rtol=1e-6, atol=1e-6 * max(1.0, np.abs(g).max())), shape
+        batch.close()
+    anns[0].set_tuning(4, 64, 2)
     with pytest.raises(Exception):
         NetworkBatch(anns)
-    batch.close()
 
 
 def test_batch_gives_up_lds_pass_slots_and_results_stay_the_same():
