@@ -826,16 +826,44 @@ __global__ void k_grad_total(DevLayout L, const int32_t *__restrict__ edge_use_b
 // after the P tables are staged in LDS.  CLVs are FP32, renormalised at
 // every op by an exact power of two so the site maximum lies in [0.5, 1);
 // the per-site exponent k (true = stored * 2^-k) is carried as an int32.
-// Site sums, the mixture and the gradient reduction are FP64.
+// Site sums, the mixture and the gradient reduction across waves are FP64;
+// inside a wave the gradient's per-(site, category) shares (each of order
+// weight / branch length) are summed in FP32 per row of 16 lanes.
 constexpr double LN2 = 0.69314718055994530941723212145818;
 constexpr int EXP_EMPTY = -(1 << 28);  // exponent of an all-zero adjoint
 
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+
+// Maximum over the C category lanes of a site, comparing the float bit
+// patterns as signed integers: the float order for values >= 0, and every
+// positive value beats every negative one, which is all a site maximum
+// needs.  (fmaxf would add a NaN canonicalisation around every cross-lane
+// move; the integer max takes the DPP move as an operand modifier.)
 template <int C>
 __device__ __forceinline__ float cat_maxf(float m) {
-  if constexpr (C > 1) m = fmaxf(m, lane_xor<1>(m));
-  if constexpr (C > 2) m = fmaxf(m, lane_xor<2>(m));
-  if constexpr (C > 4) m = fmaxf(m, lane_xor<4>(m));
-  return m;
+  int u = __float_as_int(m);
+  if constexpr (C > 1) u = max(u, dpp_i<DPP_XOR1>(u));
+  if constexpr (C > 2) u = max(u, dpp_i<DPP_XOR2>(u));
+  if constexpr (C > 4) u = max(u, __shfl_xor(u, 4));
+  return __int_as_float(u);
+}
+
+// Sum over the 64 lanes of a wave of FP32 values, returned in FP64: four
+// row_shr steps in FP32 inside each row of 16 lanes (lane 15 of a row ends
+// with the row's sum), then the four row sums in FP64, in a fixed order.
+__device__ __forceinline__ double wave_sum_f32(float v) {
+  v += dpp<0x111>(v);
+  v += dpp<0x112>(v);
+  v += dpp<0x114>(v);
+  v += dpp<0x118>(v);
+  const int b = __float_as_int(v);
+  return (static_cast<double>(__int_as_float(__builtin_amdgcn_readlane(b, 15))) +
+          static_cast<double>(__int_as_float(__builtin_amdgcn_readlane(b, 31)))) +
+         (static_cast<double>(__int_as_float(__builtin_amdgcn_readlane(b, 47))) +
+          static_cast<double>(__int_as_float(__builtin_amdgcn_readlane(b, 63))));
 }
 
 __device__ __forceinline__ void tip_vec4(uint32_t mask, float (&x)[4]) {
@@ -1070,8 +1098,10 @@ __device__ __forceinline__ float site_absmax(const float (&x)[4]) {
 // between the sites of a wave.
 __device__ __forceinline__ int align_add(float (&x)[4], int ex, const float (&y)[4], int ey) {
   const int e = max(ex, ey);
+  // power-of-two scales: the products are exact, as ldexp would be
+  const float sx = ldexpf(1.0f, ex - e), sy = ldexpf(1.0f, ey - e);
 #pragma unroll
-  for (int i = 0; i < 4; ++i) x[i] = ldexpf(x[i], ex - e) + ldexpf(y[i], ey - e);
+  for (int i = 0; i < 4; ++i) x[i] = fmaf(x[i], sx, y[i] * sy);
   return e;
 }
 
@@ -1295,7 +1325,8 @@ __device__ __forceinline__ void adjoint_persistent_body(
             mv4(Qr, y[kk], qx);
             const float gd =
                 fmaf(z[kk][3], qx[3], fmaf(z[kk][2], qx[2], fmaf(z[kk][1], qx[1], z[kk][0] * qx[0])));
-            double gv = wave_sum(ldexp(static_cast<double>(gd), ez));
+            // the site's share of dlnL/dt is O(weight / t): FP32 holds it
+            const double gv = wave_sum_f32(ldexpf(gd, ez));
             if ((threadIdx.x & 63) == 0) grad_partials[grad_index(tile, n_uses, op.gslot[kk], PWB, wave)] = gv;
             const int32_t kid = op.kid[kk];
             if (kid >= 0 && (op.flags & (kk == 0 ? OP_PASS_KID0 : OP_PASS_KID1))) {
@@ -1356,12 +1387,13 @@ __device__ __forceinline__ float pair_swap(float v) { return dpp<DPP_XOR1>(v); }
 // maximum / sum over the 2C lanes of a site (halves at lane bit 0,
 // categories at bits 1 ..)
 template <int C>
-__device__ __forceinline__ float site_max2(float m) {
-  m = fmaxf(m, pair_swap(m));
-  if constexpr (C > 1) m = fmaxf(m, dpp<DPP_XOR2>(m));
-  if constexpr (C > 2) m = fmaxf(m, __shfl_xor(m, 4));
-  if constexpr (C > 4) m = fmaxf(m, __shfl_xor(m, 8));
-  return m;
+__device__ __forceinline__ float site_max2(float m) {  // integer max, as cat_maxf
+  int u = __float_as_int(m);
+  u = max(u, dpp_i<DPP_XOR1>(u));
+  if constexpr (C > 1) u = max(u, dpp_i<DPP_XOR2>(u));
+  if constexpr (C > 2) u = max(u, __shfl_xor(u, 4));
+  if constexpr (C > 4) u = max(u, __shfl_xor(u, 8));
+  return __int_as_float(u);
 }
 template <int C>
 __device__ __forceinline__ double site_sum2(double m) {
@@ -1401,8 +1433,9 @@ __device__ __forceinline__ void mv2t(const float *__restrict__ M, float z0, floa
 // x 2^ex + y 2^ey (two values), aligned to the larger exponent
 __device__ __forceinline__ int align_add2(float &x0, float &x1, int ex, float y0, float y1, int ey) {
   const int e = max(ex, ey);
-  x0 = ldexpf(x0, ex - e) + ldexpf(y0, ey - e);
-  x1 = ldexpf(x1, ex - e) + ldexpf(y1, ey - e);
+  const float sx = ldexpf(1.0f, ex - e), sy = ldexpf(1.0f, ey - e);
+  x0 = fmaf(x0, sx, y0 * sy);
+  x1 = fmaf(x1, sx, y1 * sy);
   return e;
 }
 
@@ -1578,7 +1611,7 @@ __device__ __forceinline__ void adjoint_split_body(
             mv2(Qr, y[kk][0], y[kk][1], pair_swap(y[kk][0]), pair_swap(y[kk][1]), q0, q1);
             mv2t(sP + (op.edge[kk] * C + c) * 16 + h * 8, z0, z1, b0, b1);
             const float gd = fmaf(z1, q1, z0 * q0);
-            const double gv = wave_sum(ldexp(static_cast<double>(gd), ez));
+            const double gv = wave_sum_f32(ldexpf(gd, ez));
             if ((threadIdx.x & 63) == 0) grad_partials[grad_index(tile, n_uses, op.gslot[kk], PWB, wave)] = gv;
             const int32_t kid = op.kid[kk];
             if (kid < 0) continue;
