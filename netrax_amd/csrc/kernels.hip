@@ -76,6 +76,24 @@ __device__ __forceinline__ const CAS T *to_constant(const T *p) {
   return (const CAS T *)p;
 }
 
+// A thread's word in one slot of a per-slot arena (CLVs, adjoints,
+// scalers, tip rows): the slot's base address is the same for the whole
+// wave (scalar arithmetic) and the thread's offset into every slot is one
+// fixed 32-bit byte offset, so each access is a scalar base plus that
+// offset, with no per-lane 64-bit address arithmetic.
+template <typename T>
+__device__ __forceinline__ T *at(T *slot_base, uint32_t byte_off) {
+  using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+  return reinterpret_cast<T *>(reinterpret_cast<B *>(slot_base) + byte_off);
+}
+#ifdef __HIP_DEVICE_COMPILE__
+template <typename T>
+__device__ __forceinline__ GAS T *at(GAS T *slot_base, uint32_t byte_off) {
+  using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+  return reinterpret_cast<GAS T *>(reinterpret_cast<GAS B *>(slot_base) + byte_off);
+}
+#endif
+
 // Cross-lane moves by DPP (a VALU operand modifier, no LDS round trip).
 // Controls: quad_perm(p0, p1, p2, p3) = p0 | p1 << 2 | p2 << 4 | p3 << 6
 // within each group of 4 lanes; row_shr:n = 0x110 + n shifts up by n lanes
@@ -621,8 +639,10 @@ __device__ __forceinline__ void adjoint_inputs(const DevOp &op, int64_t site, in
   }
 }
 
+// 4 states: at least 3 waves per SIMD (the register count the kernel
+// needs anyway); 20 states: the compiler's choice (forcing 3 would spill)
 template <int S, int C>
-__global__ void __launch_bounds__(TILE *C, 3)
+__global__ void __launch_bounds__(TILE *C, S == 4 ? 3 : 1)
     k_adjoint_level(const DevOp *__restrict__ ops, int n_ops, int n_staged, DevLayout L,
                     const double *__restrict__ clv, const int32_t *__restrict__ scl,
                     double *__restrict__ adj, const typename TipType<S>::type *__restrict__ tips,
@@ -1190,7 +1210,9 @@ __device__ __forceinline__ void adjoint_persistent_body(
   double pi[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) pi[i] = freqs[part * 4 + i];
-  const size_t row = C * 4;
+  const int64_t row_elems = n * C * 4;  // floats of one CLV / adjoint slot
+  const uint32_t off_row = static_cast<uint32_t>((site * C + c) * 4) * 4u;  // bytes into such a slot
+  const uint32_t off_site = static_cast<uint32_t>(site) * 4u;  // bytes into a scaler / exponent slot
   for (int gi = n_groups - 1; gi >= 0; --gi) {
     const int g0 = group_begin[gi];
     const int nb = group_begin[gi + 1] - g0;
@@ -1203,7 +1225,7 @@ __device__ __forceinline__ void adjoint_persistent_body(
     for (int j = 0; j < G; ++j) {
       if (j < nb) {
         const DevOp op = ops[g0 + j];
-        kq[j] = scl[static_cast<size_t>(op.out) * n + site];
+        kq[j] = *at(scl + op.out * n, off_site);
         if (op.flags & OP_HAS_CONSUMERS) {
           if (op.flags & OP_A_PASS) {
             const int slot = pass_slot(op.flags, 2);
@@ -1214,11 +1236,11 @@ __device__ __forceinline__ void adjoint_persistent_body(
             aq[j][3] = v.w;
             gq[j] = pass_e[slot * TS + site_in_tile];
           } else {
-            load4(adj + (static_cast<size_t>(op.out) * n + site) * row + c * 4, aq[j]);
-            gq[j] = adj_exp[static_cast<size_t>(op.out) * n + site];
+            load4(at(adj + op.out * row_elems, off_row), aq[j]);
+            gq[j] = *at(adj_exp + op.out * n, off_site);
           }
         }
-        if (op.root >= 0) load4(clv + (static_cast<size_t>(op.out) * n + site) * row + c * 4, xq[j]);
+        if (op.root >= 0) load4(at(clv + op.out * row_elems, off_row), xq[j]);
       }
     }
 #pragma unroll
@@ -1229,12 +1251,12 @@ __device__ __forceinline__ void adjoint_persistent_body(
           const DevOp op = ops[g0 + j];
           const int32_t kid = op.kid[kk];
           if (kk < op.nk && kid >= 0) {
-            load4(clv + (static_cast<size_t>(kid) * n + site) * row + c * 4, x[j][kk]);
-            sk[j][kk] = scl[static_cast<size_t>(kid) * n + site];
+            load4(at(clv + kid * row_elems, off_row), x[j][kk]);
+            sk[j][kk] = *at(scl + kid * n, off_site);
             if (op.flags & (kk == 0 ? OP_ACC_EARLY0 : OP_ACC_EARLY1)) {
               // the child's accumulated adjoint, last written two or more groups ago
-              load4(adj + (static_cast<size_t>(kid) * n + site) * row + c * 4, acc[j][kk]);
-              eacc[j][kk] = adj_exp[static_cast<size_t>(kid) * n + site];
+              load4(at(adj + kid * row_elems, off_row), acc[j][kk]);
+              eacc[j][kk] = *at(adj_exp + kid * n, off_site);
             }
           }
         }
@@ -1247,7 +1269,7 @@ __device__ __forceinline__ void adjoint_persistent_body(
         if (j < nb) {
           const DevOp op = ops[g0 + j];
           const int32_t kid = op.kid[kk];
-          if (kk < op.nk && kid < 0) tm[j][kk] = tips[static_cast<size_t>(-kid - 1) * n + site];
+          if (kk < op.nk && kid < 0) tm[j][kk] = *at(tips + (-kid - 1) * n, static_cast<uint32_t>(site));
         }
       }
     }
@@ -1334,8 +1356,8 @@ __device__ __forceinline__ void adjoint_persistent_body(
               pass_delivery<C>(pass_m + 4 * (slot * NT + threadIdx.x), pass_e + slot * TS + site_in_tile,
                                (op.flags & (kk == 0 ? OP_FIRST_KID0 : OP_FIRST_KID1)) != 0, back[kk], ez);
             } else if (kid >= 0) {
-              GAS float *dst_m = adj + (static_cast<size_t>(kid) * n + site) * row + c * 4;
-              GAS int32_t *dst_e = adj_exp + static_cast<size_t>(kid) * n + site;
+              GAS float *dst_m = at(adj + kid * row_elems, off_row);
+              GAS int32_t *dst_e = at(adj_exp + kid * n, off_site);
               const bool first = (op.flags & (kk == 0 ? OP_FIRST_KID0 : OP_FIRST_KID1)) != 0;
               if (first || (op.flags & (kk == 0 ? OP_ACC_EARLY0 : OP_ACC_EARLY1))) {
                 finish_delivery<C>(dst_m, dst_e, first, lane0, back[kk], ez, acc[j][kk], eacc[j][kk]);
