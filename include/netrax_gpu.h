@@ -176,6 +176,24 @@ int nrx_brlen_derivatives(nrx_engine *eng, int32_t variant, int32_t edge,
                           int32_t mode, double brlen, double *d1, double *d2,
                           double *part_d1, double *part_d2);
 
+/* nrx_brlen_derivatives without any device allocation.  No counterpart in
+ * the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving or optimisation loops, harnesses that
+ * forbid hipMalloc).  Same arguments, results and errors as
+ * nrx_brlen_derivatives, evaluated on the engine's state as left by the
+ * last nrx_loglikelihood / nrx_gradient of the same variant: NRX_ERR_STATE
+ * when parameters changed since (refreshing the CLVs could allocate).  Every
+ * scratch buffer (path tables, per-wave partial sums) lives in one fixed
+ * device array of 16 MiB per GPU, held until the process ends; an engine
+ * whose derivative scratch would exceed it is refused with NRX_ERR_ARG
+ * (about 32 displayed trees x 350k sites at 4 rate categories fit).  The
+ * paths are walked again on every call (no sumtables are kept).  Calls that
+ * overlap on one GPU (from several threads or streams) share that array and
+ * race: run them one at a time per device. */
+int nrx_brlen_derivatives_noalloc(nrx_engine *eng, int32_t variant, int32_t edge,
+                                  int32_t mode, double brlen, double *d1, double *d2,
+                                  double *part_d1, double *part_d2);
+
 /* Per-(partition, displayed tree) log-likelihoods of the last forward pass:
  * tree_logl [n_parts][n_classes], class_logprob [n_classes]. */
 int nrx_tree_logls(nrx_engine *eng, double *tree_logl, double *class_logprob);

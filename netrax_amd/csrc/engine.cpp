@@ -648,59 +648,74 @@ static int evaluate(Engine &e, int variant, bool incremental, bool update_pm, bo
 }
 
 
-// Sumtables of branch `edge` for every used displayed tree that contains it
-// (alive edge set, as isActiveBranch + the target variant existing,
-// LikelihoodDerivatives.cpp:313-343).  A tree whose root lies below the
-// branch (the branch is on the dead path above the displayed-tree root)
-// contributes derivative 0.
-static void build_sumtables(Engine &e, int edge) {
-  const int nc = static_cast<int>(e.dag.classes.size());
+// Every op by variant slot, for walking root-to-branch paths on the device.
+static std::vector<DevOp> ops_by_slot_table(const Engine &e) {
   const int nv = static_cast<int>(e.dag.var.size());
-  if (e.d_ops_by_slot.p == nullptr || e.d_ops_by_slot.bytes < sizeof(DevOp) * nv) {
-    std::vector<DevOp> all(nv);
-    for (int s = 0; s < nv; ++s) {
-      const Variant &v = e.dag.var[s];
-      DevOp op{};
-      op.out = s;
-      op.nk = v.nkids;
-      for (int k = 0; k < 2; ++k) {
-        op.kid[k] = k < v.nkids ? v.kid[k] : NO_CHILD;
-        op.edge[k] = k < v.nkids ? v.edge[k] : 0;
-      }
-      op.root = -1;
-      all[s] = op;
+  std::vector<DevOp> all(nv);
+  for (int s = 0; s < nv; ++s) {
+    const Variant &v = e.dag.var[s];
+    DevOp op{};
+    op.out = s;
+    op.nk = v.nkids;
+    for (int k = 0; k < 2; ++k) {
+      op.kid[k] = k < v.nkids ? v.kid[k] : NO_CHILD;
+      op.edge[k] = k < v.nkids ? v.edge[k] : 0;
     }
-    upload(e.d_ops_by_slot, all, e.stream);
+    op.root = -1;
+    all[s] = op;
   }
-  e.deriv_class.clear();
-  e.deriv_path.clear();
-  std::vector<PathStep> steps;
-  std::vector<int32_t> begin{0};
+  return all;
+}
+
+// Root-to-branch paths of branch `edge` for every used displayed tree that
+// contains it (alive edge set, as isActiveBranch + the target variant
+// existing, LikelihoodDerivatives.cpp:313-343).  cls: those classes;
+// path: per class its path index, or -1 when the tree's root lies below the
+// branch (the branch is on the dead path above the displayed-tree root and
+// contributes derivative 0); steps / begin: the paths' steps.
+static void make_deriv_paths(const Engine &e, int edge, std::vector<int> &cls, std::vector<int> &path,
+                             std::vector<PathStep> &steps, std::vector<int32_t> &begin) {
+  const int nc = static_cast<int>(e.dag.classes.size());
+  cls.clear();
+  path.clear();
+  steps.clear();
+  begin.assign(1, 0);
   for (int k = 0; k < nc; ++k) {
     if (!e.class_use[k]) continue;
     const auto &ed = e.dag.classes[k].edges;
     if (!((ed[edge / 64] >> (edge % 64)) & 1ull)) continue;
-    e.deriv_class.push_back(k);
+    cls.push_back(k);
     // depth-first search below the displayed-tree root for the op holding `edge`
-    std::vector<PathStep> path;
+    std::vector<PathStep> p;
     std::function<bool(int)> find = [&](int slot) -> bool {
       const Variant &v = e.dag.var[slot];
       for (int c = 0; c < v.nkids; ++c) {
-        path.push_back({slot, c});
+        p.push_back({slot, c});
         if (v.edge[c] == edge) return true;
         if (v.kid[c] >= 0 && find(v.kid[c])) return true;
-        path.pop_back();
+        p.pop_back();
       }
       return false;
     };
     if (find(e.dag.classes[k].dtroot_slot)) {
-      e.deriv_path.push_back(static_cast<int>(begin.size()) - 1);
-      steps.insert(steps.end(), path.begin(), path.end());
+      path.push_back(static_cast<int>(begin.size()) - 1);
+      steps.insert(steps.end(), p.begin(), p.end());
       begin.push_back(static_cast<int32_t>(steps.size()));
     } else {
-      e.deriv_path.push_back(-1);
+      path.push_back(-1);
     }
   }
+}
+
+// Sumtables of branch `edge` (the device-side computePartitionSumtables).
+static void build_sumtables(Engine &e, int edge) {
+  const int nc = static_cast<int>(e.dag.classes.size());
+  const int nv = static_cast<int>(e.dag.var.size());
+  if (e.d_ops_by_slot.p == nullptr || e.d_ops_by_slot.bytes < sizeof(DevOp) * nv)
+    upload(e.d_ops_by_slot, ops_by_slot_table(e), e.stream);
+  std::vector<PathStep> steps;
+  std::vector<int32_t> begin;
+  make_deriv_paths(e, edge, e.deriv_class, e.deriv_path, steps, begin);
   const int np = static_cast<int>(begin.size()) - 1;
   e.deriv_n_paths = np;
   if (np > 0) {
@@ -715,6 +730,85 @@ static void build_sumtables(Engine &e, int edge) {
   e.deriv_base_logl.assign(e.h_result + e.rl.tree_logl, e.h_result + e.rl.tree_logl + e.P * nc);
   e.deriv_edge = edge;
   e.deriv_valid = true;
+}
+
+// Per partition: each tree's lnL at the evaluated length and its first two
+// derivatives from the path sums (sums[path][3][P]: log rho, rho'/rho,
+// rho''/rho - (rho'/rho)^2), then the mixture of computePartitionLhData
+// (LikelihoodDerivatives.cpp:30-188) or the true derivative.
+static void mix_brlen_derivatives(const Engine &e, int variant, int mode, const std::vector<double> &base_logl,
+                                  const std::vector<int> &cls, const std::vector<int> &path,
+                                  const double *sums, double *d1, double *d2, double *part_d1,
+                                  double *part_d2) {
+  const int nc = static_cast<int>(e.dag.classes.size());
+  double D1 = 0.0, D2 = 0.0;
+  std::vector<double> ll(nc), l1(nc), l2(nc);
+  std::vector<char> has(nc, 0);
+  for (int k : cls) has[k] = 1;
+  for (int p = 0; p < e.P; ++p) {
+    for (int k = 0; k < nc; ++k) {
+      ll[k] = base_logl[static_cast<size_t>(p) * nc + k];
+      l1[k] = l2[k] = 0.0;
+    }
+    for (size_t i = 0; i < cls.size(); ++i) {
+      const int k = cls[i], j = path[i];
+      if (j < 0) continue;
+      const double *v = sums + static_cast<size_t>(j) * 3 * e.P;
+      ll[k] += v[0 * e.P + p];
+      l1[k] = v[1 * e.P + p];
+      l2[k] = v[2 * e.P + p];
+    }
+    double pd1 = 0.0, pd2 = 0.0;
+    std::vector<int> members;
+    for (int k = 0; k < nc; ++k)
+      if (e.class_use[k] && (mode == NRX_DERIV_TRUE || has[k])) members.push_back(k);
+    if (members.size() == 1 && mode == NRX_DERIV_REFERENCE) {
+      pd1 = l1[members[0]];  // single-tree mode (LikelihoodDerivatives.cpp:145-151)
+      pd2 = l2[members[0]];
+    } else if (!members.empty()) {
+      if (variant == NRX_AVERAGE_DISPLAYED_TREES) {
+        double mx = -INFINITY;
+        for (int k : members) mx = std::max(mx, e.class_logprob[k] + ll[k]);
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        for (int k : members) {
+          const double wgt = std::exp(e.class_logprob[k] + ll[k] - mx);
+          s0 += wgt;
+          s1 += wgt * l1[k];
+          s2 += wgt * (l1[k] * l1[k] + l2[k]);
+        }
+        pd1 = s1 / s0;
+        pd2 = (s2 * s0 - s1 * s1) / (s0 * s0);
+      } else if (mode == NRX_DERIV_REFERENCE) {
+        // the reference picks the tree maximising tree_logl * tree_prob
+        // (LikelihoodDerivatives.cpp:161-167)
+        double best = -INFINITY;
+        for (int k : members) {
+          const double score = ll[k] * std::exp(e.class_logprob[k]);
+          if (score > best) {
+            best = score;
+            pd1 = l1[k];
+            pd2 = l2[k];
+          }
+        }
+      } else {
+        double best = -INFINITY;
+        for (int k : members) {
+          const double score = e.class_logprob[k] + ll[k];
+          if (score > best) {
+            best = score;
+            pd1 = l1[k];
+            pd2 = l2[k];
+          }
+        }
+      }
+    }
+    if (part_d1) part_d1[p] = pd1;
+    if (part_d2) part_d2[p] = pd2;
+    D1 += pd1;
+    D2 += pd2;
+  }
+  *d1 = D1;
+  *d2 = D2;
 }
 
 
@@ -1307,7 +1401,6 @@ int nrx_brlen_derivatives(nrx_engine *eng, int32_t variant, int32_t edge, int32_
       return fail(NRX_ERR_UNSUPPORTED,
                   "per-branch sumtables of 20-state data need the FP64 form (the FP32 form "
                   "returns the gradient of every branch through nrx_gradient)");
-    const int nc = static_cast<int>(e.dag.classes.size());
     if (!e.deriv_valid || e.deriv_edge != edge) {
       const int rc = evaluate(e, variant, true, true, false);
       if (rc != NRX_OK) return rc;
@@ -1320,76 +1413,77 @@ int nrx_brlen_derivatives(nrx_engine *eng, int32_t variant, int32_t edge, int32_
                              hipMemcpyDeviceToHost, e.stream));
       NRX_HIP(hipStreamSynchronize(e.stream));
     }
-    // per partition: tree lnL at brlen and its first two derivatives, then
-    // the mixture of computePartitionLhData (LikelihoodDerivatives.cpp:30-188)
-    double D1 = 0.0, D2 = 0.0;
-    std::vector<double> ll(nc), l1(nc), l2(nc);
-    std::vector<char> has(nc, 0);
-    for (size_t i = 0; i < e.deriv_class.size(); ++i) has[e.deriv_class[i]] = 1;
-    for (int p = 0; p < e.P; ++p) {
-      for (int k = 0; k < nc; ++k) {
-        ll[k] = e.deriv_base_logl[static_cast<size_t>(p) * nc + k];
-        l1[k] = l2[k] = 0.0;
-      }
-      for (size_t i = 0; i < e.deriv_class.size(); ++i) {
-        const int k = e.deriv_class[i], j = e.deriv_path[i];
-        if (j < 0) continue;
-        const double *v = &e.h_st_sums[static_cast<size_t>(j) * 3 * e.P];
-        ll[k] += v[0 * e.P + p];
-        l1[k] = v[1 * e.P + p];
-        l2[k] = v[2 * e.P + p];
-      }
-      double pd1 = 0.0, pd2 = 0.0;
-      std::vector<int> members;
-      for (int k = 0; k < nc; ++k)
-        if (e.class_use[k] && (mode == NRX_DERIV_TRUE || has[k])) members.push_back(k);
-      if (members.size() == 1 && mode == NRX_DERIV_REFERENCE) {
-        pd1 = l1[members[0]];  // single-tree mode (LikelihoodDerivatives.cpp:145-151)
-        pd2 = l2[members[0]];
-      } else if (!members.empty()) {
-        if (variant == NRX_AVERAGE_DISPLAYED_TREES) {
-          double mx = -INFINITY;
-          for (int k : members) mx = std::max(mx, e.class_logprob[k] + ll[k]);
-          double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-          for (int k : members) {
-            const double wgt = std::exp(e.class_logprob[k] + ll[k] - mx);
-            s0 += wgt;
-            s1 += wgt * l1[k];
-            s2 += wgt * (l1[k] * l1[k] + l2[k]);
-          }
-          pd1 = s1 / s0;
-          pd2 = (s2 * s0 - s1 * s1) / (s0 * s0);
-        } else if (mode == NRX_DERIV_REFERENCE) {
-          // the reference picks the tree maximising tree_logl * tree_prob
-          // (LikelihoodDerivatives.cpp:161-167)
-          double best = -INFINITY;
-          for (int k : members) {
-            const double score = ll[k] * std::exp(e.class_logprob[k]);
-            if (score > best) {
-              best = score;
-              pd1 = l1[k];
-              pd2 = l2[k];
-            }
-          }
-        } else {
-          double best = -INFINITY;
-          for (int k : members) {
-            const double score = e.class_logprob[k] + ll[k];
-            if (score > best) {
-              best = score;
-              pd1 = l1[k];
-              pd2 = l2[k];
-            }
-          }
-        }
-      }
-      if (part_d1) part_d1[p] = pd1;
-      if (part_d2) part_d2[p] = pd2;
-      D1 += pd1;
-      D2 += pd2;
+    mix_brlen_derivatives(e, variant, mode, e.deriv_base_logl, e.deriv_class, e.deriv_path, e.h_st_sums.data(),
+                          d1, d2, part_d1, part_d2);
+    return NRX_OK;
+  });
+}
+
+int nrx_brlen_derivatives_noalloc(nrx_engine *eng, int32_t variant, int32_t edge, int32_t mode, double brlen,
+                                  double *d1, double *d2, double *part_d1, double *part_d2) {
+  return guarded([&]() -> int {
+    if (!eng || !d1 || !d2) throw std::invalid_argument("null argument");
+    Engine &e = *static_cast<Engine *>(eng);
+    if (edge < 0 || edge >= e.E) throw std::invalid_argument("edge index out of range");
+    if (mode != NRX_DERIV_REFERENCE && mode != NRX_DERIV_TRUE)
+      throw std::invalid_argument("unknown derivative mode");
+    if (variant != NRX_AVERAGE_DISPLAYED_TREES && variant != NRX_BEST_DISPLAYED_TREE)
+      throw std::invalid_argument("unknown likelihood variant");
+    if (!(brlen >= 0.0)) throw std::invalid_argument("branch length must be >= 0");
+    brlen = std::min(std::max(brlen, e.opt.brlen_min), e.opt.brlen_max);
+    if (e.opt.precision == NRX_FP32 && e.S == AA_STATES)
+      return fail(NRX_ERR_UNSUPPORTED,
+                  "per-branch derivatives of 20-state data need the FP64 form (the FP32 form "
+                  "returns the gradient of every branch through nrx_gradient)");
+    std::vector<int> cls, path;
+    std::vector<PathStep> steps;
+    std::vector<int32_t> begin;
+    make_deriv_paths(e, edge, cls, path, steps, begin);
+    const int np = static_cast<int>(begin.size()) - 1;
+    const int nc = static_cast<int>(e.dag.classes.size());
+    // layout of the fixed scratch: op table, path steps and bounds, per-wave
+    // partials, per-(path, partition) sums
+    const std::vector<DevOp> table = ops_by_slot_table(e);
+    auto aligned = [](size_t b) { return (b + 255) & ~size_t(255); };
+    const size_t o_ops = 0;
+    const size_t o_steps = aligned(o_ops + sizeof(DevOp) * table.size());
+    const size_t o_begin = aligned(o_steps + sizeof(PathStep) * steps.size());
+    const size_t o_part = aligned(o_begin + sizeof(int32_t) * begin.size());
+    const size_t o_sums = aligned(o_part + sizeof(double) * np * 3 * static_cast<size_t>(e.L.n_tiles) * e.pw);
+    std::vector<double> sums(static_cast<size_t>(np) * 3 * e.P, 0.0);
+    const size_t need = o_sums + sizeof(double) * sums.size();
+    if (need > NOALLOC_SCRATCH_BYTES)
+      throw std::invalid_argument("branch derivatives of this engine need " + std::to_string(need) +
+                                  " bytes of scratch, more than the no-alloc limit of " +
+                                  std::to_string(NOALLOC_SCRATCH_BYTES));
+    // the CLVs, P matrices and tree lnLs of the last evaluation are used as
+    // they are: refreshing them could allocate
+    const bool stale = !e.cache_valid || e.cache_variant != variant || e.model_dirty || e.probs_dirty ||
+                       std::any_of(e.edge_dirty.begin(), e.edge_dirty.end(), [](char c) { return c; }) ||
+                       std::any_of(e.clv_dirty.begin(), e.clv_dirty.end(), [](char c) { return c; });
+    if (stale)
+      return fail(NRX_ERR_STATE,
+                  "parameters changed since the last evaluation of this variant: call nrx_loglikelihood "
+                  "(or nrx_gradient) first");
+    if (np > 0) {
+      NRX_HIP(hipSetDevice(e.opt.device));
+      char *base = static_cast<char *>(noalloc_scratch());
+      NRX_HIP(hipMemcpyAsync(base + o_ops, table.data(), sizeof(DevOp) * table.size(), hipMemcpyHostToDevice,
+                             e.stream));
+      NRX_HIP(hipMemcpyAsync(base + o_steps, steps.data(), sizeof(PathStep) * steps.size(),
+                             hipMemcpyHostToDevice, e.stream));
+      NRX_HIP(hipMemcpyAsync(base + o_begin, begin.data(), sizeof(int32_t) * begin.size(),
+                             hipMemcpyHostToDevice, e.stream));
+      launch_deriv_fused(e, reinterpret_cast<const PathStep *>(base + o_steps),
+                         reinterpret_cast<const int32_t *>(base + o_begin), np,
+                         reinterpret_cast<const DevOp *>(base + o_ops), brlen,
+                         reinterpret_cast<double *>(base + o_part), reinterpret_cast<double *>(base + o_sums));
+      NRX_HIP(hipMemcpyAsync(sums.data(), base + o_sums, sizeof(double) * sums.size(), hipMemcpyDeviceToHost,
+                             e.stream));
+      NRX_HIP(hipStreamSynchronize(e.stream));
     }
-    *d1 = D1;
-    *d2 = D2;
+    const std::vector<double> base_logl(e.h_result + e.rl.tree_logl, e.h_result + e.rl.tree_logl + e.P * nc);
+    mix_brlen_derivatives(e, variant, mode, base_logl, cls, path, sums.data(), d1, d2, part_d1, part_d2);
     return NRX_OK;
   });
 }

@@ -225,6 +225,12 @@ void launch_mfma_tables(Engine &e);
 void launch_forward_mfma(Engine &e);
 void launch_adjoint_mfma(Engine &e);
 void launch_sumtables(Engine &e, int n_paths);
+// allocation-free branch derivatives: a fixed device array of this many
+// bytes per GPU holds every scratch buffer of nrx_brlen_derivatives_noalloc
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(16) << 20;
+void *noalloc_scratch();  // its device address on the current device
+void launch_deriv_fused(Engine &e, const PathStep *steps, const int32_t *path_begin, int n_paths,
+                        const DevOp *ops_by_slot, double brlen, double *partials, double *sums);
 void launch_sumtable_eval(Engine &e, int n_paths, double brlen);
 
 struct HipCheck {

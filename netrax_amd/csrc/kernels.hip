@@ -15,6 +15,10 @@
 // workgroup ids) and its children's CLVs stay in that XCD's L2.
 #include <cfloat>
 #include <cmath>
+#include <map>
+#include <mutex>
+#include <stdexcept>
+#include <type_traits>
 
 #include "engine_state.hpp"
 
@@ -2115,6 +2119,79 @@ __device__ __forceinline__ void load_any(const float *p, double (&x)[S]) {
   for (int i = 0; i < S; ++i) x[i] = static_cast<double>(p[i]);
 }
 
+// Sumtable row of path j at (site, category): the path adjoint z walked
+// from the displayed tree's root to the branch, then (z^T V)_k (V^-1 x)_k
+// with x the CLV below the branch.
+template <int S, int C, typename T>
+__device__ __forceinline__ void path_sumtable(const PathStep *__restrict__ steps, int b0, int b1,
+                                              const DevOp *__restrict__ ops_by_slot, const DevLayout &L,
+                                              int part, int64_t site, int c, const T *__restrict__ clv,
+                                              const int32_t *__restrict__ scl, int scale_unit,
+                                              const typename TipType<S>::type *__restrict__ tips,
+                                              const double *__restrict__ P, const double *__restrict__ V,
+                                              const double *__restrict__ Vinv,
+                                              const double *__restrict__ freqs, double (&st)[S]) {
+  const int64_t n = L.n_sites_pad;
+  const size_t row = static_cast<size_t>(C * S);
+  const double *Vp = V + static_cast<size_t>(part) * S * S;
+  const double *Wp = Vinv + static_cast<size_t>(part) * S * S;
+  auto prow = [&](int edge) {
+    return P + ((static_cast<size_t>(part) * L.n_edges + edge) * C + c) * (S * S);
+  };
+  const int root = steps[b0].op;
+  double a[S];
+  {
+    double xr[S];
+    load_any<S>(clv + (static_cast<size_t>(root) * n + site) * row + c * S, xr);
+    double t = 0.0;
+#pragma unroll
+    for (int i = 0; i < S; ++i) t = fma(freqs[part * S + i], xr[i], t);
+    t = cat_sum<C>(t) * (1.0 / C);
+#pragma unroll
+    for (int i = 0; i < S; ++i) a[i] = freqs[part * S + i] / (C * t);
+  }
+  for (int sidx = b0; sidx < b1; ++sidx) {
+    const DevOp op = ops_by_slot[steps[sidx].op];
+    const int k = steps[sidx].k;
+    const int kq = scl[static_cast<size_t>(op.out) * n + site];
+    const int32_t child = op.kid[k];
+    const int kc = child >= 0 ? scl[static_cast<size_t>(child) * n + site] : 0;
+    double y[S];
+    int ks = 0;
+    if (op.nk == 2) {
+      double xs[S];
+      const int32_t sib = op.kid[1 - k];
+      if (sib >= 0) {
+        load_any<S>(clv + (static_cast<size_t>(sib) * n + site) * row + c * S, xs);
+        ks = scl[static_cast<size_t>(sib) * n + site];
+      } else {
+        tip_vec<S>(tips[static_cast<size_t>(-sib - 1) * n + site], xs);
+      }
+      matvec<S>(prow(op.edge[1 - k]), xs, y);
+    } else {
+#pragma unroll
+      for (int i = 0; i < S; ++i) y[i] = 1.0;
+    }
+    const double F = ldexp(1.0, scale_unit * (kq - kc - ks));
+    double z[S];
+#pragma unroll
+    for (int i = 0; i < S; ++i) z[i] = a[i] * F * y[i];
+    if (sidx + 1 < b1) {
+      matvec_t<S>(prow(op.edge[k]), z, a);  // adjoint of the on-path child
+    } else {
+      double xc[S], u[S], v[S];
+      if (child >= 0)
+        load_any<S>(clv + (static_cast<size_t>(child) * n + site) * row + c * S, xc);
+      else
+        tip_vec<S>(tips[static_cast<size_t>(-child - 1) * n + site], xc);
+      matvec_t<S>(Vp, z, u);  // u_k = sum_i z_i V[i][k]
+      matvec<S>(Wp, xc, v);   // v_k = sum_j Vinv[k][j] x_j
+#pragma unroll
+      for (int kk = 0; kk < S; ++kk) st[kk] = u[kk] * v[kk];
+    }
+  }
+}
+
 template <int S, int C, typename T>
 __global__ void __launch_bounds__(TILE *C)
     k_sumtables(const PathStep *__restrict__ steps, const int32_t *__restrict__ path_begin,
@@ -2129,73 +2206,46 @@ __global__ void __launch_bounds__(TILE *C)
   const int part = tile_part[tile];
   const int c = threadIdx.x % C;
   const int64_t site = static_cast<int64_t>(tile) * TILE + threadIdx.x / C;
-  const int64_t n = L.n_sites_pad;
   const size_t row = static_cast<size_t>(C * S);
-  const double *Vp = V + static_cast<size_t>(part) * S * S;
-  const double *Wp = Vinv + static_cast<size_t>(part) * S * S;
-  auto prow = [&](int edge) {
-    return P + ((static_cast<size_t>(part) * L.n_edges + edge) * C + c) * (S * S);
-  };
   for (int j = 0; j < n_paths; ++j) {
-    const int b0 = path_begin[j], b1 = path_begin[j + 1];
-    const int root = steps[b0].op;
-    double a[S];
-    {
-      double xr[S];
-      load_any<S>(clv + (static_cast<size_t>(root) * n + site) * row + c * S, xr);
-      double t = 0.0;
+    double stv[S];
+    path_sumtable<S, C, T>(steps, path_begin[j], path_begin[j + 1], ops_by_slot, L, part, site, c, clv, scl,
+                           scale_unit, tips, P, V, Vinv, freqs, stv);
+    double *out = st + (static_cast<size_t>(j) * L.n_sites_pad + site) * row + c * S;
 #pragma unroll
-      for (int i = 0; i < S; ++i) t = fma(freqs[part * S + i], xr[i], t);
-      t = cat_sum<C>(t) * (1.0 / C);
-#pragma unroll
-      for (int i = 0; i < S; ++i) a[i] = freqs[part * S + i] / (C * t);
-    }
-    for (int sidx = b0; sidx < b1; ++sidx) {
-      const DevOp op = ops_by_slot[steps[sidx].op];
-      const int k = steps[sidx].k;
-      const int kq = scl[static_cast<size_t>(op.out) * n + site];
-      const int32_t child = op.kid[k];
-      const int kc = child >= 0 ? scl[static_cast<size_t>(child) * n + site] : 0;
-      double y[S];
-      int ks = 0;
-      if (op.nk == 2) {
-        double xs[S];
-        const int32_t sib = op.kid[1 - k];
-        if (sib >= 0) {
-          load_any<S>(clv + (static_cast<size_t>(sib) * n + site) * row + c * S, xs);
-          ks = scl[static_cast<size_t>(sib) * n + site];
-        } else {
-          tip_vec<S>(tips[static_cast<size_t>(-sib - 1) * n + site], xs);
-        }
-        matvec<S>(prow(op.edge[1 - k]), xs, y);
-      } else {
-#pragma unroll
-        for (int i = 0; i < S; ++i) y[i] = 1.0;
-      }
-      const double F = ldexp(1.0, scale_unit * (kq - kc - ks));
-      double z[S];
-#pragma unroll
-      for (int i = 0; i < S; ++i) z[i] = a[i] * F * y[i];
-      if (sidx + 1 < b1) {
-        matvec_t<S>(prow(op.edge[k]), z, a);  // adjoint of the on-path child
-      } else {
-        double xc[S], u[S], v[S];
-        if (child >= 0)
-          load_any<S>(clv + (static_cast<size_t>(child) * n + site) * row + c * S, xc);
-        else
-          tip_vec<S>(tips[static_cast<size_t>(-child - 1) * n + site], xc);
-        matvec_t<S>(Vp, z, u);  // u_k = sum_i z_i V[i][k]
-        matvec<S>(Wp, xc, v);   // v_k = sum_j Vinv[k][j] x_j
-        double *out = st + (static_cast<size_t>(j) * n + site) * row + c * S;
-#pragma unroll
-        for (int kk = 0; kk < S; ++kk) out[kk] = u[kk] * v[kk];
-      }
-    }
+    for (int kk = 0; kk < S; ++kk) out[kk] = stv[kk];
   }
 }
 
 // Per displayed tree j and wave: sums over sites of w log rho(b),
-// w rho'/rho and w (rho''/rho - (rho'/rho)^2) at branch length b.
+// w rho'/rho and w (rho''/rho - (rho'/rho)^2) at branch length b, from the
+// sumtable row of (site, category).
+template <int S, int C>
+__device__ __forceinline__ void sumtable_partials(const double (&in)[S], const double (&ex)[S],
+                                                  const double (&lr)[S], double w, int c, double *out3) {
+  double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+#pragma unroll
+  for (int k = 0; k < S; ++k) {
+    const double q = in[k] * ex[k];
+    t0 += q;
+    t1 += q * lr[k];
+    t2 += q * lr[k] * lr[k];
+  }
+  t0 = cat_sum<C>(t0);
+  t1 = cat_sum<C>(t1);
+  t2 = cat_sum<C>(t2);
+  double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+  if (c == 0 && w != 0.0) {
+    const double r1 = t1 / t0;
+    v0 = w * log(t0);
+    v1 = w * r1;
+    v2 = w * (t2 / t0 - r1 * r1);
+  }
+  out3[0] = wave_sum(v0);
+  out3[1] = wave_sum(v1);
+  out3[2] = wave_sum(v2);
+}
+
 template <int S, int C>
 __global__ void __launch_bounds__(TILE *C)
     k_sumtable_eval(const double *__restrict__ st, int n_paths, DevLayout L,
@@ -2222,32 +2272,53 @@ __global__ void __launch_bounds__(TILE *C)
   }
   for (int j = 0; j < n_paths; ++j) {
     const double *in = st + (static_cast<size_t>(j) * n + site) * row + c * S;
-    double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+    double stv[S], v[3];
 #pragma unroll
-    for (int k = 0; k < S; ++k) {
-      const double q = in[k] * ex[k];
-      t0 += q;
-      t1 += q * lr[k];
-      t2 += q * lr[k] * lr[k];
-    }
-    t0 = cat_sum<C>(t0);
-    t1 = cat_sum<C>(t1);
-    t2 = cat_sum<C>(t2);
-    double v0 = 0.0, v1 = 0.0, v2 = 0.0;
-    if (c == 0 && w != 0.0) {
-      const double r1 = t1 / t0;
-      v0 = w * log(t0);
-      v1 = w * r1;
-      v2 = w * (t2 / t0 - r1 * r1);
-    }
-    v0 = wave_sum(v0);
-    v1 = wave_sum(v1);
-    v2 = wave_sum(v2);
-    if ((threadIdx.x & 63) == 0) {
-      partials[(static_cast<size_t>(j) * 3 + 0) * n_wt + wt] = v0;
-      partials[(static_cast<size_t>(j) * 3 + 1) * n_wt + wt] = v1;
-      partials[(static_cast<size_t>(j) * 3 + 2) * n_wt + wt] = v2;
-    }
+    for (int k = 0; k < S; ++k) stv[k] = in[k];
+    sumtable_partials<S, C>(stv, ex, lr, w, c, v);
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) partials[(static_cast<size_t>(j) * 3 + q) * n_wt + wt] = v[q];
+  }
+}
+
+// The two kernels above in one, with no sumtable stored: each (path, site,
+// category) walks its path adjoint and evaluates at b right away (the
+// allocation-free variant of the branch derivatives re-walks the paths on
+// every call instead of keeping sumtables).
+template <int S, int C, typename T>
+__global__ void __launch_bounds__(TILE *C)
+    k_deriv_fused(const PathStep *__restrict__ steps, const int32_t *__restrict__ path_begin, int n_paths,
+                  const DevOp *__restrict__ ops_by_slot, DevLayout L, const int32_t *__restrict__ tile_part,
+                  const T *__restrict__ clv, const int32_t *__restrict__ scl, int scale_unit,
+                  const typename TipType<S>::type *__restrict__ tips, const double *__restrict__ P,
+                  const double *__restrict__ V, const double *__restrict__ Vinv,
+                  const double *__restrict__ freqs, const double *__restrict__ ev,
+                  const double *__restrict__ rates, const double *__restrict__ weights, double brlen,
+                  double *__restrict__ partials) {
+  constexpr int PW = TILE * C / 64;
+  const int tile = blockIdx.x;
+  const int part = tile_part[tile];
+  const int c = threadIdx.x % C;
+  const int64_t site = static_cast<int64_t>(tile) * TILE + threadIdx.x / C;
+  const size_t n_wt = static_cast<size_t>(L.n_tiles) * PW;
+  const int wt = tile * PW + threadIdx.x / 64;
+  const double w = weights[site];
+  const double r = rates[part * C + c];
+  double ex[S], lr[S];
+#pragma unroll
+  for (int k = 0; k < S; ++k) {
+    lr[k] = ev[part * S + k] * r;
+    ex[k] = exp(lr[k] * brlen);
+  }
+  for (int j = 0; j < n_paths; ++j) {
+    double stv[S], v[3];
+    path_sumtable<S, C, T>(steps, path_begin[j], path_begin[j + 1], ops_by_slot, L, part, site, c, clv, scl,
+                           scale_unit, tips, P, V, Vinv, freqs, stv);
+    sumtable_partials<S, C>(stv, ex, lr, w, c, v);
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) partials[(static_cast<size_t>(j) * 3 + q) * n_wt + wt] = v[q];
   }
 }
 
@@ -2667,6 +2738,53 @@ void launch_sumtable_eval(Engine &e, int n_paths, double brlen) {
   NRX_HIP(hipGetLastError());
 }
 
+
+// Scratch of the allocation-free branch derivatives (nrx_brlen_derivatives_noalloc):
+// a fixed device array, one copy per GPU, whose address host code takes
+// once per device from hipGetSymbolAddress (the array's name in host code
+// is not a device address).
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+void *noalloc_scratch() {
+  static std::mutex mu;
+  static std::map<int, void *> ptr_of_device;
+  int dev = 0;
+  NRX_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(mu);
+  const auto it = ptr_of_device.find(dev);
+  if (it != ptr_of_device.end()) return it->second;
+  void *ptr = nullptr;
+  NRX_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  if (!ptr) throw std::runtime_error("HIP error: no device address for the no-alloc scratch");
+  ptr_of_device[dev] = ptr;
+  return ptr;
+}
+
+void launch_deriv_fused(Engine &e, const PathStep *steps, const int32_t *path_begin, int n_paths,
+                        const DevOp *ops_by_slot, double brlen, double *partials, double *sums) {
+  if (n_paths == 0) return;
+  dispatch_sc(e.S, e.C, [&](auto s, auto c) {
+    constexpr int S = decltype(s)::value, C = decltype(c)::value;
+    using TT = typename TipType<S>::type;
+    if (e.opt.precision == NRX_FP32)
+      hipLaunchKernelGGL((k_deriv_fused<S, C, float>), dim3(e.L.n_tiles), dim3(TILE * C), 0, e.stream, steps,
+                         path_begin, n_paths, ops_by_slot, e.L, e.d_tile_part.as<int32_t>(), e.d_clv.as<float>(),
+                         e.d_scl.as<int32_t>(), 1, e.d_tips.as<TT>(), e.d_P.as<double>(), e.d_V.as<double>(),
+                         e.d_Vinv.as<double>(), e.d_freqs.as<double>(), e.d_eval.as<double>(),
+                         e.d_rates.as<double>(), e.d_weights.as<double>(), brlen, partials);
+    else
+      hipLaunchKernelGGL((k_deriv_fused<S, C, double>), dim3(e.L.n_tiles), dim3(TILE * C), 0, e.stream, steps,
+                         path_begin, n_paths, ops_by_slot, e.L, e.d_tile_part.as<int32_t>(),
+                         e.d_clv.as<double>(), e.d_scl.as<int32_t>(), 256, e.d_tips.as<TT>(),
+                         e.d_P.as<double>(), e.d_V.as<double>(), e.d_Vinv.as<double>(), e.d_freqs.as<double>(),
+                         e.d_eval.as<double>(), e.d_rates.as<double>(), e.d_weights.as<double>(), brlen,
+                         partials);
+  });
+  NRX_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_reduce_segments, dim3(n_paths * 3 * e.P), dim3(256), 0, e.stream, partials,
+                     static_cast<size_t>(e.L.n_tiles) * e.pw, e.pw, e.d_part_tile_begin.as<int32_t>(), e.P, sums);
+  NRX_HIP(hipGetLastError());
+}
 
 void launch_pmatrices_multi(hipStream_t s, const SmallDesc *d, int n, int max_elems) {
   hipLaunchKernelGGL(k_pmatrices_multi, dim3((max_elems + 255) / 256, n), dim3(256), 0, s, d);
