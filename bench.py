@@ -266,6 +266,38 @@ def bench_candidates(n_cand, steps, warmup):
     return out
 
 
+def bench_newton_sweep(w):
+    """Newton-Raphson sweep over every branch of the headline network (plain
+    FP64 engine, reference-normalised derivatives,
+    BranchLengthOptimization.cpp:422-477) from the same perturbed lengths:
+    the device iteration (nrx_optimize_branches: per branch one captured
+    graph of Newton rounds and one host wait) against the host-driven loop
+    (netrax_amd.optimization.optimize_branches: one C-ABI call and one wait
+    per Newton round)."""
+    from netrax_amd import AnnotatedNetwork, NetraxOptions
+    from netrax_amd.optimization import optimize_branches
+
+    rng = np.random.default_rng(11)
+    start = w.network.brlen * rng.uniform(0.5, 2.0, w.network.n_edges)
+    res = {}
+    for name in ("device", "host"):
+        net = w.network.copy()
+        net.brlen = start.copy()
+        ann = AnnotatedNetwork(net, w.partitions, NetraxOptions())
+        l0 = ann.loglikelihood()
+        t0 = time.perf_counter()
+        l1 = ann.optimize_branches() if name == "device" else optimize_branches(ann)
+        res[name] = (time.perf_counter() - t0, l0, l1, ann.network.brlen.copy())
+        ann.close()
+    dv, ho = res["device"], res["host"]
+    out = {"workload": "every branch of the 50-taxon network (110), plain FP64 engine, lengths perturbed x U(0.5, 2)",
+           "device_sweep_s": round(dv[0], 4), "host_sweep_s": round(ho[0], 4),
+           "lnl_start": dv[1], "lnl_device": dv[2], "lnl_host": ho[2],
+           "max_rel_brlen_diff": float(np.max(np.abs(dv[3] - ho[3]) / np.maximum(np.abs(ho[3]), 1e-12)))}
+    log(f"newton sweep: device {dv[0]:.3f}s, host {ho[0]:.3f}s, lnL {dv[1]:.4f} -> {dv[2]:.4f}")
+    return out
+
+
 def cpu_baseline(w, sample_patterns, threads, budget_s):
     from netrax_amd.simulate import Partition
     from oracle import oracle as O
@@ -482,6 +514,7 @@ def main():
                                                / max(1.0, np.abs(p64["grad"]).max())),
         }
     if not args.no_extras and not args.only:
+        out["newton_sweep"] = bench_newton_sweep(w)
         out["protein"] = bench_protein(max(5, args.steps // 2), args.warmup)
         out["candidate_batch"] = bench_candidates(args.candidates, 3, 1)
     if not args.no_oracle_check:

@@ -194,6 +194,27 @@ int nrx_brlen_derivatives_noalloc(nrx_engine *eng, int32_t variant, int32_t edge
                                   int32_t mode, double brlen, double *d1, double *d2,
                                   double *part_d1, double *part_d2);
 
+/* Newton-Raphson branch-length sweep with the iteration on the device.
+ * Replaces optimize_branches / optimize_branch /
+ * optimize_branch_newton_raphson (src/optimization/BranchLengthOptimization.cpp:
+ * 202-241, 317-330, 345-420, 422-477): every listed edge (duplicates
+ * dropped, taken in index order) is optimised once, within [brlen_min,
+ * brlen_max] to a tolerance of brlen_min / 10, from its sumtables' first
+ * and second derivatives (mode: NRX_DERIV_REFERENCE or NRX_DERIV_TRUE); a
+ * length that does not improve lnL is given up.  Per edge: the forward
+ * pass and sumtables on the host's order, then a captured graph runs six
+ * Newton rounds (FP64 derivative mixture, safeguarded step and bracket on
+ * the device; rounds after convergence do nothing) and the acceptance
+ * check, and the host waits once; a branch not converged after six rounds
+ * takes another launch, up to max_iters rounds in all.
+ * *logl (may be NULL) receives the final network lnL; fails with
+ * NRX_ERR_STATE ("Overall loglikelihood got worse") when the sweep lost
+ * more than 1e-3.  The new lengths are read with nrx_get_branch_lengths. */
+int nrx_optimize_branches(nrx_engine *eng, int32_t variant, int32_t mode, const int32_t *edges,
+                          int32_t n_edges, int32_t max_iters, double *logl);
+/* The engine's branch lengths [n_edges], as clamped. */
+int nrx_get_branch_lengths(nrx_engine *eng, double *brlen);
+
 /* Per-(partition, displayed tree) log-likelihoods of the last forward pass:
  * tree_logl [n_parts][n_classes], class_logprob [n_classes]. */
 int nrx_tree_logls(nrx_engine *eng, double *tree_logl, double *class_logprob);

@@ -92,6 +92,8 @@ static hipGraphExec_t capture_graph(hipStream_t stream, F &&enqueue) {
 Engine::~Engine() {
   if (stream) (void)hipStreamSynchronize(stream);
   destroy_graphs(graph_exec);
+  if (nr_graph) (void)hipGraphExecDestroy(nr_graph);
+  if (h_nr_state) (void)hipHostFree(h_nr_state);
   if (h_result) (void)hipHostFree(h_result);
   if (h_brlen) (void)hipHostFree(h_brlen);
   if (ev_brlen) (void)hipEventDestroy(ev_brlen);
@@ -1484,6 +1486,115 @@ int nrx_brlen_derivatives_noalloc(nrx_engine *eng, int32_t variant, int32_t edge
     }
     const std::vector<double> base_logl(e.h_result + e.rl.tree_logl, e.h_result + e.rl.tree_logl + e.P * nc);
     mix_brlen_derivatives(e, variant, mode, base_logl, cls, path, sums.data(), d1, d2, part_d1, part_d2);
+    return NRX_OK;
+  });
+}
+
+int nrx_get_branch_lengths(nrx_engine *eng, double *brlen) {
+  return guarded([&]() -> int {
+    if (!eng || !brlen) throw std::invalid_argument("null argument");
+    const Engine &e = *static_cast<Engine *>(eng);
+    std::copy(e.brlen.begin(), e.brlen.end(), brlen);
+    return NRX_OK;
+  });
+}
+
+int nrx_optimize_branches(nrx_engine *eng, int32_t variant, int32_t mode, const int32_t *edges, int32_t n_edges,
+                          int32_t max_iters, double *logl) {
+  return guarded([&]() -> int {
+    if (!eng || (n_edges > 0 && !edges)) throw std::invalid_argument("null argument");
+    Engine &e = *static_cast<Engine *>(eng);
+    if (mode != NRX_DERIV_REFERENCE && mode != NRX_DERIV_TRUE)
+      throw std::invalid_argument("unknown derivative mode");
+    if (variant != NRX_AVERAGE_DISPLAYED_TREES && variant != NRX_BEST_DISPLAYED_TREE)
+      throw std::invalid_argument("unknown likelihood variant");
+    if (max_iters < 1 || max_iters > 256) throw std::invalid_argument("max_iters must lie in [1, 256]");
+    if (e.opt.precision == NRX_FP32 && e.S == AA_STATES)
+      return fail(NRX_ERR_UNSUPPORTED, "per-branch sumtables of 20-state data need the FP64 form");
+    std::vector<int> cand(edges, edges + std::max(0, n_edges));
+    for (int ed : cand)
+      if (ed < 0 || ed >= e.E) throw std::invalid_argument("edge index out of range");
+    std::sort(cand.begin(), cand.end());  // optimize_branches takes the candidates in index order
+    cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+    const int nc = static_cast<int>(e.dag.classes.size());
+    // sumtable buffers for every class at once: their addresses then stay
+    // fixed for the captured Newton sequence
+    e.d_st.alloc(sizeof(double) * e.clv_elems_per_slot * std::max(nc, 1));
+    e.d_st_partials.alloc(sizeof(double) * std::max(nc, 1) * 3 * static_cast<size_t>(e.L.n_tiles) * e.pw);
+    e.d_st_sums.alloc(sizeof(double) * std::max(nc, 1) * 3 * e.P);
+    e.d_nr_state.alloc(sizeof(NewtonState));
+    e.d_nr_cls.alloc(sizeof(DerivClassDev) * std::max(nc, 1));
+    e.d_nr_base.alloc(sizeof(double) * std::max(nc, 1) * e.P);
+    if (!e.h_nr_state)
+      NRX_HIP(hipHostMalloc(reinterpret_cast<void **>(&e.h_nr_state), sizeof(NewtonState), hipHostMallocDefault));
+    const void *bufs[3] = {e.d_st.p, e.d_st_partials.p, e.d_st_sums.p};
+    // Newton rounds per graph launch: most branches converge within one
+    // launch; a branch that has not is launched again
+    const int rounds = std::min<int>(max_iters, 6);
+    const int key[4] = {variant, mode, rounds, nc};
+    if (e.nr_graph && (!std::equal(key, key + 4, e.nr_graph_key) || !std::equal(bufs, bufs + 3, e.nr_graph_bufs))) {
+      (void)hipGraphExecDestroy(e.nr_graph);
+      e.nr_graph = nullptr;
+    }
+    int rc = evaluate(e, variant, true, true, false);
+    if (rc != NRX_OK) return rc;
+    const double start = e.cached_total;
+    const double tol = e.opt.brlen_min > 0.0 ? e.opt.brlen_min / 10.0 : 1e-7;
+    std::vector<DerivClassDev> cls(nc);
+    for (int ed : cand) {
+      rc = evaluate(e, variant, true, true, false);
+      if (rc != NRX_OK) return rc;
+      build_sumtables(e, ed);
+      const int np = e.deriv_n_paths;
+      if (np == 0) continue;  // the branch is in no displayed tree's path: derivative 0, length kept
+      std::vector<char> has(nc, 0);
+      std::vector<int> path_of(nc, -1);
+      for (size_t i = 0; i < e.deriv_class.size(); ++i) {
+        has[e.deriv_class[i]] = 1;
+        path_of[e.deriv_class[i]] = e.deriv_path[i];
+      }
+      for (int k = 0; k < nc; ++k) cls[k] = DerivClassDev{e.class_logprob[k], e.class_use[k], has[k], path_of[k], 0};
+      // (the host arrays outlive the copies: the branch ends with a wait)
+      NRX_HIP(hipMemcpyAsync(e.d_nr_cls.p, cls.data(), sizeof(DerivClassDev) * nc, hipMemcpyHostToDevice, e.stream));
+      NRX_HIP(hipMemcpyAsync(e.d_nr_base.p, e.deriv_base_logl.data(), sizeof(double) * e.deriv_base_logl.size(),
+                             hipMemcpyHostToDevice, e.stream));
+      NewtonState &st = *e.h_nr_state;
+      st = NewtonState{};
+      st.lo = e.opt.brlen_min;
+      st.hi = e.opt.brlen_max;
+      st.x = st.x0 = std::min(std::max(e.brlen[ed], st.lo), st.hi);
+      st.a = st.lo;
+      st.b = st.hi;
+      st.tol = tol;
+      st.n_paths = np;
+      st.max_iters = max_iters;
+      NRX_HIP(hipMemcpyAsync(e.d_nr_state.p, e.h_nr_state, sizeof(NewtonState), hipMemcpyHostToDevice, e.stream));
+      if (!e.nr_graph) {
+        e.nr_graph = capture_graph(e.stream, [&]() {
+          enqueue_newton(e, variant, mode, rounds, nc, e.d_nr_state.as<NewtonState>(),
+                         e.d_nr_cls.as<DerivClassDev>(), e.d_nr_base.as<double>());
+        });
+        std::copy(key, key + 4, e.nr_graph_key);
+        std::copy(bufs, bufs + 3, e.nr_graph_bufs);
+      }
+      do {
+        NRX_HIP(hipGraphLaunch(e.nr_graph, e.stream));
+        NRX_HIP(hipMemcpyAsync(e.h_nr_state, e.d_nr_state.p, sizeof(NewtonState), hipMemcpyDeviceToHost,
+                               e.stream));
+        NRX_HIP(hipStreamSynchronize(e.stream));  // one wait per launch (usually one per branch)
+      } while (!st.done);
+      const double x = st.accepted ? st.x : st.x0;
+      if (x != e.brlen[ed]) {
+        e.brlen[ed] = x;
+        e.edge_dirty[ed] = 1;  // the sumtables of this branch stay valid
+      }
+    }
+    rc = evaluate(e, variant, true, true, false);
+    if (rc != NRX_OK) return rc;
+    if (logl) *logl = e.cached_total;
+    // optimize_branches' final check (BranchLengthOptimization.cpp:422-477)
+    if (e.cached_total < start && std::fabs(e.cached_total - start) >= 1e-3)
+      throw std::runtime_error("Overall loglikelihood got worse");
     return NRX_OK;
   });
 }

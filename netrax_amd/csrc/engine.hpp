@@ -123,6 +123,25 @@ struct PathStep {
   int32_t k;
 };
 
+// Device-side Newton-Raphson of one branch length (nrx_optimize_branches):
+// the safeguarded iteration's state, and per displayed-tree class what the
+// derivative mixture needs.
+struct NewtonState {
+  double x, a, b;      // current length and the bracket of the maximiser
+  double x0, lo, hi, tol;
+  double d1, d2;       // last derivatives
+  double lnl_old, lnl_new;
+  int32_t n_paths, done, iters, accepted;
+  int32_t max_iters, pad;
+};
+struct DerivClassDev {
+  double logprob;
+  int32_t use;   // passes min_tree_logprob
+  int32_t has;   // its alive edges contain the branch
+  int32_t path;  // sumtable row, or -1
+  int32_t pad;
+};
+
 struct DevClass {
   double logprob;
   int32_t root_index;
@@ -225,6 +244,14 @@ void launch_mfma_tables(Engine &e);
 void launch_forward_mfma(Engine &e);
 void launch_adjoint_mfma(Engine &e);
 void launch_sumtables(Engine &e, int n_paths);
+// Newton-Raphson of a branch length on the device: `rounds` rounds of
+// (evaluate the sumtables at the state's length, reduce, Newton step; no-ops
+// once the state is done or has taken its max_iters steps), then the
+// evaluation at the current length and the acceptance check.  The number
+// of paths and every length come from the device state, so the sequence is
+// captured once and replayed per branch (and again while not done).
+void enqueue_newton(Engine &e, int variant, int mode, int rounds, int max_paths, NewtonState *st,
+                    const DerivClassDev *cls, const double *base_logl);
 // allocation-free branch derivatives: a fixed device array of this many
 // bytes per GPU holds every scratch buffer of nrx_brlen_derivatives_noalloc
 constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(16) << 20;

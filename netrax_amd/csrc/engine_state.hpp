@@ -98,6 +98,13 @@ struct Engine : nrx_engine {
   int deriv_n_paths = 0;
   std::vector<double> deriv_base_logl;  // [P][nc] tree lnLs when the sumtables were built
   DevBuf d_ops_by_slot, d_steps, d_path_begin, d_st, d_st_partials, d_st_sums;
+  // device Newton-Raphson: state, class table, tree lnLs; its captured
+  // sequence, valid while the sumtable buffers keep their addresses
+  DevBuf d_nr_state, d_nr_cls, d_nr_base;
+  hipGraphExec_t nr_graph = nullptr;
+  int nr_graph_key[4] = {-1, -1, -1, -1};  // variant, mode, max_iters, paths
+  const void *nr_graph_bufs[3] = {nullptr, nullptr, nullptr};
+  NewtonState *h_nr_state = nullptr;  // pinned
   std::vector<double> h_st_sums;
 
   // dirty tracking

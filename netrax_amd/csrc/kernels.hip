@@ -2282,6 +2282,205 @@ __global__ void __launch_bounds__(TILE *C)
   }
 }
 
+// ------------------------------------- Newton-Raphson of a branch on the device
+// The sumtable evaluation and its reduction at the length held in the
+// device state (nothing once the iteration is done, unless `final_eval`),
+// the Newton step and the acceptance check.  The step restates the
+// safeguarded iteration of the host sweep (netrax_amd/optimization.py
+// newton_maximise) in FP64; the mixture of the per-tree derivatives is
+// computePartitionLhData's (LikelihoodDerivatives.cpp:30-188, reference
+// normalisation) or the true derivative.
+template <int S, int C>
+__global__ void __launch_bounds__(TILE *C)
+    k_sumtable_eval_nr(const double *__restrict__ st, DevLayout L, const int32_t *__restrict__ tile_part,
+                       const double *__restrict__ ev, const double *__restrict__ rates,
+                       const double *__restrict__ weights, const NewtonState *__restrict__ ns, int final_eval,
+                       double *__restrict__ partials) {
+  if (!final_eval && ns->done) return;
+  constexpr int PW = TILE * C / 64;
+  const int n_paths = ns->n_paths;
+  const double brlen = ns->x;
+  const int tile = blockIdx.x;
+  const int part = tile_part[tile];
+  const int c = threadIdx.x % C;
+  const int64_t site = static_cast<int64_t>(tile) * TILE + threadIdx.x / C;
+  const int64_t n = L.n_sites_pad;
+  const size_t row = static_cast<size_t>(C * S);
+  const size_t n_wt = static_cast<size_t>(L.n_tiles) * PW;
+  const int wt = tile * PW + threadIdx.x / 64;
+  const double w = weights[site];
+  const double r = rates[part * C + c];
+  double ex[S], lr[S];
+#pragma unroll
+  for (int k = 0; k < S; ++k) {
+    lr[k] = ev[part * S + k] * r;
+    ex[k] = exp(lr[k] * brlen);
+  }
+  for (int j = 0; j < n_paths; ++j) {
+    const double *in = st + (static_cast<size_t>(j) * n + site) * row + c * S;
+    double stv[S], v[3];
+#pragma unroll
+    for (int k = 0; k < S; ++k) stv[k] = in[k];
+    sumtable_partials<S, C>(stv, ex, lr, w, c, v);
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) partials[(static_cast<size_t>(j) * 3 + q) * n_wt + wt] = v[q];
+  }
+}
+
+__global__ void __launch_bounds__(256)
+    k_reduce_segments_nr(const double *__restrict__ in, size_t row_stride, int pw,
+                         const int32_t *__restrict__ part_tile_begin, int P, const NewtonState *__restrict__ ns,
+                         int final_eval, double *__restrict__ out) {
+  if (!final_eval && ns->done) return;
+  if (static_cast<int>(blockIdx.x) >= ns->n_paths * 3 * P) return;
+  __shared__ double sh[4];
+  const int row = blockIdx.x / P, p = blockIdx.x % P;
+  const double *r = in + static_cast<size_t>(row) * row_stride;
+  const int t0 = part_tile_begin[p] * pw, t1 = part_tile_begin[p + 1] * pw;
+  double s = 0.0;
+  for (int t = t0 + threadIdx.x; t < t1; t += 256) s += r[t];
+  s = block_sum<256>(s, sh);
+  if (threadIdx.x == 0) out[static_cast<size_t>(row) * P + p] = s;
+}
+
+// tree lnL of class k in partition p at the evaluated length
+__device__ __forceinline__ double nr_tree_logl(const double *sums, const DerivClassDev &cl, const double *base,
+                                               int P, int nc, int k, int p) {
+  double ll = base[static_cast<size_t>(p) * nc + k];
+  if (cl.has && cl.path >= 0) ll += sums[(static_cast<size_t>(cl.path) * 3 + 0) * P + p];
+  return ll;
+}
+
+__global__ void k_newton_step(const double *__restrict__ sums, int P, int nc, const DerivClassDev *__restrict__ cls,
+                              const double *__restrict__ base, int variant, int mode, NewtonState *__restrict__ ns) {
+  if (threadIdx.x != 0 || ns->done) return;
+  // derivatives at ns->x, mixed per partition as the host does
+  double D1 = 0.0, D2 = 0.0;
+  for (int p = 0; p < P; ++p) {
+    int n_members = 0, only = -1;
+    for (int k = 0; k < nc; ++k)
+      if (cls[k].use && (mode == NRX_DERIV_TRUE || cls[k].has)) {
+        ++n_members;
+        only = k;
+      }
+    auto l1 = [&](int k) {
+      return cls[k].has && cls[k].path >= 0 ? sums[(static_cast<size_t>(cls[k].path) * 3 + 1) * P + p] : 0.0;
+    };
+    auto l2 = [&](int k) {
+      return cls[k].has && cls[k].path >= 0 ? sums[(static_cast<size_t>(cls[k].path) * 3 + 2) * P + p] : 0.0;
+    };
+    double pd1 = 0.0, pd2 = 0.0;
+    if (n_members == 1 && mode == NRX_DERIV_REFERENCE) {
+      pd1 = l1(only);
+      pd2 = l2(only);
+    } else if (n_members > 0) {
+      if (variant == NRX_AVERAGE_DISPLAYED_TREES) {
+        double mx = -INFINITY;
+        for (int k = 0; k < nc; ++k)
+          if (cls[k].use && (mode == NRX_DERIV_TRUE || cls[k].has))
+            mx = fmax(mx, cls[k].logprob + nr_tree_logl(sums, cls[k], base, P, nc, k, p));
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        for (int k = 0; k < nc; ++k) {
+          if (!(cls[k].use && (mode == NRX_DERIV_TRUE || cls[k].has))) continue;
+          const double wgt = exp(cls[k].logprob + nr_tree_logl(sums, cls[k], base, P, nc, k, p) - mx);
+          const double a1 = l1(k);
+          s0 += wgt;
+          s1 += wgt * a1;
+          s2 += wgt * (a1 * a1 + l2(k));
+        }
+        pd1 = s1 / s0;
+        pd2 = (s2 * s0 - s1 * s1) / (s0 * s0);
+      } else {
+        double best = -INFINITY;
+        for (int k = 0; k < nc; ++k) {
+          if (!(cls[k].use && (mode == NRX_DERIV_TRUE || cls[k].has))) continue;
+          const double ll = nr_tree_logl(sums, cls[k], base, P, nc, k, p);
+          // reference mode: the tree maximising tree_logl * tree_prob
+          // (LikelihoodDerivatives.cpp:161-167)
+          const double score = mode == NRX_DERIV_REFERENCE ? ll * exp(cls[k].logprob) : cls[k].logprob + ll;
+          if (score > best) {
+            best = score;
+            pd1 = l1(k);
+            pd2 = l2(k);
+          }
+        }
+      }
+    }
+    D1 += pd1;
+    D2 += pd2;
+  }
+  ns->d1 = D1;
+  ns->d2 = D2;
+  ns->iters += 1;
+  if (ns->iters >= ns->max_iters) ns->done = 1;  // this step is the last
+  // safeguarded Newton step: keep the maximiser's bracket [a, b], take the
+  // Newton step when the curvature is negative and the step stays inside,
+  // else grow geometrically toward a far bound or bisect
+  const double x = ns->x, tol = ns->tol;
+  double a = ns->a, b = ns->b;
+  if (D1 > 0.0) {
+    a = x;
+  } else if (D1 < 0.0) {
+    b = x;
+  } else {
+    ns->done = 1;
+    return;
+  }
+  double nx = 0.0;
+  bool ok = false;
+  if (D2 < 0.0) {
+    nx = x - D1 / D2;
+    ok = a < nx && nx < b;
+  }
+  if (!ok) {
+    if (D1 > 0.0 && b - x > 4.0 * (x + tol))
+      nx = fmin(b, 2.0 * x + tol);
+    else if (D1 < 0.0 && x - a > 4.0 * tol && x > 4.0 * a)
+      nx = fmax(a, 0.5 * x);
+    else
+      nx = 0.5 * (a + b);
+  }
+  ns->a = a;
+  ns->b = b;
+  if (fabs(nx - x) < tol) ns->done = 1;
+  ns->x = fmin(fmax(nx, ns->lo), ns->hi);
+}
+
+// network lnL at the final length vs at the start (both from the same tree
+// lnLs); a worse length is given up (optimize_branch,
+// BranchLengthOptimization.cpp:317-330)
+__global__ void k_newton_accept(const double *__restrict__ sums, int P, int nc,
+                                const DerivClassDev *__restrict__ cls, const double *__restrict__ base,
+                                int variant, NewtonState *__restrict__ ns) {
+  if (threadIdx.x != 0) return;
+  double l_new = 0.0, l_old = 0.0;
+  for (int p = 0; p < P; ++p) {
+    double mn = -INFINITY, mo = -INFINITY;
+    for (int k = 0; k < nc; ++k) {
+      if (!cls[k].use) continue;
+      mn = fmax(mn, cls[k].logprob + nr_tree_logl(sums, cls[k], base, P, nc, k, p));
+      mo = fmax(mo, cls[k].logprob + base[static_cast<size_t>(p) * nc + k]);
+    }
+    if (variant == NRX_BEST_DISPLAYED_TREE) {
+      l_new += mn;
+      l_old += mo;
+      continue;
+    }
+    double sn = 0.0, so = 0.0;
+    for (int k = 0; k < nc; ++k) {
+      if (!cls[k].use) continue;
+      sn += exp(cls[k].logprob + nr_tree_logl(sums, cls[k], base, P, nc, k, p) - mn);
+      so += exp(cls[k].logprob + base[static_cast<size_t>(p) * nc + k] - mo);
+    }
+    l_new += mn + log(sn);
+    l_old += mo + log(so);
+  }
+  ns->lnl_new = l_new;
+  ns->lnl_old = l_old;
+  ns->accepted = l_new >= l_old;
+}
+
 // The two kernels above in one, with no sumtable stored: each (path, site,
 // category) walks its path adjoint and evaluates at b right away (the
 // allocation-free variant of the branch derivatives re-walks the paths on
@@ -2783,6 +2982,36 @@ void launch_deriv_fused(Engine &e, const PathStep *steps, const int32_t *path_be
   NRX_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_reduce_segments, dim3(n_paths * 3 * e.P), dim3(256), 0, e.stream, partials,
                      static_cast<size_t>(e.L.n_tiles) * e.pw, e.pw, e.d_part_tile_begin.as<int32_t>(), e.P, sums);
+  NRX_HIP(hipGetLastError());
+}
+
+void enqueue_newton(Engine &e, int variant, int mode, int rounds, int max_paths, NewtonState *st,
+                    const DerivClassDev *cls, const double *base_logl) {
+  const int nc = static_cast<int>(e.dag.classes.size());
+  const size_t row_stride = static_cast<size_t>(e.L.n_tiles) * e.pw;
+  auto eval_reduce = [&](int final_eval) {
+    dispatch_sc(e.S, e.C, [&](auto s, auto c) {
+      constexpr int S = decltype(s)::value, C = decltype(c)::value;
+      hipLaunchKernelGGL((k_sumtable_eval_nr<S, C>), dim3(e.L.n_tiles), dim3(TILE * C), 0, e.stream,
+                         e.d_st.as<double>(), e.L, e.d_tile_part.as<int32_t>(), e.d_eval.as<double>(),
+                         e.d_rates.as<double>(), e.d_weights.as<double>(), st, final_eval,
+                         e.d_st_partials.as<double>());
+    });
+    NRX_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_reduce_segments_nr, dim3(max_paths * 3 * e.P), dim3(256), 0, e.stream,
+                       e.d_st_partials.as<double>(), row_stride, e.pw, e.d_part_tile_begin.as<int32_t>(), e.P, st,
+                       final_eval, e.d_st_sums.as<double>());
+    NRX_HIP(hipGetLastError());
+  };
+  for (int it = 0; it < rounds; ++it) {
+    eval_reduce(0);
+    hipLaunchKernelGGL(k_newton_step, dim3(1), dim3(64), 0, e.stream, e.d_st_sums.as<double>(), e.P, nc, cls,
+                       base_logl, variant, mode, st);
+    NRX_HIP(hipGetLastError());
+  }
+  eval_reduce(1);
+  hipLaunchKernelGGL(k_newton_accept, dim3(1), dim3(64), 0, e.stream, e.d_st_sums.as<double>(), e.P, nc, cls,
+                     base_logl, variant, st);
   NRX_HIP(hipGetLastError());
 }
 

@@ -197,6 +197,31 @@ class AnnotatedNetwork:
                                                   C.byref(d1), C.byref(d2), _dptr(p1), _dptr(p2)))
         return LoglDerivatives(d1.value, d2.value, list(p1), list(p2))
 
+    def branch_lengths(self) -> np.ndarray:
+        """The engine's (clamped) branch lengths."""
+        b = np.zeros(self.network.n_edges)
+        _lib.check(self.lib.nrx_get_branch_lengths(self.handle, _dptr(b)))
+        return b
+
+    def optimize_branches(self, edges=None, max_iters: int = 32, mode: int = _lib.NRX_DERIV_REFERENCE,
+                          variant: Optional[int] = None) -> float:
+        """Newton-Raphson sweep over `edges` (all by default) with the
+        iteration on the device (nrx_optimize_branches); returns the final
+        lnL and updates ``network.brlen``."""
+        v = int(self.options.likelihood_variant if variant is None else variant)
+        ed = np.ascontiguousarray(range(self.network.n_edges) if edges is None else list(edges), np.int32)
+        out = C.c_double()
+        rc = self.lib.nrx_optimize_branches(self.handle, v, int(mode), ed.ctypes.data_as(C.POINTER(C.c_int32)),
+                                            len(ed), int(max_iters), C.byref(out))
+        if rc == _lib.NRX_ERR_NEGINF:
+            raise RuntimeError("Invalid network likelihood: negative infinity \n")
+        if rc == _lib.NRX_ERR_STATE:
+            raise RuntimeError(self.lib.nrx_last_error().decode())
+        _lib.check(rc)
+        self.network.brlen = self.branch_lengths()
+        self.cached_logl = out.value
+        return out.value
+
     def tree_logls(self):
         """([P][classes] per-tree lnL of the last pass, [classes] log P(t))."""
         st = self.stats()

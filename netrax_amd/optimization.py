@@ -105,6 +105,14 @@ def optimize_branches(ann_network: AnnotatedNetwork, max_iters: int = RAXML_BRLE
     return logl
 
 
+def optimize_branches_device(ann_network: AnnotatedNetwork, max_iters: int = RAXML_BRLEN_SMOOTHINGS,
+                             candidates: Optional[Iterable[int]] = None,
+                             mode: int = _lib.NRX_DERIV_REFERENCE) -> float:
+    """optimize_branches with each branch's Newton iteration, bracket and
+    acceptance check on the device (one host wait per branch)."""
+    return ann_network.optimize_branches(candidates, max_iters, mode)
+
+
 def optimizeBranches(ann_network: AnnotatedNetwork, brlen_smooth_factor: float = 1.0) -> float:
     """src/optimization/Optimization.cpp:17-37 without the BIC bookkeeping."""
     return optimize_branches(ann_network, int(brlen_smooth_factor * RAXML_BRLEN_SMOOTHINGS))

@@ -212,3 +212,41 @@ def test_brlen_derivatives_clamp_lengths_below_the_minimum(prec):
         dmax = ann.brlen_derivatives(e, brlen=100.0, mode=1)
         assert dh.logl_prime == dmax.logl_prime
     ann.close()
+
+
+@pytest.mark.parametrize("name,prec", [("50taxa_50k", Precision.FP64), ("50taxa_50k", Precision.FP32),
+                                       ("two_reticulations", Precision.FP64)])
+def test_device_newton_sweep_matches_host_sweep(name, prec):
+    """nrx_optimize_branches (Newton iteration, bracket and acceptance on
+    the device) gives the lengths and lnL of the host-driven sweep
+    (optimize_branches) for both variants and derivative modes."""
+    from netrax_amd.optimization import optimize_branches
+    from netrax_amd.simulate import workload_50taxa_50k
+    if name == "two_reticulations":
+        net0, parts = load_sample(name, fixed_model())
+        edges = list(range(net0.n_edges))
+        cases = [(0, 0), (0, 1), (1, 0), (1, 1)]
+    else:
+        w = workload_50taxa_50k()
+        net0, parts = w.network, w.partitions
+        edges = sorted(np.random.default_rng(8).choice(net0.n_edges, 10, replace=False).tolist())
+        cases = [(0, 0)] if prec == Precision.FP32 else [(0, 0), (1, 1)]
+    rng = np.random.default_rng(12)
+    start = net0.brlen * rng.uniform(0.5, 2.0, net0.n_edges)
+    for variant, mode in cases:
+        say("device sweep", name, prec.name, "variant", variant, "mode", mode)
+        nh, nd = net0.copy(), net0.copy()
+        nh.brlen = start.copy()
+        nd.brlen = start.copy()
+        host = eng(nh, parts, variant, precision=prec)
+        dev = eng(nd, parts, variant, precision=prec)
+        before = computeLoglikelihood(host)
+        lh = optimize_branches(host, candidates=edges, mode=mode)
+        ld = dev.optimize_branches(edges, mode=mode)
+        assert ld >= before - 1e-6 * abs(before)
+        assert np.allclose(dev.network.brlen, host.network.brlen, rtol=1e-6, atol=1e-9), \
+            np.abs(dev.network.brlen - host.network.brlen).max()
+        assert abs(ld - lh) <= 1e-9 * abs(lh), (ld, lh)
+        assert np.array_equal(dev.branch_lengths(), dev.network.brlen)
+        host.close()
+        dev.close()
