@@ -194,6 +194,23 @@ int nrx_brlen_derivatives_noalloc(nrx_engine *eng, int32_t variant, int32_t edge
                                   int32_t mode, double brlen, double *d1, double *d2,
                                   double *part_d1, double *part_d2);
 
+/* lnL and its gradient with respect to the substitution model of every
+ * partition: d_subst [n_parts][S(S-1)/2] (exchangeabilities, upper
+ * triangle row-major as nrx_set_model takes them), d_freqs [n_parts][S]
+ * (each frequency as an independent argument of nrx_set_model, through the
+ * rate matrix and the root), d_alpha [n_parts] (Gamma shape; NaN after
+ * nrx_set_category_rates) and d_rates [n_parts][C] (the category rates);
+ * d_alpha and d_rates may be NULL.  No single reference function: replaces
+ * the numerical model optimisation raxml-ng runs through the pll-modules
+ * hook (src/optimization/ModelOptimization.cpp:25-97, likelihood target
+ * network_logl_wrapper, src/RaxmlWrapper.cpp:21-26, 647-648) with analytic
+ * derivatives.  One forward and one adjoint pass, then a pass that forms
+ * dlnL/dP per (edge, category) from the adjoints, chained through the
+ * eigendecomposition of the rate matrix on the host.  FP64 form, 4-state
+ * data (NRX_ERR_UNSUPPORTED otherwise). */
+int nrx_model_gradient(nrx_engine *eng, int32_t variant, double *total, double *d_subst,
+                       double *d_freqs, double *d_alpha, double *d_rates);
+
 /* Newton-Raphson branch-length sweep with the iteration on the device.
  * Replaces optimize_branches / optimize_branch /
  * optimize_branch_newton_raphson (src/optimization/BranchLengthOptimization.cpp:

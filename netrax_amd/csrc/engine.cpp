@@ -1302,6 +1302,8 @@ int nrx_set_model(nrx_engine *eng, int32_t partition, const double *subst_rates,
     PartInfo &pi = e.parts[partition];
     build_model(e.S, subst_rates, freqs, pi.model);
     pi.rates = gamma_mean_rates(alpha, e.C);
+    pi.subst.assign(subst_rates, subst_rates + e.S * (e.S - 1) / 2);
+    pi.alpha = alpha;
     pi.model_set = true;
     e.model_dirty = true;
     e.deriv_valid = false;
@@ -1319,6 +1321,7 @@ int nrx_set_category_rates(nrx_engine *eng, int32_t partition, const double *rat
     for (int c = 0; c < e.C; ++c)
       if (!(rates[c] >= 0.0)) throw std::invalid_argument("category rates must be >= 0");
     pi.rates.assign(rates, rates + e.C);
+    pi.alpha = std::nan("");
     e.model_dirty = true;
     e.deriv_valid = false;
     return NRX_OK;
@@ -1595,6 +1598,168 @@ int nrx_optimize_branches(nrx_engine *eng, int32_t variant, int32_t mode, const 
     // optimize_branches' final check (BranchLengthOptimization.cpp:422-477)
     if (e.cached_total < start && std::fabs(e.cached_total - start) >= 1e-3)
       throw std::runtime_error("Overall loglikelihood got worse");
+    return NRX_OK;
+  });
+}
+
+int nrx_model_gradient(nrx_engine *eng, int32_t variant, double *total, double *d_subst, double *d_freqs,
+                       double *d_alpha, double *d_rates) {
+  return guarded([&]() -> int {
+    if (!eng || !total || !d_subst || !d_freqs) throw std::invalid_argument("null argument");
+    Engine &e = *static_cast<Engine *>(eng);
+    if (e.opt.precision != NRX_FP64 || e.S != 4)
+      return fail(NRX_ERR_UNSUPPORTED, "model gradients are provided for the FP64 form with 4-state data");
+    const int rc = evaluate(e, variant, true, true, true);  // CLVs, adjoints, mixture weights
+    if (rc != NRX_OK) return rc;
+    *total = e.cached_total;
+    const int S = e.S, C = e.C, P = e.P, nr = S * (S - 1) / 2;
+    const int n_ops = static_cast<int>(e.ops_all.size());
+    const int n_root = static_cast<int>(e.root_slots.size());
+    // slices per partition so that the grid fills the GPU
+    int n_split = std::max(1, (2048 + n_ops * P - 1) / std::max(1, n_ops * P));
+    for (const PartInfo &pi : e.parts) n_split = std::min(n_split, std::max(1, pi.tile_end - pi.tile_begin));
+    const int nch = P * n_split;
+    const size_t ng = static_cast<size_t>(std::max(1, e.n_uses)) * nch * C * S * S;
+    const size_t nrr = static_cast<size_t>(std::max(1, n_root)) * nch * S;
+    e.d_mg_g.alloc(sizeof(double) * ng);
+    e.d_mg_r.alloc(sizeof(double) * nrr);
+    launch_model_grad(e, n_split, e.d_mg_g.as<double>(), e.d_mg_r.as<double>());
+    std::vector<double> G(ng), Rr(nrr);
+    NRX_HIP(hipMemcpyAsync(G.data(), e.d_mg_g.p, sizeof(double) * ng, hipMemcpyDeviceToHost, e.stream));
+    NRX_HIP(hipMemcpyAsync(Rr.data(), e.d_mg_r.p, sizeof(double) * nrr, hipMemcpyDeviceToHost, e.stream));
+    NRX_HIP(hipStreamSynchronize(e.stream));
+    for (int p = 0; p < P; ++p) {
+      const PartInfo &pi = e.parts[p];
+      const EigenModel &m = pi.model;
+      const double *pf = m.freqs.data();
+      // Q and its unnormalised form R (Q = R / mu), as build_model forms them
+      std::vector<double> R(S * S, 0.0);
+      for (int i = 0, k = 0; i < S; ++i)
+        for (int j = i + 1; j < S; ++j, ++k) {
+          R[i * S + j] = pi.subst[k] * pf[j];
+          R[j * S + i] = pi.subst[k] * pf[i];
+        }
+      double mu = 0.0;
+      for (int i = 0; i < S; ++i) {
+        double rowsum = 0.0;
+        for (int j = 0; j < S; ++j)
+          if (j != i) rowsum += R[i * S + j];
+        R[i * S + i] = -rowsum;
+        mu += pf[i] * rowsum;
+      }
+      std::vector<double> Q(S * S);
+      for (int q = 0; q < S * S; ++q) Q[q] = R[q] / mu;
+      // dJ/dQ through P_{e,c} = exp(Q r_c t_e) = V exp(L s) V^-1:
+      // dJ/dQ = V^-T ((V^T G V^-T) o Psi) V^T, Psi_kl = (e^{l_k s} - e^{l_l s}) / (l_k - l_l)
+      // (s e^{l_k s} for equal eigenvalues); dJ/dr_c = sum_e t_e tr(G^T Q P)
+      std::vector<double> dQ(S * S, 0.0), dr(C, 0.0), Gm(S * S), H(S * S), T1(S * S), Ps(S * S), QP(S * S);
+      for (int ed = 0; ed < e.E; ++ed) {
+        for (int c = 0; c < C; ++c) {
+          std::fill(Gm.begin(), Gm.end(), 0.0);
+          for (int i = e.edge_use_begin[ed]; i < e.edge_use_begin[ed + 1]; ++i) {
+            const int u = e.edge_use_list[i];
+            for (int sl = 0; sl < n_split; ++sl) {
+              const double *g = &G[((static_cast<size_t>(u) * nch + p * n_split + sl) * C + c) * S * S];
+              for (int q = 0; q < S * S; ++q) Gm[q] += g[q];
+            }
+          }
+          const double s = pi.rates[c] * e.brlen[ed];
+          // H = V^T G Vinv^T
+          for (int a = 0; a < S; ++a)
+            for (int j = 0; j < S; ++j) {
+              double acc = 0.0;
+              for (int i = 0; i < S; ++i) acc += m.V[i * S + a] * Gm[i * S + j];
+              T1[a * S + j] = acc;
+            }
+          for (int a = 0; a < S; ++a)
+            for (int b = 0; b < S; ++b) {
+              double acc = 0.0;
+              for (int j = 0; j < S; ++j) acc += T1[a * S + j] * m.Vinv[b * S + j];
+              H[a * S + b] = acc;
+            }
+          for (int a = 0; a < S; ++a)
+            for (int b = 0; b < S; ++b) {
+              const double la = m.eval[a], lb = m.eval[b];
+              const double psi = std::fabs(la - lb) > 1e-10 * std::max(1.0, std::fabs(la))
+                                     ? (std::exp(la * s) - std::exp(lb * s)) / (la - lb)
+                                     : s * std::exp(0.5 * (la + lb) * s);
+              H[a * S + b] *= psi;
+            }
+          // dQ += Vinv^T H V^T
+          for (int i = 0; i < S; ++i)
+            for (int b = 0; b < S; ++b) {
+              double acc = 0.0;
+              for (int a = 0; a < S; ++a) acc += m.Vinv[a * S + i] * H[a * S + b];
+              T1[i * S + b] = acc;
+            }
+          for (int i = 0; i < S; ++i)
+            for (int j = 0; j < S; ++j) {
+              double acc = 0.0;
+              for (int b = 0; b < S; ++b) acc += T1[i * S + b] * m.V[j * S + b];
+              dQ[i * S + j] += acc;
+            }
+          // tr(G^T Q P) with P = V exp(L s) Vinv
+          for (int i = 0; i < S; ++i)
+            for (int j = 0; j < S; ++j) {
+              double acc = 0.0;
+              for (int k = 0; k < S; ++k) acc += m.V[i * S + k] * std::exp(m.eval[k] * s) * m.Vinv[k * S + j];
+              Ps[i * S + j] = acc;
+            }
+          double tr = 0.0;
+          for (int i = 0; i < S; ++i)
+            for (int j = 0; j < S; ++j) {
+              double acc = 0.0;
+              for (int k = 0; k < S; ++k) acc += Q[i * S + k] * Ps[k * S + j];
+              tr += Gm[i * S + j] * acc;
+            }
+          dr[c] += e.brlen[ed] * tr;
+        }
+      }
+      // chain to exchangeabilities and frequencies: Q = R / mu,
+      // R_ij = rho_ij pi_j (i != j), R_ii = -sum_{j != i} R_ij,
+      // mu = sum_i pi_i sum_{j != i} rho_ij pi_j
+      double dmu = 0.0;
+      for (int q = 0; q < S * S; ++q) dmu -= dQ[q] * R[q] / (mu * mu);
+      std::vector<double> D(S * S, 0.0);  // dJ/dR_ij (i != j) with the diagonal's dependence folded in
+      for (int i = 0; i < S; ++i)
+        for (int j = 0; j < S; ++j)
+          if (i != j) D[i * S + j] = (dQ[i * S + j] - dQ[i * S + i]) / mu;
+      for (int i = 0, k = 0; i < S; ++i)
+        for (int j = i + 1; j < S; ++j, ++k)
+          d_subst[static_cast<size_t>(p) * nr + k] =
+              D[i * S + j] * pf[j] + D[j * S + i] * pf[i] + dmu * 2.0 * pf[i] * pf[j];
+      for (int kq = 0; kq < S; ++kq) {
+        double g = 0.0, dmu_dpi = 0.0;
+        for (int i = 0; i < S; ++i) {
+          if (i == kq) continue;
+          const double rho = pi.subst[i < kq ? (i * (2 * S - i - 1)) / 2 + (kq - i - 1)
+                                             : (kq * (2 * S - kq - 1)) / 2 + (i - kq - 1)];
+          g += D[i * S + kq] * rho;
+          dmu_dpi += 2.0 * rho * pf[i];
+        }
+        g += dmu * dmu_dpi;
+        // the frequencies' direct share at the displayed-tree roots
+        for (int r = 0; r < n_root; ++r)
+          for (int sl = 0; sl < n_split; ++sl)
+            g += Rr[(static_cast<size_t>(r) * nch + p * n_split + sl) * S + kq];
+        d_freqs[static_cast<size_t>(p) * S + kq] = g;
+      }
+      if (d_rates)
+        for (int c = 0; c < C; ++c) d_rates[static_cast<size_t>(p) * C + c] = dr[c];
+      if (d_alpha) {
+        // dlnL/dalpha = sum_c dJ/dr_c dr_c/dalpha; the mean-rate function's
+        // derivative by a central difference (it is smooth in alpha)
+        double ga = 0.0;
+        if (C > 1 && std::isfinite(pi.alpha)) {
+          const double h = 1e-5 * pi.alpha;
+          const std::vector<double> up = gamma_mean_rates(pi.alpha + h, C), dn = gamma_mean_rates(pi.alpha - h, C);
+          for (int c = 0; c < C; ++c) ga += dr[c] * (up[c] - dn[c]) / (2.0 * h);
+        } else if (C > 1) {
+          ga = std::nan("");  // rates were set explicitly: no alpha
+        }
+        d_alpha[p] = ga;
+      }
+    }
     return NRX_OK;
   });
 }

@@ -244,6 +244,10 @@ void launch_mfma_tables(Engine &e);
 void launch_forward_mfma(Engine &e);
 void launch_adjoint_mfma(Engine &e);
 void launch_sumtables(Engine &e, int n_paths);
+// FP64 4-state: per use, chunk and category the summed z x^T (dlnL/dP);
+// per root CLV and chunk the frequencies' direct share (a chunk is one of
+// n_split slices of a partition's tiles, chunk = partition * n_split + slice)
+void launch_model_grad(Engine &e, int n_split, double *g_out, double *r_out);
 // Newton-Raphson of a branch length on the device: `rounds` rounds of
 // (evaluate the sumtables at the state's length, reduce, Newton step; no-ops
 // once the state is done or has taken its max_iters steps), then the

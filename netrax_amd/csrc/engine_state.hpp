@@ -43,6 +43,8 @@ struct PartInfo {
   bool model_set = false;
   EigenModel model;
   std::vector<double> rates;  // [C]
+  std::vector<double> subst;  // exchangeabilities as given
+  double alpha = 0.0;         // Gamma shape of the rates; NaN after nrx_set_category_rates
 };
 
 // Result block offsets (doubles) inside the device/host result buffers.
@@ -101,6 +103,7 @@ struct Engine : nrx_engine {
   // device Newton-Raphson: state, class table, tree lnLs; its captured
   // sequence, valid while the sumtable buffers keep their addresses
   DevBuf d_nr_state, d_nr_cls, d_nr_base;
+  DevBuf d_mg_g, d_mg_r;  // model gradient partials
   hipGraphExec_t nr_graph = nullptr;
   int nr_graph_key[4] = {-1, -1, -1, -1};  // variant, mode, max_iters, paths
   const void *nr_graph_bufs[3] = {nullptr, nullptr, nullptr};

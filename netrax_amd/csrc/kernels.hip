@@ -803,6 +803,140 @@ __global__ void __launch_bounds__(TILE *C, S == 4 ? 3 : 1)
   }
 }
 
+// ---------------------------------------------- model gradient (FP64)
+// dlnL / dP_{e,c} for every (op, child) use: G = sum over sites of z (x)^T
+// with z the adjoint of y = P x (recomputed from the op's stored adjoint,
+// its children and the displayed-tree seed, exactly as k_adjoint_level
+// forms it) and x the child's CLV; and, at displayed-tree roots, the direct
+// dependence of lnL on the frequencies, sum_s beta w (1/C) x_i / Ltilde.
+// Grid (op, chunk): a chunk is a 1/n_split slice of one partition's tiles,
+// so every partial belongs to one partition; a workgroup walks its tiles
+// and reduces its threads' sums once, in a fixed order.
+template <int S, int C>
+__global__ void __launch_bounds__(TILE *C)
+    k_model_grad(const DevOp *__restrict__ ops, DevLayout L, const double *__restrict__ clv,
+                 const int32_t *__restrict__ scl, const double *__restrict__ adj,
+                 const typename TipType<S>::type *__restrict__ tips, const double *__restrict__ P,
+                 const double *__restrict__ freqs, const double *__restrict__ weights,
+                 const double *__restrict__ beta, int n_root, const int32_t *__restrict__ part_tile_begin,
+                 int n_split, int n_uses, double *__restrict__ g_out, double *__restrict__ r_out) {
+  __shared__ double red[TILE * C / 64][2 * S * S + S];
+  const DevOp op = ops[blockIdx.x];
+  const int chunk = blockIdx.y;
+  const int part = chunk / n_split, slice = chunk % n_split;
+  const int tb = part_tile_begin[part], te = part_tile_begin[part + 1];
+  const int t0 = tb + (te - tb) * slice / n_split, t1 = tb + (te - tb) * (slice + 1) / n_split;
+  const int c = threadIdx.x % C;
+  const int64_t n = L.n_sites_pad;
+  const size_t row = static_cast<size_t>(C * S);
+  const double *Pk[2];
+#pragma unroll
+  for (int k = 0; k < 2; ++k)
+    Pk[k] = P + ((static_cast<size_t>(part) * L.n_edges + (k < op.nk ? op.edge[k] : 0)) * C + c) * (S * S);
+  double g[2][S * S], rr[S];
+#pragma unroll
+  for (int q = 0; q < S * S; ++q) g[0][q] = g[1][q] = 0.0;
+#pragma unroll
+  for (int i = 0; i < S; ++i) rr[i] = 0.0;
+  const double bt = op.root >= 0 ? beta[part * n_root + op.root] : 0.0;
+  for (int tile = t0; tile < t1; ++tile) {
+    const int64_t site = static_cast<int64_t>(tile) * TILE + threadIdx.x / C;
+    const double w = weights[site];
+    double a[S], x[2][S], y[2][S];
+#pragma unroll
+    for (int i = 0; i < S; ++i) a[i] = 0.0;
+    if (op.flags & OP_HAS_CONSUMERS) load_clv<S>(adj + (static_cast<size_t>(op.out) * n + site) * row + c * S, a);
+    int ks = 0;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      if (k >= op.nk) continue;
+      const int32_t kid = op.kid[k];
+      if (kid >= 0) {
+        load_clv<S>(clv + (static_cast<size_t>(kid) * n + site) * row + c * S, x[k]);
+        ks += scl[static_cast<size_t>(kid) * n + site];
+      } else {
+        tip_vec<S>(tips[static_cast<size_t>(-kid - 1) * n + site], x[k]);
+      }
+      matvec<S>(Pk[k], x[k], y[k]);
+    }
+    if (op.root >= 0) {
+      // seed beta w (1/C) pi_i / Ltilde of a displayed-tree root, and the
+      // frequencies' direct share beta w (1/C) x_i / Ltilde
+      double xr[S];
+      load_clv<S>(clv + (static_cast<size_t>(op.out) * n + site) * row + c * S, xr);
+      double t = 0.0;
+#pragma unroll
+      for (int i = 0; i < S; ++i) t = fma(freqs[part * S + i], xr[i], t);
+      t = cat_sum<C>(t) * (1.0 / C);
+      const double f = w != 0.0 ? bt * w * (1.0 / C) / t : 0.0;
+#pragma unroll
+      for (int i = 0; i < S; ++i) {
+        a[i] = fma(f, freqs[part * S + i], a[i]);
+        rr[i] = fma(f, xr[i], rr[i]);
+      }
+    }
+    const double F = (scl[static_cast<size_t>(op.out) * n + site] - ks) > 0 ? SCALE_FACTOR : 1.0;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      if (k >= op.nk) continue;
+#pragma unroll
+      for (int i = 0; i < S; ++i) {
+        const double z = a[i] * F * (op.nk == 2 ? y[1 - k][i] : 1.0);
+#pragma unroll
+        for (int j = 0; j < S; ++j) g[k][i * S + j] = fma(z, x[k][j], g[k][i * S + j]);
+      }
+    }
+  }
+  // sums over the sites of each category: lanes of one category are C apart
+  const int wave = threadIdx.x / 64, lane = threadIdx.x & 63;
+  auto site_sum = [&](double v) {
+#pragma unroll
+    for (int off = C; off < 64; off <<= 1) v += __shfl_xor(v, off);
+    return v;
+  };
+#pragma unroll
+  for (int k = 0; k < 2; ++k)
+#pragma unroll
+    for (int q = 0; q < S * S; ++q) g[k][q] = site_sum(g[k][q]);
+  // lane c (< C) of every wave now holds its category's wave sums; combine
+  // the waves through LDS in wave order, one category at a time
+  for (int cc = 0; cc < C; ++cc) {
+    if (lane == cc) {
+#pragma unroll
+      for (int k = 0; k < 2; ++k)
+#pragma unroll
+        for (int q = 0; q < S * S; ++q) red[wave][k * S * S + q] = g[k][q];
+    }
+    __syncthreads();
+    if (threadIdx.x < 2 * S * S) {
+      const int k = threadIdx.x / (S * S), q = threadIdx.x % (S * S);
+      double sum = 0.0;
+      for (int wv = 0; wv < TILE * C / 64; ++wv) sum += red[wv][threadIdx.x];
+      if (k < op.nk)
+        g_out[((static_cast<size_t>(op.gslot[k]) * gridDim.y + chunk) * C + cc) * (S * S) + q] = sum;
+    }
+    __syncthreads();
+  }
+  if (op.root >= 0) {
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+      double v = rr[i];
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) v += __shfl_xor(v, off);
+      rr[i] = v;
+    }
+    if (lane == 0)
+#pragma unroll
+      for (int i = 0; i < S; ++i) red[wave][2 * S * S + i] = rr[i];
+    __syncthreads();
+    if (threadIdx.x < S) {
+      double sum = 0.0;
+      for (int wv = 0; wv < TILE * C / 64; ++wv) sum += red[wv][2 * S * S + threadIdx.x];
+      r_out[(static_cast<size_t>(op.root) * gridDim.y + chunk) * S + threadIdx.x] = sum;
+    }
+  }
+}
+
 // ------------------------------------------------------- reductions
 // out[row][p] = sum of in[row][t] over the partials t of partition p
 // (partials_per_tile per tile, tiles of a partition are contiguous).  One
@@ -2982,6 +3116,20 @@ void launch_deriv_fused(Engine &e, const PathStep *steps, const int32_t *path_be
   NRX_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_reduce_segments, dim3(n_paths * 3 * e.P), dim3(256), 0, e.stream, partials,
                      static_cast<size_t>(e.L.n_tiles) * e.pw, e.pw, e.d_part_tile_begin.as<int32_t>(), e.P, sums);
+  NRX_HIP(hipGetLastError());
+}
+
+void launch_model_grad(Engine &e, int n_split, double *g_out, double *r_out) {
+  if (e.S != 4) throw std::invalid_argument("model gradients are built for 4-state data");
+  dispatch_c(e.C, [&](auto c) {
+    constexpr int C = decltype(c)::value;
+    hipLaunchKernelGGL((k_model_grad<4, C>), dim3(static_cast<unsigned>(e.ops_all.size()), e.P * n_split),
+                       dim3(TILE * C), 0, e.stream, e.d_ops.as<DevOp>(), e.L, e.d_clv.as<double>(),
+                       e.d_scl.as<int32_t>(), e.d_adj.as<double>(), e.d_tips.as<uint8_t>(), e.d_P.as<double>(),
+                       e.d_freqs.as<double>(), e.d_weights.as<double>(), e.d_beta.as<double>(),
+                       static_cast<int>(e.root_slots.size()), e.d_part_tile_begin.as<int32_t>(), n_split, e.n_uses,
+                       g_out, r_out);
+  });
   NRX_HIP(hipGetLastError());
 }
 

@@ -197,6 +197,31 @@ class AnnotatedNetwork:
                                                   C.byref(d1), C.byref(d2), _dptr(p1), _dptr(p2)))
         return LoglDerivatives(d1.value, d2.value, list(p1), list(p2))
 
+    def model_gradient(self, variant: Optional[int] = None):
+        """(lnL, [per partition: dict(subst=, freqs=, alpha=, rates=)]) --
+        dlnL with respect to each partition's exchangeabilities,
+        frequencies, Gamma shape and category rates (nrx_model_gradient)."""
+        v = int(self.options.likelihood_variant if variant is None else variant)
+        P = len(self.partitions)
+        S = self.partitions[0].model.states
+        Cn = self.partitions[0].model.ncat
+        tot = C.c_double()
+        ds = np.zeros((P, S * (S - 1) // 2))
+        df = np.zeros((P, S))
+        da = np.zeros(P)
+        dr = np.zeros((P, Cn))
+        rc = self.lib.nrx_model_gradient(self.handle, v, C.byref(tot), _dptr(ds), _dptr(df), _dptr(da), _dptr(dr))
+        if rc == _lib.NRX_ERR_NEGINF:
+            raise RuntimeError("Invalid network likelihood: negative infinity \n")
+        _lib.check(rc)
+        self.cached_logl = tot.value
+        return tot.value, [dict(subst=ds[p], freqs=df[p], alpha=da[p], rates=dr[p]) for p in range(P)]
+
+    def set_category_rates(self, i: int, rates) -> None:
+        r = np.ascontiguousarray(rates, np.float64)
+        _lib.check(self.lib.nrx_set_category_rates(self.handle, i, _dptr(r)))
+        self.cached_logl = None
+
     def branch_lengths(self) -> np.ndarray:
         """The engine's (clamped) branch lengths."""
         b = np.zeros(self.network.n_edges)
