@@ -232,6 +232,27 @@ int nrx_optimize_branches(nrx_engine *eng, int32_t variant, int32_t mode, const 
 /* The engine's branch lengths [n_edges], as clamped. */
 int nrx_get_branch_lengths(nrx_engine *eng, double *brlen);
 
+/* lnL and its first and second derivatives in every reticulation
+ * probability (d1, d2 [n_reticulations]; either may be NULL), from one
+ * evaluation that reuses every valid CLV: a probability change only
+ * re-weights the displayed trees.  Derivative counterpart of the Brent
+ * target brent_target_networks_prob
+ * (src/optimization/ReticulationOptimization.cpp:42-47). */
+int nrx_prob_derivatives(nrx_engine *eng, int32_t variant, double *total, double *d1, double *d2);
+
+/* Replaces optimize_reticulations / optimize_reticulation
+ * (src/optimization/ReticulationOptimization.cpp:68-119): rounds over the
+ * reticulations, at most max_iters, ending early when a round leaves lnL
+ * unchanged; each probability is moved inside [brprob_min, brprob_max]
+ * (NetraxOptions.hpp:102-103) by a safeguarded Newton iteration on the
+ * exact derivative in place of Brent's method, to an interval of width
+ * `tolerance`, keeping the best lnL seen.  logl (may be NULL) receives the
+ * final lnL; nrx_get_reticulation_probs reads the probabilities. */
+int nrx_optimize_reticulations(nrx_engine *eng, int32_t variant, int32_t max_iters, double brprob_min,
+                               double brprob_max, double tolerance, double *logl);
+/* The engine's reticulation probabilities (first-parent) [n_reticulations]. */
+int nrx_get_reticulation_probs(nrx_engine *eng, double *probs);
+
 /* Per-(partition, displayed tree) log-likelihoods of the last forward pass:
  * tree_logl [n_parts][n_classes], class_logprob [n_classes]. */
 int nrx_tree_logls(nrx_engine *eng, double *tree_logl, double *class_logprob);

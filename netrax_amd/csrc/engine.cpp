@@ -1764,6 +1764,131 @@ int nrx_model_gradient(nrx_engine *eng, int32_t variant, double *total, double *
   });
 }
 
+// First and second derivatives of lnL in each reticulation probability,
+// from the mixture weights of the last evaluation.  A class's probability
+// is multilinear in the probabilities, so each partition's likelihood (sum
+// of class weights, AVERAGE) or winning class (BEST) gives
+// d2 lnL_p / dp_r^2 = -(d lnL_p / dp_r)^2: the second derivative is
+// -sum_p (d1_{p,r})^2, with d1_{p,r} = sum_k alpha_{p,k} dlogP_k/dp_r.
+static void prob_derivatives(const Engine &e, double *d1, double *d2) {
+  const int nc = static_cast<int>(e.dag.classes.size()), R = e.R;
+  const double *alpha = e.h_result + e.rl.alpha;
+  for (int r = 0; r < R; ++r) {
+    double s1 = 0.0, s2 = 0.0;
+    for (int p = 0; p < e.P; ++p) {
+      double g = 0.0;
+      for (int k = 0; k < nc; ++k) g += alpha[p * nc + k] * e.class_dlogprob[static_cast<size_t>(k) * R + r];
+      s1 += g;
+      s2 -= g * g;
+    }
+    if (d1) d1[r] = s1;
+    if (d2) d2[r] = s2;
+  }
+}
+
+int nrx_prob_derivatives(nrx_engine *eng, int32_t variant, double *total, double *d1, double *d2) {
+  return guarded([&]() -> int {
+    if (!eng || !total) throw std::invalid_argument("null argument");
+    Engine &e = *static_cast<Engine *>(eng);
+    const int rc = evaluate(e, variant, true, true, false);
+    if (rc != NRX_OK) return rc;
+    *total = e.cached_total;
+    prob_derivatives(e, d1, d2);
+    return NRX_OK;
+  });
+}
+
+int nrx_optimize_reticulations(nrx_engine *eng, int32_t variant, int32_t max_iters, double brprob_min,
+                               double brprob_max, double tolerance, double *logl) {
+  return guarded([&]() -> int {
+    if (!eng) throw std::invalid_argument("null argument");
+    Engine &e = *static_cast<Engine *>(eng);
+    if (!(brprob_min > 0.0 && brprob_max < 1.0 && brprob_min < brprob_max) || !(tolerance > 0.0))
+      throw std::invalid_argument("bad reticulation probability bounds or tolerance");
+    int rc = evaluate(e, variant, true, true, false);
+    if (rc != NRX_OK) return rc;
+    std::vector<double> probs = e.probs, d1(e.R), d2(e.R);
+    // lnL, d1 and d2 of reticulation r at probability x
+    auto probe = [&](int r, double x, double &f, double &g, double &h) -> int {
+      probs[r] = x;
+      set_probs(e, probs.data());
+      const int c = evaluate(e, variant, true, true, false);
+      if (c != NRX_OK) return c;
+      prob_derivatives(e, d1.data(), d2.data());
+      f = e.cached_total;
+      g = d1[r];
+      h = d2[r];
+      return NRX_OK;
+    };
+    double act = e.cached_total;
+    for (int iter = 0; iter < max_iters; ++iter) {
+      double loop = act;
+      for (int r = 0; r < e.R; ++r) {
+        // safeguarded Newton on d lnL / dp_r = 0 inside [lo, hi]: lnL is
+        // concave in p_r when the used classes stay the same, so the slope's
+        // sign brackets the maximum; a step that leaves the bracket bisects
+        // it, and a bound is tried directly when the slope points past it
+        double x = probs[r], f, g, h;
+        rc = probe(r, x, f, g, h);
+        if (rc != NRX_OK) return rc;
+        double best_x = x, best_f = f, lo = brprob_min, hi = brprob_max;
+        bool tried_lo = false, tried_hi = false;
+        for (int it = 0; it < 100; ++it) {
+          if (g > 0.0)
+            lo = x;
+          else if (g < 0.0)
+            hi = x;
+          else
+            break;
+          if (hi - lo <= tolerance) break;
+          double xn = h < 0.0 ? x - g / h : std::nan("");
+          if (!(xn > lo && xn < hi)) {
+            if (g > 0.0 && hi == brprob_max && !tried_hi) {
+              xn = brprob_max;
+              tried_hi = true;
+            } else if (g < 0.0 && lo == brprob_min && !tried_lo) {
+              xn = brprob_min;
+              tried_lo = true;
+            } else {
+              xn = 0.5 * (lo + hi);
+            }
+          }
+          const double step = std::fabs(xn - x);
+          x = xn;
+          rc = probe(r, x, f, g, h);
+          if (rc != NRX_OK) return rc;
+          if (f > best_f) {
+            best_f = f;
+            best_x = x;
+          }
+          if (step <= tolerance) break;
+        }
+        if (probs[r] != best_x) {
+          probs[r] = best_x;
+          set_probs(e, probs.data());
+          rc = evaluate(e, variant, true, true, false);
+          if (rc != NRX_OK) return rc;
+        }
+        loop = e.cached_total;
+      }
+      const bool same = loop == act;
+      act = loop;
+      if (same) break;
+    }
+    if (logl) *logl = act;
+    return NRX_OK;
+  });
+}
+
+int nrx_get_reticulation_probs(nrx_engine *eng, double *probs) {
+  return guarded([&]() -> int {
+    if (!eng || !probs) throw std::invalid_argument("null argument");
+    const Engine &e = *static_cast<Engine *>(eng);
+    std::copy(e.probs.begin(), e.probs.end(), probs);
+    return NRX_OK;
+  });
+}
+
 int nrx_tree_logls(nrx_engine *eng, double *tree_logl, double *class_logprob) {
   return guarded([&]() -> int {
     if (!eng) throw std::invalid_argument("null argument");

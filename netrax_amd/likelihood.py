@@ -59,6 +59,9 @@ class NetraxOptions:
     min_interesting_tree_logprob: float = math.log(1e-6)  # NetraxOptions.hpp:108
     brlen_min: float = BRLEN_MIN
     brlen_max: float = BRLEN_MAX
+    brprob_min: float = 1e-6        # NetraxOptions.hpp:102
+    brprob_max: float = 1.0 - 1e-6  # NetraxOptions.hpp:103
+    prob_tolerance: float = 1e-7    # width of the final bracket of a probability
     precision: Precision = Precision.FP64
     device: int = 0
     use_graph: bool = False
@@ -244,6 +247,41 @@ class AnnotatedNetwork:
             raise RuntimeError(self.lib.nrx_last_error().decode())
         _lib.check(rc)
         self.network.brlen = self.branch_lengths()
+        self.cached_logl = out.value
+        return out.value
+
+    def prob_derivatives(self, variant: Optional[int] = None):
+        """(lnL, d lnL/dp [R], d2 lnL/dp2 [R]) with the CLVs reused."""
+        v = int(self.options.likelihood_variant if variant is None else variant)
+        tot = C.c_double()
+        R = max(self.network.n_retic, 1)
+        d1, d2 = np.zeros(R), np.zeros(R)
+        rc = self.lib.nrx_prob_derivatives(self.handle, v, C.byref(tot), _dptr(d1), _dptr(d2))
+        if rc == _lib.NRX_ERR_NEGINF:
+            raise RuntimeError("Invalid network likelihood: negative infinity \n")
+        _lib.check(rc)
+        self.cached_logl = tot.value
+        return tot.value, d1[:self.network.n_retic], d2[:self.network.n_retic]
+
+    def reticulation_probs(self) -> np.ndarray:
+        p = np.zeros(max(self.network.n_retic, 1))
+        _lib.check(self.lib.nrx_get_reticulation_probs(self.handle, _dptr(p)))
+        return p[:self.network.n_retic]
+
+    def optimize_reticulations(self, max_iters: int = 10, variant: Optional[int] = None) -> float:
+        """Reticulation probabilities by Newton on the exact derivative
+        (nrx_optimize_reticulations); returns the final lnL and updates
+        ``network.retic_prob``."""
+        v = int(self.options.likelihood_variant if variant is None else variant)
+        o = self.options
+        out = C.c_double()
+        rc = self.lib.nrx_optimize_reticulations(self.handle, v, int(max_iters), float(o.brprob_min),
+                                                 float(o.brprob_max), float(o.prob_tolerance), C.byref(out))
+        if rc == _lib.NRX_ERR_NEGINF:
+            raise RuntimeError("Invalid network likelihood: negative infinity \n")
+        _lib.check(rc)
+        if self.network.n_retic:
+            self.network.retic_prob = self.reticulation_probs()
         self.cached_logl = out.value
         return out.value
 
