@@ -279,6 +279,23 @@ int nrx_batch_destroy(nrx_batch *batch);
 int nrx_batch_evaluate(nrx_batch *batch, int32_t variant, double *lnl, double *dbrlen,
                        double *dprob, double *device_ms);
 
+/* Optimises every candidate of the batch, then scores them all in one
+ * batched evaluation (lnl [n]): optimizeAfterMove's per-candidate steps
+ * (src/search/Filtering.cpp:158-208) ahead of the scoring loop of
+ * filterCandidates (:236-270).  Candidate i's branches are
+ * edges[edge_begin[i] .. edge_begin[i+1]), swept by the device Newton
+ * iteration of nrx_optimize_branches with brlen_iters iterations (1: the
+ * prefilter's single step, 32: a full sweep), then its reticulations
+ * retics[retic_begin[i] .. retic_begin[i+1]) by nrx_optimize_reticulations'
+ * search with prob_iters rounds.  Each engine runs on its own stream;
+ * n_threads host workers (<= 0: up to 16) drive the candidates so their
+ * host waits overlap.  status [n] (may be NULL) receives each candidate's
+ * return code; without it the first failure is returned after scoring. */
+int nrx_batch_optimize(nrx_batch *batch, int32_t variant, int32_t mode, const int32_t *edge_begin,
+                       const int32_t *edges, int32_t brlen_iters, const int32_t *retic_begin,
+                       const int32_t *retics, int32_t prob_iters, double brprob_min, double brprob_max,
+                       double prob_tolerance, int32_t n_threads, double *lnl, int32_t *status);
+
 /* Device timing of the last evaluation (zeros unless timing is enabled). */
 int nrx_last_timing(nrx_engine *eng, nrx_timing *out);
 

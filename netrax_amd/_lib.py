@@ -24,7 +24,7 @@ EXPORTED = [
     "nrx_set_category_rates", "nrx_set_branch_lengths", "nrx_set_reticulation_probs",
     "nrx_loglikelihood", "nrx_gradient", "nrx_brlen_derivatives", "nrx_brlen_derivatives_noalloc",
     "nrx_tree_logls", "nrx_optimize_branches", "nrx_get_branch_lengths", "nrx_model_gradient",
-    "nrx_prob_derivatives", "nrx_optimize_reticulations", "nrx_get_reticulation_probs",
+    "nrx_prob_derivatives", "nrx_optimize_reticulations", "nrx_get_reticulation_probs", "nrx_batch_optimize",
     "nrx_get_stats", "nrx_last_timing", "nrx_batch_create", "nrx_batch_destroy",
     "nrx_batch_evaluate", "nrx_autotune", "nrx_set_tuning", "nrx_get_tuning",
 ]
@@ -107,6 +107,9 @@ def load(build_if_missing: bool = True):
     lib.nrx_optimize_reticulations.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_double,
                                                C.c_double, pd]
     lib.nrx_get_reticulation_probs.argtypes = [C.c_void_p, pd]
+    pi = C.POINTER(C.c_int32)
+    lib.nrx_batch_optimize.argtypes = [C.c_void_p, C.c_int32, C.c_int32, pi, pi, C.c_int32, pi, pi, C.c_int32,
+                                       C.c_double, C.c_double, C.c_double, C.c_int32, pd, pi]
     lib.nrx_get_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
     lib.nrx_last_timing.argtypes = [C.c_void_p, C.POINTER(Timing)]
     lib.nrx_batch_create.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_void_p)]

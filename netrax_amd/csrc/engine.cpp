@@ -9,12 +9,14 @@
 // src/graph/AnnotatedNetwork.cpp:359-394).
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <exception>
 #include <functional>
 #include <stdexcept>
 #include <string>
+#include <thread>
 
 #include "engine_state.hpp"
 #include "schedule.hpp"
@@ -1502,11 +1504,12 @@ int nrx_get_branch_lengths(nrx_engine *eng, double *brlen) {
   });
 }
 
-int nrx_optimize_branches(nrx_engine *eng, int32_t variant, int32_t mode, const int32_t *edges, int32_t n_edges,
-                          int32_t max_iters, double *logl) {
-  return guarded([&]() -> int {
-    if (!eng || (n_edges > 0 && !edges)) throw std::invalid_argument("null argument");
-    Engine &e = *static_cast<Engine *>(eng);
+// The device Newton sweep of one engine over `edges` (see
+// nrx_optimize_branches); throws on bad arguments.
+static int optimize_branches_impl(Engine &e, int variant, int mode, const int32_t *edges, int n_edges,
+                                  int max_iters, double *logl) {
+  {
+    if (n_edges > 0 && !edges) throw std::invalid_argument("null argument");
     if (mode != NRX_DERIV_REFERENCE && mode != NRX_DERIV_TRUE)
       throw std::invalid_argument("unknown derivative mode");
     if (variant != NRX_AVERAGE_DISPLAYED_TREES && variant != NRX_BEST_DISPLAYED_TREE)
@@ -1599,6 +1602,14 @@ int nrx_optimize_branches(nrx_engine *eng, int32_t variant, int32_t mode, const 
     if (e.cached_total < start && std::fabs(e.cached_total - start) >= 1e-3)
       throw std::runtime_error("Overall loglikelihood got worse");
     return NRX_OK;
+  }
+}
+
+int nrx_optimize_branches(nrx_engine *eng, int32_t variant, int32_t mode, const int32_t *edges, int32_t n_edges,
+                          int32_t max_iters, double *logl) {
+  return guarded([&]() -> int {
+    if (!eng) throw std::invalid_argument("null argument");
+    return optimize_branches_impl(*static_cast<Engine *>(eng), variant, mode, edges, n_edges, max_iters, logl);
   });
 }
 
@@ -1798,11 +1809,20 @@ int nrx_prob_derivatives(nrx_engine *eng, int32_t variant, double *total, double
   });
 }
 
-int nrx_optimize_reticulations(nrx_engine *eng, int32_t variant, int32_t max_iters, double brprob_min,
-                               double brprob_max, double tolerance, double *logl) {
-  return guarded([&]() -> int {
-    if (!eng) throw std::invalid_argument("null argument");
-    Engine &e = *static_cast<Engine *>(eng);
+// Rounds of per-reticulation Newton searches over `retics` (all when
+// null) of one engine (see nrx_optimize_reticulations).
+static int optimize_reticulations_impl(Engine &e, int variant, int max_iters, const int32_t *retics, int n_retics,
+                                       double brprob_min, double brprob_max, double tolerance, double *logl) {
+  {
+    std::vector<int> list;
+    if (retics) {
+      for (int i = 0; i < n_retics; ++i) {
+        if (retics[i] < 0 || retics[i] >= e.R) throw std::invalid_argument("reticulation index out of range");
+        list.push_back(retics[i]);
+      }
+    } else {
+      for (int r = 0; r < e.R; ++r) list.push_back(r);
+    }
     if (!(brprob_min > 0.0 && brprob_max < 1.0 && brprob_min < brprob_max) || !(tolerance > 0.0))
       throw std::invalid_argument("bad reticulation probability bounds or tolerance");
     int rc = evaluate(e, variant, true, true, false);
@@ -1823,7 +1843,7 @@ int nrx_optimize_reticulations(nrx_engine *eng, int32_t variant, int32_t max_ite
     double act = e.cached_total;
     for (int iter = 0; iter < max_iters; ++iter) {
       double loop = act;
-      for (int r = 0; r < e.R; ++r) {
+      for (int r : list) {
         // safeguarded Newton on d lnL / dp_r = 0 inside [lo, hi]: lnL is
         // concave in p_r when the used classes stay the same, so the slope's
         // sign brackets the maximum; a step that leaves the bracket bisects
@@ -1877,6 +1897,15 @@ int nrx_optimize_reticulations(nrx_engine *eng, int32_t variant, int32_t max_ite
     }
     if (logl) *logl = act;
     return NRX_OK;
+  }
+}
+
+int nrx_optimize_reticulations(nrx_engine *eng, int32_t variant, int32_t max_iters, double brprob_min,
+                               double brprob_max, double tolerance, double *logl) {
+  return guarded([&]() -> int {
+    if (!eng) throw std::invalid_argument("null argument");
+    return optimize_reticulations_impl(*static_cast<Engine *>(eng), variant, max_iters, nullptr, 0, brprob_min,
+                                       brprob_max, tolerance, logl);
   });
 }
 
@@ -1977,6 +2006,63 @@ int nrx_last_timing(nrx_engine *eng, nrx_timing *out) {
     if (!eng || !out) throw std::invalid_argument("null argument");
     *out = static_cast<Engine *>(eng)->last_timing;
     return NRX_OK;
+  });
+}
+
+int nrx_batch_optimize(nrx_batch *batch, int32_t variant, int32_t mode, const int32_t *edge_begin,
+                       const int32_t *edges, int32_t brlen_iters, const int32_t *retic_begin,
+                       const int32_t *retics, int32_t prob_iters, double brprob_min, double brprob_max,
+                       double prob_tolerance, int32_t n_threads, double *lnl, int32_t *status) {
+  return guarded([&]() -> int {
+    if (!batch || !edge_begin || !retic_begin || !lnl) throw std::invalid_argument("null argument");
+    Batch &b = *reinterpret_cast<Batch *>(batch);
+    const int n = static_cast<int>(b.engines.size());
+    for (int i = 0; i < n; ++i) {
+      if (edge_begin[i + 1] < edge_begin[i] || retic_begin[i + 1] < retic_begin[i])
+        throw std::invalid_argument("candidate lists must be given by non-decreasing offsets");
+      if ((edge_begin[i + 1] > edge_begin[i] && !edges) || (retic_begin[i + 1] > retic_begin[i] && !retics))
+        throw std::invalid_argument("null candidate list");
+    }
+    if (brlen_iters < 0 || prob_iters < 0) throw std::invalid_argument("iteration counts must be >= 0");
+    // every engine runs on its own stream: the workers' host waits overlap
+    // and the device interleaves the candidates' kernels
+    const int workers = std::max(1, std::min(n_threads > 0 ? n_threads : 16, n));
+    std::atomic<int> next{0};
+    std::vector<int> rcs(n, NRX_OK);
+    std::vector<std::string> msgs(n);
+    auto work = [&]() {
+      (void)hipSetDevice(b.engines[0]->opt.device);
+      for (int i = next++; i < n; i = next++) {
+        Engine &e = *b.engines[i];
+        rcs[i] = guarded([&]() -> int {
+          double l = 0.0;
+          int rc = NRX_OK;
+          const int ne = edge_begin[i + 1] - edge_begin[i];
+          if (ne > 0 && brlen_iters > 0)
+            rc = optimize_branches_impl(e, variant, mode, edges + edge_begin[i], ne, brlen_iters, &l);
+          const int nr = retic_begin[i + 1] - retic_begin[i];
+          if (rc == NRX_OK && nr > 0 && prob_iters > 0)
+            rc = optimize_reticulations_impl(e, variant, prob_iters, retics + retic_begin[i], nr, brprob_min,
+                                             brprob_max, prob_tolerance, &l);
+          return rc;
+        });
+        if (rcs[i] != NRX_OK) msgs[i] = g_last_error;
+      }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < workers; ++t) pool.emplace_back(work);
+    work();
+    for (std::thread &t : pool) t.join();
+    int first_bad = -1;
+    for (int i = 0; i < n; ++i) {
+      if (status) status[i] = rcs[i];
+      if (rcs[i] != NRX_OK && first_bad < 0) first_bad = i;
+    }
+    // the optimised candidates scored together in one batched launch
+    const int rc = nrx_batch_evaluate(batch, variant, lnl, nullptr, nullptr, nullptr);
+    if (first_bad >= 0 && !status)
+      return fail(rcs[first_bad], "candidate " + std::to_string(first_bad) + ": " + msgs[first_bad]);
+    return rc;
   });
 }
 

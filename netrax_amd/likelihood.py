@@ -419,6 +419,43 @@ class NetworkBatch:
                 ro += r
         return lnl, dbs, dps, ms.value
 
+    def optimize(self, edges, retics, brlen_iters: int = 1, prob_iters: int = 1, variant: int = 0,
+                 mode: int = _lib.NRX_DERIV_REFERENCE, n_threads: int = 0):
+        """Per-candidate branch and reticulation optimisation, then one
+        batched scoring (nrx_batch_optimize).  ``edges[i]`` / ``retics[i]``
+        list candidate i's branches and reticulations.  Returns (lnL [n],
+        status [n]) and refreshes each network's lengths and probabilities."""
+        if not getattr(self, "handle", None):
+            raise RuntimeError("the batch is closed")
+        self._check_open()
+        n = len(self.networks)
+        if len(edges) != n or len(retics) != n:
+            raise ValueError("one edge list and one reticulation list per candidate expected")
+        eb = np.zeros(n + 1, np.int32)
+        rb = np.zeros(n + 1, np.int32)
+        eb[1:] = np.cumsum([len(x) for x in edges])
+        rb[1:] = np.cumsum([len(x) for x in retics])
+        ev = np.ascontiguousarray(np.concatenate([np.asarray(x, np.int32) for x in edges] + [np.zeros(1, np.int32)]))
+        rv = np.ascontiguousarray(np.concatenate([np.asarray(x, np.int32) for x in retics] + [np.zeros(1, np.int32)]))
+        o = self.networks[0].options
+        lnl = np.zeros(n)
+        st = np.zeros(n, np.int32)
+        ip = C.POINTER(C.c_int32)
+        rc = self.lib.nrx_batch_optimize(self.handle, int(variant), int(mode), eb.ctypes.data_as(ip),
+                                         ev.ctypes.data_as(ip), int(brlen_iters), rb.ctypes.data_as(ip),
+                                         rv.ctypes.data_as(ip), int(prob_iters), float(o.brprob_min),
+                                         float(o.brprob_max), float(o.prob_tolerance), int(n_threads),
+                                         _dptr(lnl), st.ctypes.data_as(ip))
+        if rc == _lib.NRX_ERR_NEGINF:
+            raise RuntimeError(self.lib.nrx_last_error().decode())
+        _lib.check(rc)
+        for a in self.networks:
+            a.network.brlen = a.branch_lengths()
+            if a.network.n_retic:
+                a.network.retic_prob = a.reticulation_probs()
+            a.cached_logl = None
+        return lnl, st
+
     def close(self) -> None:
         if getattr(self, "handle", None):
             self.lib.nrx_batch_destroy(self.handle)

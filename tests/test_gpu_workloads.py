@@ -250,3 +250,50 @@ def test_device_newton_sweep_matches_host_sweep(name, prec):
         assert np.array_equal(dev.branch_lengths(), dev.network.brlen)
         host.close()
         dev.close()
+
+
+def _move_edges(net):
+    """Branches around the reticulations (those a move touches), plus the
+    first edge, in index order."""
+    es = {0}
+    for r in range(net.n_retic):
+        es.add(int(net.retic_first_edge[r]))
+        es.add(int(net.retic_second_edge[r]))
+    return sorted(es)
+
+
+def test_batch_optimize_matches_engines_optimised_alone_and_oracle():
+    from netrax_amd.simulate import candidate_batch
+    cands = candidate_batch(2, 8, seed=7, sites=(2000, 8000))
+    say("batch optimise: 16 candidates")
+    edges = [_move_edges(n) for n, _p in cands]
+    retics = [list(range(n.n_retic)) for n, _p in cands]
+    orig = [n.copy() for n, _p in cands]  # the engines update their networks
+    start = []
+    anns = []
+    for n, p in cands:
+        a = eng(n, p, use_graph=True)
+        start.append(a.loglikelihood())
+        anns.append(a)
+    batch = NetworkBatch(anns)
+    lnl, st = batch.optimize(edges, retics, brlen_iters=32, prob_iters=10, n_threads=8)
+    assert np.all(st == 0), st
+    say("batch optimise: lnL gain", np.round(lnl - np.array(start), 3))
+    # the same steps on twin engines, one at a time
+    for i, (n, p) in enumerate(cands):
+        twin = eng(orig[i].copy(), p, use_graph=True)
+        twin.optimize_branches(edges[i], max_iters=32)
+        lt = twin.optimize_reticulations(10) if n.n_retic else twin.loglikelihood()
+        assert abs(lt - lnl[i]) <= 1e-12 * abs(lt), (i, lt, lnl[i])
+        np.testing.assert_array_equal(twin.branch_lengths(), anns[i].branch_lengths())
+        assert lnl[i] >= start[i] - FP32_RTOL * abs(start[i]), (i, lnl[i], start[i])
+        twin.close()
+    # the optimised parameters score the same on the oracle
+    for i in (0, 7, 8, 15):
+        n, p = cands[i]
+        ol = O.loglikelihood(n, p, O.AVERAGE, brlen=anns[i].branch_lengths(),
+                             probs=anns[i].reticulation_probs() if n.n_retic else None)[0]
+        assert abs(lnl[i] - ol) <= FP32_RTOL * abs(ol), (i, lnl[i], ol)
+    batch.close()
+    for a in anns:
+        a.close()
