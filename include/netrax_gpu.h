@@ -296,6 +296,29 @@ int nrx_batch_optimize(nrx_batch *batch, int32_t variant, int32_t mode, const in
                        const int32_t *retics, int32_t prob_iters, double brprob_min, double brprob_max,
                        double prob_tolerance, int32_t n_threads, double *lnl, int32_t *status);
 
+/* Memory budget of the FP64 form: bytes for the CLV, adjoint and scaler
+ * rows (0: every CLV resident, the default).  Under a budget smaller than
+ * the resident rows the level schedule is cut into segments; CLVs read by
+ * a later segment keep their rows and each segment's other CLVs share a
+ * pool, so the forward pass keeps only those and the reverse sweep
+ * recomputes each segment's pooled CLVs before its adjoint (at most one
+ * more forward pass).  lnL and gradient are bit-identical to the resident
+ * evaluation.  Branch derivatives, sumtables and model gradients need
+ * every CLV and return NRX_ERR_UNSUPPORTED while a budget is active.
+ * NRX_ERR_ARG when even the finest segmentation does not fit (the previous
+ * setting stays).  No single reference counterpart: the reference keeps
+ * per-displayed-tree CLVs and recomputes along re-rooted paths instead
+ * (src/likelihood/VirtualRerooting.cpp:192-252). */
+int nrx_set_memory_budget(nrx_engine *eng, uint64_t bytes);
+typedef struct {
+  int32_t segments;      /* 1: every CLV resident */
+  int32_t rows;          /* CLV rows allocated */
+  int32_t boundary_rows; /* rows kept across segments */
+  int32_t resident_rows; /* rows without a budget */
+  double row_bytes;      /* bytes of one CLV + adjoint + scaler row */
+} nrx_memory_plan_info;
+int nrx_memory_plan(nrx_engine *eng, nrx_memory_plan_info *out);
+
 /* Device timing of the last evaluation (zeros unless timing is enabled). */
 int nrx_last_timing(nrx_engine *eng, nrx_timing *out);
 

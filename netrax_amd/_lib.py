@@ -25,6 +25,7 @@ EXPORTED = [
     "nrx_loglikelihood", "nrx_gradient", "nrx_brlen_derivatives", "nrx_brlen_derivatives_noalloc",
     "nrx_tree_logls", "nrx_optimize_branches", "nrx_get_branch_lengths", "nrx_model_gradient",
     "nrx_prob_derivatives", "nrx_optimize_reticulations", "nrx_get_reticulation_probs", "nrx_batch_optimize",
+    "nrx_set_memory_budget", "nrx_memory_plan",
     "nrx_get_stats", "nrx_last_timing", "nrx_batch_create", "nrx_batch_destroy",
     "nrx_batch_evaluate", "nrx_autotune", "nrx_set_tuning", "nrx_get_tuning",
 ]
@@ -67,6 +68,11 @@ class Tuning(C.Structure):
                 ("ms", C.c_double)]
 
 
+class MemoryPlan(C.Structure):
+    _fields_ = [("segments", C.c_int32), ("rows", C.c_int32), ("boundary_rows", C.c_int32),
+                ("resident_rows", C.c_int32), ("row_bytes", C.c_double)]
+
+
 _lib = None
 
 
@@ -107,6 +113,8 @@ def load(build_if_missing: bool = True):
     lib.nrx_optimize_reticulations.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_double,
                                                C.c_double, pd]
     lib.nrx_get_reticulation_probs.argtypes = [C.c_void_p, pd]
+    lib.nrx_set_memory_budget.argtypes = [C.c_void_p, C.c_uint64]
+    lib.nrx_memory_plan.argtypes = [C.c_void_p, C.POINTER(MemoryPlan)]
     pi = C.POINTER(C.c_int32)
     lib.nrx_batch_optimize.argtypes = [C.c_void_p, C.c_int32, C.c_int32, pi, pi, C.c_int32, pi, pi, C.c_int32,
                                        C.c_double, C.c_double, C.c_double, C.c_int32, pd, pi]

@@ -32,10 +32,17 @@ int fail(int code, const std::string &msg) {
   return code;
 }
 
+// a call this engine's configuration does not provide (NRX_ERR_UNSUPPORTED)
+struct unsupported : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
 template <typename F>
 int guarded(F &&f) {
   try {
     return f();
+  } catch (const unsupported &ex) {
+    return fail(NRX_ERR_UNSUPPORTED, ex.what());
   } catch (const std::invalid_argument &ex) {
     return fail(NRX_ERR_ARG, ex.what());
   } catch (const std::runtime_error &ex) {
@@ -142,6 +149,125 @@ static void compute_class_probs(Engine &e) {
 }
 
 static void upload_netdesc(Engine &e);
+
+// Bytes of one CLV row with its adjoint and scaler rows (FP64 form).
+static size_t budget_row_bytes(const Engine &e) {
+  return sizeof(double) * e.clv_elems_per_slot * 2 + sizeof(int32_t) * static_cast<size_t>(e.L.n_sites_pad);
+}
+
+static void resize_rows(Engine &e, size_t rows) {
+  rows = std::max<size_t>(rows, 1);
+  e.d_clv.release();
+  e.d_clv.alloc(sizeof(double) * e.clv_elems_per_slot * rows);
+  e.d_scl.release();
+  e.d_scl.alloc(sizeof(int32_t) * static_cast<size_t>(e.L.n_sites_pad) * rows);
+  e.d_adj.release();
+  e.d_adj.alloc(sizeof(double) * e.clv_elems_per_slot * rows);
+  std::fill(e.clv_dirty.begin(), e.clv_dirty.end(), 1);
+  e.cache_valid = false;
+  e.deriv_valid = false;
+}
+
+// Segments of the level schedule whose rows fit the memory budget: the
+// fewest segments (least recomputation) that fit, cut at level boundaries
+// with about equal op counts.  The reverse sweep then recomputes every
+// segment but the last one, at most one more forward pass.
+static void plan_budget(Engine &e) {
+  auto &bp = e.bp;
+  const int nv = static_cast<int>(e.dag.var.size());
+  const bool was_on = bp.n_seg > 1;
+  const size_t row = budget_row_bytes(e);
+  if (e.mem_budget == 0 || e.opt.precision != NRX_FP64 ||
+      static_cast<size_t>(std::max(nv, 1)) * row <= e.mem_budget) {
+    bp = {};
+    if (was_on) resize_rows(e, nv);
+    return;
+  }
+  const int nch = static_cast<int>(e.chunks.size());
+  const int total = static_cast<int>(e.ops_all.size());
+  std::vector<int> seg_of(nv, -1), last(nv, -1), cuts;
+  size_t smallest = SIZE_MAX;
+  bool found = false;
+  for (int K = 2; K <= nch && !found; ++K) {
+    cuts.assign(1, 0);
+    int acc = 0;
+    for (int c = 0; c < nch; ++c) {
+      acc += e.chunks[c].op_end - e.chunks[c].op_begin;
+      const int k = static_cast<int>(cuts.size());
+      if (k < K && c + 1 < nch && static_cast<long>(acc) * K >= static_cast<long>(k) * total) cuts.push_back(c + 1);
+    }
+    cuts.push_back(nch);
+    const int ns = static_cast<int>(cuts.size()) - 1;
+    std::fill(seg_of.begin(), seg_of.end(), -1);
+    std::fill(last.begin(), last.end(), -1);
+    for (int j = 0; j < ns; ++j)
+      for (int c = cuts[j]; c < cuts[j + 1]; ++c)
+        for (int i = e.chunks[c].op_begin; i < e.chunks[c].op_end; ++i) {
+          const DevOp &op = e.ops_all[i];
+          seg_of[op.out] = j;
+          for (int k = 0; k < op.nk; ++k)
+            if (op.kid[k] >= 0) last[op.kid[k]] = std::max(last[op.kid[k]], j);
+        }
+    int nb = 0;
+    std::vector<int> internal(ns, 0);
+    for (int s = 0; s < nv; ++s) {
+      if (seg_of[s] < 0) continue;
+      if (last[s] > seg_of[s])
+        ++nb;
+      else
+        ++internal[seg_of[s]];
+    }
+    const size_t phys = static_cast<size_t>(nb) + *std::max_element(internal.begin(), internal.end());
+    smallest = std::min(smallest, phys * row);
+    if (phys * row <= e.mem_budget) {
+      found = true;
+      bp = {};
+      bp.n_seg = ns;
+      bp.n_boundary = nb;
+      bp.n_phys = static_cast<int>(phys);
+      bp.seg_chunk = cuts;
+    }
+  }
+  if (!found)
+    throw std::invalid_argument("memory budget of " + std::to_string(e.mem_budget) +
+                                " bytes is below the smallest segment plan (" +
+                                std::to_string(std::min(smallest, static_cast<size_t>(std::max(nv, 1)) * row)) +
+                                " bytes)");
+  // physical rows: boundary CLVs first, then each segment's pool
+  std::vector<int> phys(nv, -1), pool_next(bp.n_seg, bp.n_boundary);
+  int nb = 0;
+  for (int s = 0; s < nv; ++s)
+    if (seg_of[s] >= 0 && last[s] > seg_of[s]) phys[s] = nb++;
+  for (int s = 0; s < nv; ++s)
+    if (seg_of[s] >= 0 && phys[s] < 0) phys[s] = pool_next[seg_of[s]]++;
+  std::vector<DevOp> ops = e.ops_all, rec;
+  bp.rec_begin.assign(1, 0);
+  for (int c = 0; c < nch; ++c) {
+    for (int i = e.chunks[c].op_begin; i < e.chunks[c].op_end; ++i) {
+      DevOp &op = ops[i];
+      const bool pooled = phys[op.out] >= bp.n_boundary;
+      op.out = phys[op.out];
+      for (int k = 0; k < op.nk; ++k)
+        if (op.kid[k] >= 0) op.kid[k] = phys[op.kid[k]];
+      if (pooled) rec.push_back(op);
+    }
+    bp.rec_begin.push_back(static_cast<int>(rec.size()));
+  }
+  bp.seg_root_begin.assign(1, 0);
+  for (int j = 0; j < bp.n_seg; ++j) {
+    for (size_t r = 0; r < e.root_slots.size(); ++r)
+      if (seg_of[e.root_slots[r]] == j) {
+        bp.root_phys.push_back(phys[e.root_slots[r]]);
+        bp.root_index.push_back(static_cast<int32_t>(r));
+      }
+    bp.seg_root_begin.push_back(static_cast<int>(bp.root_phys.size()));
+  }
+  upload(e.d_ops_b, ops, e.stream);
+  upload(e.d_ops_rec, rec, e.stream);
+  upload(e.d_root_phys, bp.root_phys, e.stream);
+  upload(e.d_root_index, bp.root_index, e.stream);
+  resize_rows(e, bp.n_phys);
+}
 
 // Builds the level schedule, the batched order, root CLV set and gradient
 // rows for the classes that pass the log-probability cut, and uploads them.
@@ -302,6 +428,7 @@ static void build_schedule(Engine &e) {
   e.d_use_sums.alloc(sizeof(double) * std::max(1, e.n_uses) * static_cast<size_t>(e.P));
   NRX_HIP(hipStreamSynchronize(e.stream));
   upload_netdesc(e);
+  plan_budget(e);
   e.schedule_version++;
 }
 
@@ -560,6 +687,43 @@ static void fetch_results(Engine &e) {
   NRX_HIP(hipStreamSynchronize(e.stream));
 }
 
+// Forward pass, root sums and (with `gradient`) reverse sweep of the FP64
+// form under a memory budget; the mixture runs between the two.
+static void budget_forward(Engine &e, bool gradient) {
+  const auto &bp = e.bp;
+  const DevOp *ops = e.d_ops_b.as<DevOp>();
+  for (int j = 0; j < bp.n_seg; ++j) {
+    for (int c = bp.seg_chunk[j]; c < bp.seg_chunk[j + 1]; ++c) {
+      const LevelChunk &ch = e.chunks[c];
+      record_kernel_begin(e, 0);
+      launch_forward_level(e, ops + ch.op_begin, ch.op_end - ch.op_begin, ch);
+      record_kernel_end(e, 0);
+    }
+    const int r0 = bp.seg_root_begin[j], nr = bp.seg_root_begin[j + 1] - r0;
+    launch_root_partials(e, gradient, e.d_root_phys.as<int32_t>() + r0, e.d_root_index.as<int32_t>() + r0, nr);
+  }
+}
+
+static void budget_adjoint(Engine &e) {
+  const auto &bp = e.bp;
+  const DevOp *ops = e.d_ops_b.as<DevOp>(), *rec = e.d_ops_rec.as<DevOp>();
+  for (int j = bp.n_seg - 1; j >= 0; --j) {
+    if (j < bp.n_seg - 1)  // the last segment's pool still holds its CLVs
+      for (int c = bp.seg_chunk[j]; c < bp.seg_chunk[j + 1]; ++c) {
+        const int n = bp.rec_begin[c + 1] - bp.rec_begin[c];
+        if (n == 0) continue;
+        record_kernel_begin(e, 0);
+        launch_forward_level(e, rec + bp.rec_begin[c], n, e.chunks[c]);
+        record_kernel_end(e, 0);
+      }
+    for (int c = bp.seg_chunk[j + 1] - 1; c >= bp.seg_chunk[j]; --c) {
+      record_kernel_begin(e, 1);
+      launch_adjoint_level(e, e.chunks[c], ops);
+      record_kernel_end(e, 1);
+    }
+  }
+}
+
 static int evaluate(Engine &e, int variant, bool incremental, bool update_pm, bool gradient) {
   check_ready(e);
   if (variant != NRX_AVERAGE_DISPLAYED_TREES && variant != NRX_BEST_DISPLAYED_TREE)
@@ -618,6 +782,15 @@ static int evaluate(Engine &e, int variant, bool incremental, bool update_pm, bo
       results_copied = true;
     } else {
       enqueue(true);
+    }
+    std::fill(e.clv_dirty.begin(), e.clv_dirty.end(), 0);
+  } else if (e.bp.n_seg > 1) {
+    // memory budget: segments forward, then the reverse sweep recomputing
+    budget_forward(e, gradient);
+    launch_mixture(e, variant, e.pw);
+    if (gradient) {
+      budget_adjoint(e);
+      launch_grad_reduce(e);
     }
     std::fill(e.clv_dirty.begin(), e.clv_dirty.end(), 0);
   } else {
@@ -711,8 +884,15 @@ static void make_deriv_paths(const Engine &e, int edge, std::vector<int> &cls, s
   }
 }
 
+// Branch derivatives and model gradients read the CLVs of every op after
+// the evaluation; under a memory budget only some of them survive it.
+static void require_resident(const Engine &e) {
+  if (e.bp.n_seg > 1) throw unsupported("this call needs every CLV resident; the memory budget is active");
+}
+
 // Sumtables of branch `edge` (the device-side computePartitionSumtables).
 static void build_sumtables(Engine &e, int edge) {
+  require_resident(e);
   const int nc = static_cast<int>(e.dag.classes.size());
   const int nv = static_cast<int>(e.dag.var.size());
   if (e.d_ops_by_slot.p == nullptr || e.d_ops_by_slot.bytes < sizeof(DevOp) * nv)
@@ -1450,6 +1630,7 @@ int nrx_brlen_derivatives_noalloc(nrx_engine *eng, int32_t variant, int32_t edge
     const int nc = static_cast<int>(e.dag.classes.size());
     // layout of the fixed scratch: op table, path steps and bounds, per-wave
     // partials, per-(path, partition) sums
+    require_resident(e);
     const std::vector<DevOp> table = ops_by_slot_table(e);
     auto aligned = [](size_t b) { return (b + 255) & ~size_t(255); };
     const size_t o_ops = 0;
@@ -1620,6 +1801,7 @@ int nrx_model_gradient(nrx_engine *eng, int32_t variant, double *total, double *
     Engine &e = *static_cast<Engine *>(eng);
     if (e.opt.precision != NRX_FP64 || e.S != 4)
       return fail(NRX_ERR_UNSUPPORTED, "model gradients are provided for the FP64 form with 4-state data");
+    require_resident(e);
     const int rc = evaluate(e, variant, true, true, true);  // CLVs, adjoints, mixture weights
     if (rc != NRX_OK) return rc;
     *total = e.cached_total;
@@ -2005,6 +2187,38 @@ int nrx_last_timing(nrx_engine *eng, nrx_timing *out) {
   return guarded([&]() -> int {
     if (!eng || !out) throw std::invalid_argument("null argument");
     *out = static_cast<Engine *>(eng)->last_timing;
+    return NRX_OK;
+  });
+}
+
+int nrx_set_memory_budget(nrx_engine *eng, uint64_t bytes) {
+  return guarded([&]() -> int {
+    if (!eng) throw std::invalid_argument("null argument");
+    Engine &e = *static_cast<Engine *>(eng);
+    if (bytes != 0 && e.opt.precision != NRX_FP64)
+      return fail(NRX_ERR_UNSUPPORTED, "memory budgets are provided for the FP64 form");
+    const uint64_t old = e.mem_budget;
+    e.mem_budget = bytes;
+    try {
+      plan_budget(e);
+    } catch (...) {
+      e.mem_budget = old;  // the previous plan stays in force
+      throw;
+    }
+    return NRX_OK;
+  });
+}
+
+int nrx_memory_plan(nrx_engine *eng, nrx_memory_plan_info *out) {
+  return guarded([&]() -> int {
+    if (!eng || !out) throw std::invalid_argument("null argument");
+    const Engine &e = *static_cast<Engine *>(eng);
+    out->segments = std::max(1, e.bp.n_seg);
+    const int nv = static_cast<int>(e.dag.var.size());
+    out->rows = e.bp.n_seg > 1 ? e.bp.n_phys : nv;
+    out->boundary_rows = e.bp.n_seg > 1 ? e.bp.n_boundary : nv;
+    out->resident_rows = nv;
+    out->row_bytes = static_cast<double>(budget_row_bytes(e));
     return NRX_OK;
   });
 }

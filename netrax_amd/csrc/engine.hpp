@@ -235,8 +235,12 @@ int adjoint_pass_slots(int C, int G, int TS, int H, int E, int64_t n_work);
 void launch_pmatrices(Engine &e);
 void launch_forward_level(Engine &e, const DevOp *ops, int n_ops, const LevelChunk &ch);
 void launch_root_partials(Engine &e, bool seeds);
+// root partials of n roots: physical rows `rows`, root indices `index`
+void launch_root_partials(Engine &e, bool seeds, const int32_t *rows, const int32_t *index, int n);
 void launch_mixture(Engine &e, int variant, int partials_per_tile);
 void launch_adjoint_level(Engine &e, const LevelChunk &ch);
+// the chunk's ops taken from `ops` (same positions as in the op list)
+void launch_adjoint_level(Engine &e, const LevelChunk &ch, const DevOp *ops);
 void launch_grad_reduce(Engine &e);
 void launch_forward_persistent(Engine &e);
 void launch_adjoint_persistent(Engine &e);

@@ -104,6 +104,23 @@ struct Engine : nrx_engine {
   // sequence, valid while the sumtable buffers keep their addresses
   DevBuf d_nr_state, d_nr_cls, d_nr_base;
   DevBuf d_mg_g, d_mg_r;  // model gradient partials
+  // Memory budget of the FP64 form: with n_seg > 1 the CLV, adjoint and
+  // scaler rows live in n_phys physical rows.  The level chunks are cut into
+  // n_seg segments; a CLV consumed in a later segment than its own keeps a
+  // fixed row (the boundary rows), the others share the rows of a pool that
+  // each segment reuses.  The forward pass runs the segments in order; the
+  // reverse sweep recomputes a segment's pooled CLVs before its adjoint.
+  uint64_t mem_budget = 0;  // bytes of CLV + adjoint + scaler rows, 0 = all resident
+  struct BudgetPlan {
+    int n_seg = 0;  // 0 or 1: every CLV resident
+    int n_phys = 0, n_boundary = 0;
+    std::vector<int> seg_chunk;         // [n_seg + 1] chunk ranges of the segments
+    std::vector<int> rec_begin;         // [n_chunks + 1] ranges of ops_rec per chunk
+    std::vector<int32_t> root_phys;     // per segment, its roots' physical rows ...
+    std::vector<int32_t> root_index;    // ... and their root indices
+    std::vector<int> seg_root_begin;    // [n_seg + 1] ranges of the two above
+  } bp;
+  DevBuf d_ops_b, d_ops_rec, d_root_phys, d_root_index;
   hipGraphExec_t nr_graph = nullptr;
   int nr_graph_key[4] = {-1, -1, -1, -1};  // variant, mode, max_iters, paths
   const void *nr_graph_bufs[3] = {nullptr, nullptr, nullptr};

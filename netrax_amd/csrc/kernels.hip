@@ -423,7 +423,7 @@ __global__ void __launch_bounds__(TILE *C)
                     const double *__restrict__ clv, const int32_t *__restrict__ scl,
                     const double *__restrict__ freqs, const double *__restrict__ weights,
                     const int32_t *__restrict__ tile_part, double *__restrict__ partials,
-                    double *__restrict__ seed) {
+                    double *__restrict__ seed, const int32_t *__restrict__ root_index) {
   constexpr int PW = TILE * C / 64;
   const int tile = blockIdx.x;
   if (tile >= L.n_tiles) return;
@@ -434,8 +434,9 @@ __global__ void __launch_bounds__(TILE *C)
   const size_t n_wt = static_cast<size_t>(L.n_tiles) * PW;
   const int wt = tile * PW + threadIdx.x / 64;
   const double w = weights[site];
-  for (int r = 0; r < n_root; ++r) {
-    const int slot = root_slots[r];
+  for (int rr = 0; rr < n_root; ++rr) {
+    const int slot = root_slots[rr];
+    const int r = root_index ? root_index[rr] : rr;
     double x[S];
     load_clv<S>(clv + (static_cast<size_t>(slot) * n + site) * (C * S) + c * S, x);
     double t = 0.0;
@@ -2827,17 +2828,20 @@ void launch_forward_level(Engine &e, const DevOp *ops, int n_ops, const LevelChu
   NRX_HIP(hipGetLastError());
 }
 
-void launch_root_partials(Engine &e, bool seeds) {
-  const int nr = static_cast<int>(e.root_slots.size());
+void launch_root_partials(Engine &e, bool seeds, const int32_t *rows, const int32_t *index, int n) {
+  if (n == 0) return;
   dispatch_sc(e.S, e.C, [&](auto s, auto c) {
     constexpr int S = decltype(s)::value, C = decltype(c)::value;
-    hipLaunchKernelGGL((k_root_partials<S, C>), dim3(e.L.n_tiles), dim3(TILE * C), 0, e.stream,
-                       e.d_root_slots.as<int32_t>(), nr, e.L, e.d_clv.as<double>(),
-                       e.d_scl.as<int32_t>(), e.d_freqs.as<double>(), e.d_weights.as<double>(),
-                       e.d_tile_part.as<int32_t>(), e.d_root_partials.as<double>(),
-                       seeds ? e.d_site_seed.as<double>() : nullptr);
+    hipLaunchKernelGGL((k_root_partials<S, C>), dim3(e.L.n_tiles), dim3(TILE * C), 0, e.stream, rows, n, e.L,
+                       e.d_clv.as<double>(), e.d_scl.as<int32_t>(), e.d_freqs.as<double>(),
+                       e.d_weights.as<double>(), e.d_tile_part.as<int32_t>(), e.d_root_partials.as<double>(),
+                       seeds ? e.d_site_seed.as<double>() : nullptr, index);
   });
   NRX_HIP(hipGetLastError());
+}
+
+void launch_root_partials(Engine &e, bool seeds) {
+  launch_root_partials(e, seeds, e.d_root_slots.as<int32_t>(), nullptr, static_cast<int>(e.root_slots.size()));
 }
 
 void launch_mixture(Engine &e, int variant, int pw) {
@@ -2854,7 +2858,9 @@ void launch_mixture(Engine &e, int variant, int pw) {
   NRX_HIP(hipGetLastError());
 }
 
-void launch_adjoint_level(Engine &e, const LevelChunk &ch) {
+void launch_adjoint_level(Engine &e, const LevelChunk &ch) { launch_adjoint_level(e, ch, e.d_ops.as<DevOp>()); }
+
+void launch_adjoint_level(Engine &e, const LevelChunk &ch, const DevOp *ops) {
   const int n_ops = ch.op_end - ch.op_begin;
   if (n_ops == 0) return;
   dispatch_sc(e.S, e.C, [&](auto s, auto c) {
@@ -2863,7 +2869,7 @@ void launch_adjoint_level(Engine &e, const LevelChunk &ch) {
     const int n_staged = S == 4 ? std::min(n_ops, plain_staged_ops(C)) : 0;
     const size_t lds = S == 4 ? sizeof(double) * (n_staged * 2 + 1) * C * S * S : 0;
     hipLaunchKernelGGL((k_adjoint_level<S, C>), dim3(e.L.n_tiles_grid), dim3(TILE * C), lds, e.stream,
-                       e.d_ops.as<DevOp>() + ch.op_begin, n_ops, n_staged, e.L, e.d_clv.as<double>(),
+                       ops + ch.op_begin, n_ops, n_staged, e.L, e.d_clv.as<double>(),
                        e.d_scl.as<int32_t>(), e.d_adj.as<double>(), e.d_tips.as<TT>(),
                        e.d_P.as<double>(), e.d_Qd.as<double>(), e.d_freqs.as<double>(),
                        e.d_site_seed.as<double>(), e.d_tile_part.as<int32_t>(), e.d_beta.as<double>(),

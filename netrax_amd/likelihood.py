@@ -225,6 +225,18 @@ class AnnotatedNetwork:
         _lib.check(self.lib.nrx_set_category_rates(self.handle, i, _dptr(r)))
         self.cached_logl = None
 
+    def set_memory_budget(self, nbytes: int) -> None:
+        """Bytes for the CLV, adjoint and scaler rows of the FP64 form (0:
+        all resident); see nrx_set_memory_budget."""
+        _lib.check(self.lib.nrx_set_memory_budget(self.handle, int(nbytes)))
+        self.cached_logl = None
+
+    def memory_plan(self) -> dict:
+        mp = _lib.MemoryPlan()
+        _lib.check(self.lib.nrx_memory_plan(self.handle, C.byref(mp)))
+        return dict(segments=mp.segments, rows=mp.rows, boundary_rows=mp.boundary_rows,
+                    resident_rows=mp.resident_rows, row_bytes=mp.row_bytes)
+
     def branch_lengths(self) -> np.ndarray:
         """The engine's (clamped) branch lengths."""
         b = np.zeros(self.network.n_edges)
