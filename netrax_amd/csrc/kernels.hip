@@ -804,6 +804,212 @@ __global__ void __launch_bounds__(TILE *C, S == 4 ? 3 : 1)
   }
 }
 
+// ------------------------------------------- 20 states, FP64 plain form
+// One wavefront per rate category: lane l of wave c holds site l of the
+// 64-site tile in category c.  Every P (and r_c Q) element a wave reads is
+// then the same for all of its lanes, so the matrices come in through the
+// scalar data path as SGPR operands of the FMAs -- no vector load and no
+// LDS read per matrix element, which is what bounds a (site, category) per
+// lane mapping at 20 states (400 elements per matrix).  The per-site
+// quantity that spans categories (the scaling maximum) crosses the waves
+// through LDS.  The CLV layout is the same as every other FP64 kernel's.
+// A matrix the kernel only reads, addressed uniformly across the wave: in
+// the constant address space its loads are scalar loads.
+using cmat = const __attribute__((address_space(4))) double *;
+__device__ __forceinline__ cmat as_cmat(const double *p) { return (cmat)p; }
+
+__device__ __forceinline__ void load_row20(const double *__restrict__ p, double (&x)[20]) {
+#pragma unroll
+  for (int i = 0; i < 10; ++i) {
+    const double2 v = reinterpret_cast<const double2 *>(p)[i];
+    x[2 * i] = v.x;
+    x[2 * i + 1] = v.y;
+  }
+}
+
+__device__ __forceinline__ void store_row20(double *__restrict__ p, const double (&x)[20]) {
+#pragma unroll
+  for (int i = 0; i < 10; ++i) reinterpret_cast<double2 *>(p)[i] = make_double2(x[2 * i], x[2 * i + 1]);
+}
+
+// Workgroup (tile, op): the op's CLV rows of one 64-site tile, C waves.
+template <int C>
+__global__ void __launch_bounds__(TILE *C)
+    k_forward_level_aa(const DevOp *__restrict__ ops, DevLayout L, double *__restrict__ clv,
+                       int32_t *__restrict__ scl, const uint32_t *__restrict__ tips, const double *__restrict__ P,
+                       const int32_t *__restrict__ tile_part) {
+  constexpr int S = 20;
+  __shared__ double smax[C][64];
+  const int tile = blockIdx.x;
+  if (tile >= L.n_tiles) return;  // grid padded to a multiple of 8 tiles
+  const DevOp op = ops[blockIdx.y];
+  const int part = tile_part[tile];
+  const int c = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63;
+  const int64_t site = static_cast<int64_t>(tile) * TILE + lane;
+  const int64_t n = L.n_sites_pad;
+  const size_t row = static_cast<size_t>(C * S);
+  // loads of both children first
+  double x[2][S];
+  int sc = 0;
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    if (k < op.nk) {
+      const int32_t kid = op.kid[k];
+      if (kid >= 0) {
+        load_row20(clv + (static_cast<size_t>(kid) * n + site) * row + c * S, x[k]);
+        sc += scl[static_cast<size_t>(kid) * n + site];
+      } else {
+        tip_vec<S>(tips[static_cast<size_t>(-kid - 1) * n + site], x[k]);
+      }
+    }
+  }
+  double v[S];
+#pragma unroll
+  for (int i = 0; i < S; ++i) v[i] = 1.0;
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    if (k < op.nk) {
+      const cmat Pk = as_cmat(P + ((static_cast<size_t>(part) * L.n_edges + op.edge[k]) * C + c) * (S * S));
+#pragma unroll
+      for (int i = 0; i < S; ++i) {
+        double s = 0.0;
+#pragma unroll
+        for (int jj = 0; jj < S; ++jj) s = fma(Pk[i * S + jj], x[k][jj], s);
+        v[i] *= s;
+      }
+    }
+  }
+  double m = v[0];
+#pragma unroll
+  for (int i = 1; i < S; ++i) m = fmax(m, v[i]);
+  smax[c][lane] = m;
+  __syncthreads();
+  double mm = smax[0][lane];
+#pragma unroll
+  for (int cc = 1; cc < C; ++cc) mm = fmax(mm, smax[cc][lane]);
+  if (mm < SCALE_THRESHOLD) {
+#pragma unroll
+    for (int i = 0; i < S; ++i) v[i] *= SCALE_FACTOR;
+    sc += 1;
+  }
+  store_row20(clv + (static_cast<size_t>(op.out) * n + site) * row + c * S, v);
+  if (c == 0) scl[static_cast<size_t>(op.out) * n + site] = sc;
+}
+
+// Reverse sweep of one level (push form, as k_adjoint_level): one
+// workgroup per tile walks the level's ops in order; per op and child k,
+// z_k = Abar F y_other, dJ/dt_k += z_k . (r_c Q y_k), Abar_k += P_k^T z_k,
+// formed row by row of P so that z never has to be held whole.
+template <int C>
+__global__ void __launch_bounds__(TILE *C)
+    k_adjoint_level_aa(const DevOp *__restrict__ ops, int n_ops, DevLayout L, const double *__restrict__ clv,
+                       const int32_t *__restrict__ scl, double *__restrict__ adj,
+                       const uint32_t *__restrict__ tips, const double *__restrict__ P,
+                       const double *__restrict__ Qd, const double *__restrict__ freqs,
+                       const double *__restrict__ seed, const int32_t *__restrict__ tile_part,
+                       const double *__restrict__ beta, int n_root, int n_uses,
+                       double *__restrict__ grad_partials) {
+  constexpr int S = 20;
+  constexpr int PW = TILE * C / 64;
+  const int tile = blockIdx.x;
+  if (tile >= L.n_tiles) return;  // grid padded to a multiple of 8 tiles
+  const int part = tile_part[tile];
+  const int c = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63;
+  const int64_t site = static_cast<int64_t>(tile) * TILE + lane;
+  const int64_t n = L.n_sites_pad;
+  const size_t row = static_cast<size_t>(C * S);
+  const double *Qc = Qd + (static_cast<size_t>(part) * C + c) * (S * S);
+  for (int j = 0; j < n_ops; ++j) {
+    const DevOp op = ops[j];
+    double a[S];
+    if (op.flags & OP_HAS_CONSUMERS) {
+      load_row20(adj + (static_cast<size_t>(op.out) * n + site) * row + c * S, a);
+    } else {
+#pragma unroll
+      for (int i = 0; i < S; ++i) a[i] = 0.0;
+    }
+    const int kq = scl[static_cast<size_t>(op.out) * n + site];
+    if (op.root >= 0) {
+      // seed beta * w * (1/C) pi_i / Ltilde at a displayed-tree root
+      const double f = beta[part * n_root + op.root] * seed[static_cast<size_t>(op.root) * n + site];
+#pragma unroll
+      for (int i = 0; i < S; ++i) a[i] = fma(f, freqs[part * S + i], a[i]);
+    }
+    double y[2][S];
+    int ks = 0;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      if (k < op.nk) {
+        double x[S];
+        const int32_t kid = op.kid[k];
+        if (kid >= 0) {
+          load_row20(clv + (static_cast<size_t>(kid) * n + site) * row + c * S, x);
+          ks += scl[static_cast<size_t>(kid) * n + site];
+        } else {
+          tip_vec<S>(tips[static_cast<size_t>(-kid - 1) * n + site], x);
+        }
+        const cmat Pk = as_cmat(P + ((static_cast<size_t>(part) * L.n_edges + op.edge[k]) * C + c) * (S * S));
+#pragma unroll
+        for (int i = 0; i < S; ++i) {
+          double s = 0.0;
+#pragma unroll
+          for (int jj = 0; jj < S; ++jj) s = fma(Pk[i * S + jj], x[jj], s);
+          y[k][i] = s;
+        }
+      }
+    }
+    const double F = (kq - ks) > 0 ? SCALE_FACTOR : 1.0;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      if (k >= op.nk) continue;
+      const int32_t kid = op.kid[k];
+      cmat Pk = as_cmat(P + ((static_cast<size_t>(part) * L.n_edges + op.edge[k]) * C + c) * (S * S));
+      cmat Qk = as_cmat(Qc);
+      // opaque to the optimiser: the matrices are read again here, through
+      // the scalar cache, instead of being kept in (or hoisted out of the op
+      // loop into) 800 SGPRs each, which would spill
+      asm volatile("" : "+s"(Pk));
+      asm volatile("" : "+s"(Qk));
+      double back[S], gk = 0.0;
+      // one straight-line body per case (internal child or tip): a branch
+      // inside the row loop splits it into blocks across which the scalar
+      // registers holding matrix rows stay live and spill
+      auto rows = [&](auto with_back) {
+#pragma unroll
+        for (int i = 0; i < S; ++i) {
+          const double zi = a[i] * F * (op.nk == 2 ? y[1 - k][i] : 1.0);
+          double d = 0.0;
+#pragma unroll
+          for (int jj = 0; jj < S; ++jj) d = fma(Qk[i * S + jj], y[k][jj], d);
+          gk = fma(zi, d, gk);
+          if constexpr (decltype(with_back)::value) {
+#pragma unroll
+            for (int jj = 0; jj < S; ++jj) back[jj] = i == 0 ? Pk[jj] * zi : fma(Pk[i * S + jj], zi, back[jj]);
+          }
+        }
+      };
+      if (kid >= 0)
+        rows(std::true_type{});
+      else
+        rows(std::false_type{});
+      const double g = wave_sum(gk);
+      if (lane == 0) grad_partials[grad_index(tile, n_uses, op.gslot[k], PW, c)] = g;
+      if (kid >= 0) {
+        double *dst = adj + (static_cast<size_t>(kid) * n + site) * row + c * S;
+        if (!(op.flags & (k == 0 ? OP_FIRST_KID0 : OP_FIRST_KID1))) {
+          double acc[S];
+          load_row20(dst, acc);
+#pragma unroll
+          for (int i = 0; i < S; ++i) back[i] += acc[i];
+        }
+        store_row20(dst, back);
+      }
+    }
+  }
+}
+
 // ---------------------------------------------- model gradient (FP64)
 // dlnL / dP_{e,c} for every (op, child) use: G = sum over sites of z (x)^T
 // with z the adjoint of y = P x (recomputed from the op's stored adjoint,
@@ -2814,8 +3020,29 @@ void launch_pmatrices(Engine &e) {
   NRX_HIP(hipGetLastError());
 }
 
+template <typename F>
+void dispatch_c(int C, F &&f) {
+  switch (C) {
+    case 1: f(std::integral_constant<int, 1>{}); break;
+    case 2: f(std::integral_constant<int, 2>{}); break;
+    case 4: f(std::integral_constant<int, 4>{}); break;
+    case 8: f(std::integral_constant<int, 8>{}); break;
+    default: throw std::invalid_argument("unsupported number of rate categories");
+  }
+}
+
 void launch_forward_level(Engine &e, const DevOp *ops, int n_ops, const LevelChunk &) {
   if (n_ops == 0) return;
+  if (e.S == AA_STATES) {
+    dispatch_c(e.C, [&](auto c) {
+      constexpr int C = decltype(c)::value;
+      hipLaunchKernelGGL((k_forward_level_aa<C>), dim3(e.L.n_tiles_grid, n_ops), dim3(TILE * C), 0, e.stream,
+                         ops, e.L, e.d_clv.as<double>(), e.d_scl.as<int32_t>(), e.d_tips.as<uint32_t>(),
+                         e.d_P.as<double>(), e.d_tile_part.as<int32_t>());
+    });
+    NRX_HIP(hipGetLastError());
+    return;
+  }
   dispatch_sc(e.S, e.C, [&](auto s, auto c) {
     constexpr int S = decltype(s)::value, C = decltype(c)::value;
     constexpr int G = S == 4 ? PLAIN_G : 1;
@@ -2863,6 +3090,19 @@ void launch_adjoint_level(Engine &e, const LevelChunk &ch) { launch_adjoint_leve
 void launch_adjoint_level(Engine &e, const LevelChunk &ch, const DevOp *ops) {
   const int n_ops = ch.op_end - ch.op_begin;
   if (n_ops == 0) return;
+  if (e.S == AA_STATES) {
+    dispatch_c(e.C, [&](auto c) {
+      constexpr int C = decltype(c)::value;
+      hipLaunchKernelGGL((k_adjoint_level_aa<C>), dim3(e.L.n_tiles_grid), dim3(TILE * C), 0, e.stream,
+                         ops + ch.op_begin, n_ops, e.L, e.d_clv.as<double>(), e.d_scl.as<int32_t>(),
+                         e.d_adj.as<double>(), e.d_tips.as<uint32_t>(), e.d_P.as<double>(), e.d_Qd.as<double>(),
+                         e.d_freqs.as<double>(), e.d_site_seed.as<double>(), e.d_tile_part.as<int32_t>(),
+                         e.d_beta.as<double>(), static_cast<int>(e.root_slots.size()), e.n_uses,
+                         e.d_grad_partials.as<double>());
+    });
+    NRX_HIP(hipGetLastError());
+    return;
+  }
   dispatch_sc(e.S, e.C, [&](auto s, auto c) {
     constexpr int S = decltype(s)::value, C = decltype(c)::value;
     using TT = typename TipType<S>::type;
@@ -2889,16 +3129,6 @@ void launch_grad_reduce(Engine &e) {
 }
 
 
-template <typename F>
-void dispatch_c(int C, F &&f) {
-  switch (C) {
-    case 1: f(std::integral_constant<int, 1>{}); break;
-    case 2: f(std::integral_constant<int, 2>{}); break;
-    case 4: f(std::integral_constant<int, 4>{}); break;
-    case 8: f(std::integral_constant<int, 8>{}); break;
-    default: throw std::invalid_argument("unsupported number of rate categories");
-  }
-}
 
 // Shapes of the 4-state persistent kernels: group width G in {2, 3, 4, 8}
 // and TS in {16, 32, 64} sites per workgroup with TS C a multiple of 64.
