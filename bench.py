@@ -298,32 +298,70 @@ def bench_newton_sweep(w):
     return out
 
 
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+
+    return platform.processor() or "unknown"
+
+
 def cpu_baseline(w, sample_patterns, threads, budget_s):
+    """The oracle (C FP64 restatement, OpenMP over sites) timed on the host
+    cores: lnL + full gradient of the whole workload (every pattern), with
+    all `threads` threads (median of at least 3 runs) and with 1 thread (at
+    least 1 run, more while the time budget lasts).  `sample_patterns` > 0
+    times only the first patterns instead and scales linearly (a quick
+    look, not the reported baseline)."""
     from netrax_amd.simulate import Partition
     from oracle import oracle as O
 
-    O.set_threads(threads)
     full = sum(p.n_sites for p in w.partitions)
-    parts = []
-    for p in w.partitions:
-        k = max(1, int(round(sample_patterns * p.n_sites / full)))
-        parts.append(Partition(p.model, np.ascontiguousarray(p.masks[:, :k]),
-                               np.ascontiguousarray(p.weights[:k])))
+    parts = w.partitions
+    if sample_patterns:
+        parts = []
+        for p in w.partitions:
+            k = max(1, int(round(sample_patterns * p.n_sites / full)))
+            parts.append(Partition(p.model, np.ascontiguousarray(p.masks[:, :k]),
+                                   np.ascontiguousarray(p.weights[:k])))
     got = sum(p.n_sites for p in parts)
-    O.gradient(w.network, parts)  # warm
-    times = []
-    t_start = time.perf_counter()
-    while len(times) < 3 or (time.perf_counter() - t_start < budget_s and len(times) < 50):
-        t0 = time.perf_counter()
-        O.gradient(w.network, parts)
-        times.append(time.perf_counter() - t0)
-    med = statistics.median(times)
-    per_eval_full = med * full / got  # CPU cost is linear in patterns
-    return {"value": round(1.0 / per_eval_full, 5), "unit": "evals/s", "cores": threads,
+    scale = full / got
+
+    def timed(nthreads, min_runs, budget):
+        O.set_threads(nthreads)
+        times = []
+        t_start = time.perf_counter()
+        while len(times) < min_runs or (time.perf_counter() - t_start < budget and len(times) < 20):
+            t0 = time.perf_counter()
+            O.gradient(w.network, parts)
+            times.append(time.perf_counter() - t0)
+        return statistics.median(times) * scale, len(times)
+
+    O.set_threads(threads)
+    O.gradient(w.network, parts)  # warm (pages, thread pool)
+    t_all, n_all = timed(threads, 3, budget_s / 4)
+    t_one, n_one = timed(1, 1, budget_s / 2)
+    O.set_threads(threads)
+    return {"value": round(1.0 / t_all, 5), "unit": "evals/s", "cores": threads,
             "kind": "port",
-            "sample": (f"oracle (C FP64 restatement, OpenMP) lnL+gradient on the first {got} of "
-                       f"{full} patterns, median of {len(times)} runs = {med:.4f} s, "
-                       f"scaled linearly to all patterns")}
+            "threads_all": {"threads": threads, "evals_per_s": round(1.0 / t_all, 5),
+                            "s_per_eval": round(t_all, 4), "runs": n_all},
+            "threads_1": {"threads": 1, "evals_per_s": round(1.0 / t_one, 5),
+                          "s_per_eval": round(t_one, 4), "runs": n_one},
+            "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+            "gradient_method": ("per-tree outside pass: one forward and one reverse pass per displayed "
+                                "tree, not the reference's per-branch reroot + sumtable evaluation "
+                                "(E x (reroot + sumtable) per gradient), so this is faster than the "
+                                "reference's own CPU path would be"),
+            "sample": (f"oracle (C FP64 restatement, OpenMP over sites) lnL + full gradient on "
+                       + (f"all {full} patterns" if got == full else
+                          f"the first {got} of {full} patterns, scaled linearly")
+                       + f"; median of {n_all} runs at {threads} threads, {n_one} at 1 thread")}
 
 
 def oracle_check(w, lnl, g, gp, threads):
@@ -397,9 +435,9 @@ def main():
                     help="time only this form or section (profiling runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-oracle-check", action="store_true")
-    ap.add_argument("--cpu-sample-patterns", type=int, default=4000)
+    ap.add_argument("--cpu-sample-patterns", type=int, default=0, help="0 = every pattern")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, host cores)")
-    ap.add_argument("--cpu-budget-s", type=float, default=12.0)
+    ap.add_argument("--cpu-budget-s", type=float, default=24.0)
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the protein and candidate-batch sections")
     ap.add_argument("--candidates", type=int, default=256)

@@ -333,7 +333,10 @@ static void build_schedule(Engine &e) {
     e.chunks.push_back(ch);
   }
   {
-    std::vector<int> order;  // adjoint: chunks backwards, ops in index order
+    // adjoint: chunks backwards, each chunk's runs in order, ops in index
+    // order (the runs are contiguous ranges of the chunk)
+    split_level_runs(e.ops_all, e.chunks, e.adj_runs, e.adj_run_begin);
+    std::vector<int> order;
     for (auto it = e.chunks.rbegin(); it != e.chunks.rend(); ++it)
       for (int i = it->op_begin; i < it->op_end; ++i) order.push_back(i);
     set_adjoint_flags(e.ops_all, order, nv);
@@ -687,6 +690,26 @@ static void fetch_results(Engine &e) {
   NRX_HIP(hipStreamSynchronize(e.stream));
 }
 
+// Reverse sweep of level chunk c (plain form).  20 states: one launch per
+// run of ops without a common child, one op per workgroup, so that a level
+// fills the GPU.  4 states: one launch per chunk, a workgroup per tile
+// walking its ops (the chunk's runs are contiguous, so the walk is the
+// order the first-delivery flags were set in); per-run launches measured
+// slower there (fixed cost per launch against little work per op).
+static void adjoint_chunk(Engine &e, int c, const DevOp *ops) {
+  if (e.S == AA_STATES) {
+    for (int r = e.adj_run_begin[c]; r < e.adj_run_begin[c + 1]; ++r) {
+      record_kernel_begin(e, 1);
+      launch_adjoint_level(e, e.adj_runs[r], ops);
+      record_kernel_end(e, 1);
+    }
+  } else {
+    record_kernel_begin(e, 1);
+    launch_adjoint_level(e, e.chunks[c], ops);
+    record_kernel_end(e, 1);
+  }
+}
+
 // Forward pass, root sums and (with `gradient`) reverse sweep of the FP64
 // form under a memory budget; the mixture runs between the two.
 static void budget_forward(Engine &e, bool gradient) {
@@ -716,11 +739,7 @@ static void budget_adjoint(Engine &e) {
         launch_forward_level(e, rec + bp.rec_begin[c], n, e.chunks[c]);
         record_kernel_end(e, 0);
       }
-    for (int c = bp.seg_chunk[j + 1] - 1; c >= bp.seg_chunk[j]; --c) {
-      record_kernel_begin(e, 1);
-      launch_adjoint_level(e, e.chunks[c], ops);
-      record_kernel_end(e, 1);
-    }
+    for (int c = bp.seg_chunk[j + 1] - 1; c >= bp.seg_chunk[j]; --c) adjoint_chunk(e, c, ops);
   }
 }
 
@@ -799,11 +818,8 @@ static int evaluate(Engine &e, int variant, bool incremental, bool update_pm, bo
     launch_mixture(e, variant, e.pw);
     if (gradient) {
       e.d_adj.alloc(sizeof(double) * e.clv_elems_per_slot * e.dag.var.size());
-      for (auto it = e.chunks.rbegin(); it != e.chunks.rend(); ++it) {
-        record_kernel_begin(e, 1);
-        launch_adjoint_level(e, *it);
-        record_kernel_end(e, 1);
-      }
+      for (int c = static_cast<int>(e.chunks.size()) - 1; c >= 0; --c)
+        adjoint_chunk(e, c, e.d_ops.as<DevOp>());
       launch_grad_reduce(e);
     }
   }

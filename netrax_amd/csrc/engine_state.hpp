@@ -75,6 +75,9 @@ struct Engine : nrx_engine {
   // plain form: ops in level order, cut into launches with bounded LDS tables
   std::vector<DevOp> ops_all;
   std::vector<LevelChunk> chunks;
+  // plain form reverse sweep: runs of ops without a common child, per chunk
+  std::vector<LevelChunk> adj_runs;
+  std::vector<int> adj_run_begin;
   // batched form: groups of independent ops in a depth-first-biased order
   std::vector<DevOp> ops_dfs;
   std::vector<int32_t> group_begin;  // [n_groups + 1] into ops_dfs

@@ -661,6 +661,7 @@ __global__ void __launch_bounds__(TILE *C, S == 4 ? 3 : 1)
   constexpr int PW = TILE * C / 64;
   const int tile = blockIdx.x;
   if (tile >= L.n_tiles) return;  // grid padded to a multiple of 8 tiles
+
   const int part = tile_part[tile];
   if constexpr (S == 4)
     for (int i = threadIdx.x; i < C * S * S; i += blockDim.x)
@@ -921,6 +922,9 @@ __global__ void __launch_bounds__(TILE *C)
   const int64_t n = L.n_sites_pad;
   const size_t row = static_cast<size_t>(C * S);
   const double *Qc = Qd + (static_cast<size_t>(part) * C + c) * (S * S);
+  // one op of the run (whose ops share no child) per workgroup
+  ops += blockIdx.y;
+  n_ops = min(n_ops - static_cast<int>(blockIdx.y), 1);
   for (int j = 0; j < n_ops; ++j) {
     const DevOp op = ops[j];
     double a[S];
@@ -3093,7 +3097,7 @@ void launch_adjoint_level(Engine &e, const LevelChunk &ch, const DevOp *ops) {
   if (e.S == AA_STATES) {
     dispatch_c(e.C, [&](auto c) {
       constexpr int C = decltype(c)::value;
-      hipLaunchKernelGGL((k_adjoint_level_aa<C>), dim3(e.L.n_tiles_grid), dim3(TILE * C), 0, e.stream,
+      hipLaunchKernelGGL((k_adjoint_level_aa<C>), dim3(e.L.n_tiles_grid, n_ops), dim3(TILE * C), 0, e.stream,
                          ops + ch.op_begin, n_ops, e.L, e.d_clv.as<double>(), e.d_scl.as<int32_t>(),
                          e.d_adj.as<double>(), e.d_tips.as<uint32_t>(), e.d_P.as<double>(), e.d_Qd.as<double>(),
                          e.d_freqs.as<double>(), e.d_site_seed.as<double>(), e.d_tile_part.as<int32_t>(),

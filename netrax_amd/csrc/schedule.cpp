@@ -27,6 +27,41 @@ void set_adjoint_flags(std::vector<DevOp> &ops, const std::vector<int> &adj_orde
   }
 }
 
+void split_level_runs(std::vector<DevOp> &ops, const std::vector<LevelChunk> &chunks, std::vector<LevelChunk> &runs,
+                      std::vector<int> &run_begin) {
+  runs.clear();
+  run_begin.assign(1, 0);
+  std::vector<DevOp> left, next, ordered;
+  std::vector<int32_t> taken;
+  for (const LevelChunk &ch : chunks) {
+    left.assign(ops.begin() + ch.op_begin, ops.begin() + ch.op_end);
+    ordered.clear();
+    while (!left.empty()) {
+      LevelChunk run = ch;
+      run.op_begin = ch.op_begin + static_cast<int>(ordered.size());
+      taken.clear();
+      next.clear();
+      for (const DevOp &op : left) {
+        bool clash = false;
+        for (int k = 0; k < op.nk; ++k)
+          if (op.kid[k] >= 0 && std::find(taken.begin(), taken.end(), op.kid[k]) != taken.end()) clash = true;
+        if (clash) {
+          next.push_back(op);
+          continue;
+        }
+        for (int k = 0; k < op.nk; ++k)
+          if (op.kid[k] >= 0) taken.push_back(op.kid[k]);
+        ordered.push_back(op);
+      }
+      run.op_end = ch.op_begin + static_cast<int>(ordered.size());
+      runs.push_back(run);
+      left.swap(next);
+    }
+    std::copy(ordered.begin(), ordered.end(), ops.begin() + ch.op_begin);
+    run_begin.push_back(static_cast<int>(runs.size()));
+  }
+}
+
 // OP_ACC_EARLY*: for the FP32 form's reverse sweep (groups backwards, ops in
 // index order), a delivery that accumulates into a child adjoint last
 // written in an earlier group (not earlier in its own group), so the

@@ -15,6 +15,14 @@ namespace nrx {
 // order `adj_order` (indices into ops).
 void set_adjoint_flags(std::vector<DevOp> &ops, const std::vector<int> &adj_order, int nv);
 
+// Plain form: splits every level chunk into runs of ops that share no child
+// (first fit, in op order) and reorders the chunk's ops so that each run is
+// contiguous.  The ops of a run can deliver their adjoints concurrently;
+// runs are returned as op ranges, per chunk in order, with run_begin[c]
+// the first run of chunk c (size chunks + 1).
+void split_level_runs(std::vector<DevOp> &ops, const std::vector<LevelChunk> &chunks, std::vector<LevelChunk> &runs,
+                      std::vector<int> &run_begin);
+
 // OP_ACC_EARLY* for the FP32 reverse sweep (groups backwards).
 void set_acc_early(std::vector<DevOp> &ops, const std::vector<int32_t> &group_begin, int nv);
 

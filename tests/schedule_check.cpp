@@ -119,6 +119,86 @@ static int check(const Dag &dag, int group, int n_slots) {
   return errors;
 }
 
+// Plain form: level chunks split into runs of ops without a common child,
+// replayed as concurrent deliveries: every run's ops find their own adjoint
+// complete (all consumers delivered), a run never delivers twice to one
+// child, and the first delivery to a child (and only it) writes.
+static int check_plain(const Dag &dag) {
+  const int nv = static_cast<int>(dag.var.size());
+  int max_level = 0;
+  for (const auto &v : dag.var) max_level = std::max(max_level, v.level);
+  std::vector<DevOp> ops;
+  std::vector<LevelChunk> chunks;
+  for (int l = 1; l <= max_level; ++l) {
+    LevelChunk ch{};
+    ch.level = l;
+    ch.op_begin = static_cast<int>(ops.size());
+    for (int s = 0; s < nv; ++s)
+      if (dag.var[s].level == l && dag.var[s].nkids > 0) {
+        DevOp op{};
+        op.out = s;
+        op.nk = dag.var[s].nkids;
+        for (int k = 0; k < 2; ++k) op.kid[k] = k < op.nk ? dag.var[s].kid[k] : NO_CHILD;
+        ops.push_back(op);
+      }
+    ch.op_end = static_cast<int>(ops.size());
+    if (ch.op_end > ch.op_begin) chunks.push_back(ch);
+  }
+  const std::vector<DevOp> before = ops;
+  std::vector<LevelChunk> runs;
+  std::vector<int> run_begin;
+  split_level_runs(ops, chunks, runs, run_begin);
+  int errors = 0;
+  auto fail = [&](const std::string &what) {
+    std::printf("error plain %s\n", what.c_str());
+    ++errors;
+  };
+  if (run_begin.size() != chunks.size() + 1) fail("run table size");
+  for (size_t c = 0; c < chunks.size() && errors == 0; ++c) {
+    std::vector<int> a, b;
+    for (int i = chunks[c].op_begin; i < chunks[c].op_end; ++i) {
+      a.push_back(before[i].out);
+      b.push_back(ops[i].out);
+    }
+    std::sort(a.begin(), a.end());
+    std::sort(b.begin(), b.end());
+    if (a != b) fail("chunk ops changed");
+    int at = chunks[c].op_begin;
+    for (int r = run_begin[c]; r < run_begin[c + 1]; ++r) {
+      if (runs[r].op_begin != at || runs[r].op_end <= runs[r].op_begin) fail("runs not contiguous");
+      at = runs[r].op_end;
+    }
+    if (at != chunks[c].op_end) fail("runs do not cover the chunk");
+  }
+  std::vector<int> order;
+  for (int c = static_cast<int>(chunks.size()) - 1; c >= 0; --c)
+    for (int i = chunks[c].op_begin; i < chunks[c].op_end; ++i) order.push_back(i);
+  set_adjoint_flags(ops, order, nv);
+  std::vector<int> consumers(nv, 0), delivered(nv, 0);
+  for (const DevOp &op : ops)
+    for (int k = 0; k < op.nk; ++k)
+      if (op.kid[k] >= 0) consumers[op.kid[k]]++;
+  for (int c = static_cast<int>(chunks.size()) - 1; c >= 0 && errors == 0; --c)
+    for (int r = run_begin[c]; r < run_begin[c + 1]; ++r) {
+      std::vector<int> mine;
+      for (int i = runs[r].op_begin; i < runs[r].op_end; ++i) {
+        const DevOp &op = ops[i];
+        if (delivered[op.out] != consumers[op.out]) fail("adjoint read before all deliveries");
+        for (int k = 0; k < op.nk; ++k) {
+          const int kid = op.kid[k];
+          if (kid < 0) continue;
+          if (std::find(mine.begin(), mine.end(), kid) != mine.end()) fail("two deliveries to one child in a run");
+          mine.push_back(kid);
+          const bool first = op.flags & (k == 0 ? OP_FIRST_KID0 : OP_FIRST_KID1);
+          if (first != (delivered[kid] == 0)) fail("first-delivery flag");
+        }
+      }
+      for (int kid : mine) delivered[kid]++;
+    }
+  if (errors == 0) std::printf("ok plain_runs %zu plain_chunks %zu\n", runs.size(), chunks.size());
+  return errors;
+}
+
 int main() {
   nrx_network_desc d{};
   std::cin >> d.n_nodes >> d.n_tips >> d.root >> d.n_edges >> d.n_retic;
@@ -132,7 +212,7 @@ int main() {
   d.retic_first_edge = rf.data();
   d.retic_second_edge = rs.data();
   const Dag dag = compile_dag(d);
-  int errors = 0, group = 0, slots = 0;
+  int errors = check_plain(dag), group = 0, slots = 0;
   while (std::cin >> group >> slots) errors += check(dag, group, slots);
   return errors ? 1 : 0;
 }

@@ -2,6 +2,8 @@
 
 tests/schedule_check.cpp builds the grouped op order and the reverse
 sweep's flags for a network, then replays the sweep's memory semantics:
+the plain form's level runs (ops sharing no child, delivered concurrently:
+own adjoint complete, first deliveries) and, for the FP32 order,
 every variant once and after its children, OP_GROUP_INDEP only where the
 group before is not consumed, first deliveries, early read-backs at least
 two groups after the previous delivery, and LDS pass slots (a slot read sees
@@ -46,8 +48,8 @@ def _run(exe, net):
     r = subprocess.run([exe], input=text, capture_output=True, text=True, timeout=60)
     lines = r.stdout.strip().split("\n")
     assert r.returncode == 0 and all(l.startswith("ok") for l in lines), r.stdout
-    assert len(lines) == len(PAIRS)
-    return [dict(zip(l.split()[1::2], map(int, l.split()[2::2]))) for l in lines]
+    assert len(lines) == len(PAIRS) + 1  # the plain form's runs, then every pair
+    return [dict(zip(l.split()[1::2], map(int, l.split()[2::2]))) for l in lines[1:]]
 
 
 @pytest.mark.parametrize("name", sorted(f for f in os.listdir(SAMPLES) if f.endswith(".nw")))
