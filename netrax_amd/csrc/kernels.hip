@@ -2345,6 +2345,19 @@ __device__ __forceinline__ void adjoint_mfma_body(
           }
         }
       }
+      // every A operand of the op (P, P' and, for inner children, P^T) is
+      // requested here with the op's other loads, so the chains below wait
+      // for one round trip per op instead of one per table
+      float pa[2][AA_STEPS], pd[2][AA_STEPS], pt[2][AA_STEPS];
+#pragma unroll
+      for (int kk = 0; kk < 2; ++kk) {
+        if (kk < op.nk) {
+          const GAS float *tab = Pa + (static_cast<size_t>(op.edge[kk]) * C + c) * AA_KINDS * AA_TABLE;
+          load_a(tab, lane, pa[kk]);
+          load_a(tab + 2 * AA_TABLE, lane, pd[kk]);
+          if (op.kid[kk] >= 0) load_a(tab + AA_TABLE, lane, pt[kk]);
+        }
+      }
 #pragma unroll
       for (int kk = 0; kk < 2; ++kk)
         if (kk < op.nk && op.kid[kk] < 0) tip_b(tm[kk], h, b[kk]);
@@ -2384,24 +2397,16 @@ __device__ __forceinline__ void adjoint_mfma_body(
       }
       f32x16 y[2];
 #pragma unroll
-      for (int kk = 0; kk < 2; ++kk) {
-        if (kk < op.nk) {
-          float pa[AA_STEPS];
-          load_a(Pa + (static_cast<size_t>(op.edge[kk]) * C + c) * AA_KINDS * AA_TABLE, lane, pa);
-          y[kk] = mfma_chain(pa, b[kk]);
-        }
-      }
+      for (int kk = 0; kk < 2; ++kk)
+        if (kk < op.nk) y[kk] = mfma_chain(pa[kk], b[kk]);
       const int ez = ea + kq - ks;
 #pragma unroll
       for (int kk = 0; kk < 2; ++kk) {
         if (kk < op.nk) {
-          const GAS float *tab = Pa + (static_cast<size_t>(op.edge[kk]) * C + c) * AA_KINDS * AA_TABLE;
           float z[AA_ROWS];
 #pragma unroll
           for (int r = 0; r < AA_ROWS; ++r) z[r] = op.nk == 2 ? a[r] * y[1 - kk][r] : a[r];
-          float pd[AA_STEPS];
-          load_a(tab + 2 * AA_TABLE, lane, pd);
-          const f32x16 dy = mfma_chain(pd, b[kk]);
+          const f32x16 dy = mfma_chain(pd[kk], b[kk]);
           float gd = 0.0f;
 #pragma unroll
           for (int r = 0; r < AA_ROWS; ++r) gd = fmaf(z[r], dy[r], gd);
@@ -2417,9 +2422,7 @@ __device__ __forceinline__ void adjoint_mfma_body(
 #pragma unroll
             for (int t = 0; t < AA_STEPS; ++t) bz[t] = zw[(2 * t + h) * AA_TILE + col];
             wave_sync();
-            float pt[AA_STEPS];
-            load_a(tab + AA_TABLE, lane, pt);
-            const f32x16 bk = mfma_chain(pt, bz);
+            const f32x16 bk = mfma_chain(pt[kk], bz);
             float back[AA_ROWS];
 #pragma unroll
             for (int r = 0; r < AA_ROWS; ++r) back[r] = bk[r];

@@ -124,3 +124,13 @@ def test_budget_refused_for_fp32():
     assert ex.value.code == _lib.NRX_ERR_UNSUPPORTED
     a.set_memory_budget(0)
     a.close()
+
+
+def test_budget_protein_is_bit_identical():
+    """20 states: the reverse sweep runs per run of ops without a common
+    child; under a budget the remapped op list keeps those runs."""
+    w = make_workload("bud_aa", 18, 2, [600], seed=33, protein=True)
+    ref, bud, tried = _budget_sweep(w.network, w.partitions)
+    assert tried >= 1
+    ref.close()
+    bud.close()
