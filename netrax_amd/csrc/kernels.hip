@@ -1220,19 +1220,20 @@ __device__ __forceinline__ float cat_maxf(float m) {
   return __int_as_float(u);
 }
 
-// Sum over the 64 lanes of a wave of FP32 values, returned in FP64: four
-// row_shr steps in FP32 inside each row of 16 lanes (lane 15 of a row ends
-// with the row's sum), then the four row sums in FP64, in a fixed order.
+// Sum over the 64 lanes of a wave of FP32 values, in FP32 and a fixed order,
+// returned in FP64: four row_shr steps inside each row of 16 lanes (lane 15
+// of a row ends with the row's sum S_r), then row_bcast:15 (0x142: each row
+// adds lane 15 of the row before it, so lane 31 = S0 + S1 and lane 63 =
+// S2 + S3) and row_bcast:31 (0x143: rows 2 and 3 add lane 31), after which
+// lane 63 holds the total; one lane read instead of four.
 __device__ __forceinline__ double wave_sum_f32(float v) {
   v += dpp<0x111>(v);
   v += dpp<0x112>(v);
   v += dpp<0x114>(v);
   v += dpp<0x118>(v);
-  const int b = __float_as_int(v);
-  return (static_cast<double>(__int_as_float(__builtin_amdgcn_readlane(b, 15))) +
-          static_cast<double>(__int_as_float(__builtin_amdgcn_readlane(b, 31)))) +
-         (static_cast<double>(__int_as_float(__builtin_amdgcn_readlane(b, 47))) +
-          static_cast<double>(__int_as_float(__builtin_amdgcn_readlane(b, 63))));
+  v += dpp<0x142>(v);
+  v += dpp<0x143>(v);
+  return static_cast<double>(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63)));
 }
 
 __device__ __forceinline__ void tip_vec4(uint32_t mask, float (&x)[4]) {
