@@ -154,13 +154,18 @@ __device__ __forceinline__ double readlane(double v, int lane) {
 
 // Sum over the 64 lanes of a wave, in a fixed order (deterministic); the
 // result is the same in every lane.  Row sums by four row_shr steps (lane
-// 15 of each row of 16 ends with its row's sum), then the four row sums.
+// 15 of each row of 16 ends with its row's sum S_r), then row_bcast:15
+// (0x142: each row adds lane 15 of the row before it; lane 31 = S0 + S1,
+// lane 63 = S2 + S3) and row_bcast:31 (0x143: rows 2 and 3 add lane 31), so
+// lane 63 ends with the total: one lane read instead of four.
 __device__ __forceinline__ double wave_sum(double v) {
   v += dpp<0x111>(v);
   v += dpp<0x112>(v);
   v += dpp<0x114>(v);
   v += dpp<0x118>(v);
-  return (readlane(v, 15) + readlane(v, 31)) + (readlane(v, 47) + readlane(v, 63));
+  v += dpp<0x142>(v);
+  v += dpp<0x143>(v);
+  return readlane(v, 63);
 }
 
 // Deterministic workgroup sum; result valid in thread 0.
@@ -434,7 +439,9 @@ __global__ void __launch_bounds__(TILE *C)
   const size_t n_wt = static_cast<size_t>(L.n_tiles) * PW;
   const int wt = tile * PW + threadIdx.x / 64;
   const double w = weights[site];
-  for (int rr = 0; rr < n_root; ++rr) {
+  // one root CLV per workgroup (grid tiles x roots): the roots' site sums
+  // are independent, and a 2-D grid keeps every CU busy
+  for (int rr = blockIdx.y; rr < min(n_root, static_cast<int>(blockIdx.y) + 1); ++rr) {
     const int slot = root_slots[rr];
     const int r = root_index ? root_index[rr] : rr;
     double x[S];
@@ -3067,7 +3074,7 @@ void launch_root_partials(Engine &e, bool seeds, const int32_t *rows, const int3
   if (n == 0) return;
   dispatch_sc(e.S, e.C, [&](auto s, auto c) {
     constexpr int S = decltype(s)::value, C = decltype(c)::value;
-    hipLaunchKernelGGL((k_root_partials<S, C>), dim3(e.L.n_tiles), dim3(TILE * C), 0, e.stream, rows, n, e.L,
+    hipLaunchKernelGGL((k_root_partials<S, C>), dim3(e.L.n_tiles, n), dim3(TILE * C), 0, e.stream, rows, n, e.L,
                        e.d_clv.as<double>(), e.d_scl.as<int32_t>(), e.d_freqs.as<double>(),
                        e.d_weights.as<double>(), e.d_tile_part.as<int32_t>(), e.d_root_partials.as<double>(),
                        seeds ? e.d_site_seed.as<double>() : nullptr, index);
