@@ -2167,26 +2167,27 @@ __device__ __forceinline__ void forward_mfma_body(
         for (int r = 0; r < AA_ROWS; ++r) v[j][r] = 1.0f;
         // both children's operands as raw values first, then the chains
         float a[2][AA_STEPS], b[2][AA_STEPS];
-        int sk[2] = {0, 0};
+        int sk[2];
         uint32_t tm[2];
 #pragma unroll
         for (int kk = 0; kk < 2; ++kk) {
+          // the tip mask and the scaler are loaded for every child (tip 0
+          // and variant 0 stand in for the absent one): a value loaded on
+          // one path only would be copied at the join, which waits for
+          // every load issued so far
+          const int32_t kid = kk < op.nk ? op.kid[kk] : -1;
+          tm[kk] = tips[static_cast<size_t>(kid < 0 ? -kid - 1 : 0) * n + site];
+          sk[kk] = scl[static_cast<size_t>(kid < 0 ? 0 : kid) * n + site];
           if (kk < op.nk) {
             load_a(Pa + (static_cast<size_t>(op.edge[kk]) * C + c) * AA_KINDS * AA_TABLE, lane, a[kk]);
-            const int32_t kid = op.kid[kk];
-            if (kid >= 0) {
-              load_b(clv + kid * slot_stride + tile_off, h, b[kk]);
-              sk[kk] = scl[static_cast<size_t>(kid) * n + site];
-            } else {
-              tm[kk] = tips[static_cast<size_t>(-kid - 1) * n + site];
-            }
+            if (kid >= 0) load_b(clv + kid * slot_stride + tile_off, h, b[kk]);
           }
         }
 #pragma unroll
         for (int kk = 0; kk < 2; ++kk) {
           if (kk < op.nk) {
             if (op.kid[kk] < 0) tip_b(tm[kk], h, b[kk]);
-            ks[j] += sk[kk];
+            ks[j] += op.kid[kk] < 0 ? 0 : sk[kk];
             const f32x16 d = mfma_chain(a[kk], b[kk]);
 #pragma unroll
             for (int r = 0; r < AA_ROWS; ++r) v[j][r] *= d[r];
@@ -2339,19 +2340,18 @@ __device__ __forceinline__ void adjoint_mfma_body(
         for (int r = 0; r < AA_ROWS; ++r) a[r] = 0.0f;
       }
       float b[2][AA_STEPS];
-      int sk[2] = {0, 0};
+      int skraw[2];
       uint32_t tm[2];
 #pragma unroll
       for (int kk = 0; kk < 2; ++kk) {
-        if (kk < op.nk) {
-          const int32_t kid = op.kid[kk];
-          if (kid >= 0) {
-            load_b(clv + kid * slot_stride + tile_off, h, b[kk]);
-            sk[kk] = scl[static_cast<size_t>(kid) * n + site];
-          } else {
-            tm[kk] = tips[static_cast<size_t>(-kid - 1) * n + site];
-          }
-        }
+        // the tip mask and the scaler are loaded for every child (tip 0 and
+        // variant 0 stand in for the absent one): a value loaded on one
+        // path only would be copied at the join, which waits for every
+        // load issued so far
+        const int32_t kid = kk < op.nk ? op.kid[kk] : -1;
+        tm[kk] = tips[static_cast<size_t>(kid < 0 ? -kid - 1 : 0) * n + site];
+        skraw[kk] = scl[static_cast<size_t>(kid < 0 ? 0 : kid) * n + site];
+        if (kk < op.nk && kid >= 0) load_b(clv + kid * slot_stride + tile_off, h, b[kk]);
       }
       // every A operand of the op (P, P' and, for inner children, P^T) is
       // requested here with the op's other loads, so the chains below wait
@@ -2366,9 +2366,16 @@ __device__ __forceinline__ void adjoint_mfma_body(
           if (op.kid[kk] >= 0) load_a(tab + AA_TABLE, lane, pt[kk]);
         }
       }
+      int sk[2] = {0, 0};
 #pragma unroll
-      for (int kk = 0; kk < 2; ++kk)
-        if (kk < op.nk && op.kid[kk] < 0) tip_b(tm[kk], h, b[kk]);
+      for (int kk = 0; kk < 2; ++kk) {
+        if (kk < op.nk) {
+          if (op.kid[kk] < 0)
+            tip_b(tm[kk], h, b[kk]);
+          else
+            sk[kk] = skraw[kk];
+        }
+      }
       const int ks = sk[0] + sk[1];
       if (op.root >= 0) {
         // seed beta * w * (1/C) pi_i / Ltilde, Ltilde = (1/C) sum_c pi . x_c
