@@ -857,19 +857,28 @@ __global__ void __launch_bounds__(TILE *C)
   const int64_t site = static_cast<int64_t>(tile) * TILE + lane;
   const int64_t n = L.n_sites_pad;
   const size_t row = static_cast<size_t>(C * S);
-  // loads of both children first
+  // loads of both children first; the tip mask and the scaler are loaded
+  // for every child (tip 0 and variant 0 stand in for the absent one): a
+  // value loaded on one path only would be copied at the join, which waits
+  // for every load issued so far
   double x[2][S];
+  uint32_t tm[2];
+  int sk[2];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const int32_t kid = k < op.nk ? op.kid[k] : -1;
+    tm[k] = tips[static_cast<size_t>(kid < 0 ? -kid - 1 : 0) * n + site];
+    sk[k] = scl[static_cast<size_t>(kid < 0 ? 0 : kid) * n + site];
+    if (k < op.nk && kid >= 0) load_row20(clv + (static_cast<size_t>(kid) * n + site) * row + c * S, x[k]);
+  }
   int sc = 0;
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
     if (k < op.nk) {
-      const int32_t kid = op.kid[k];
-      if (kid >= 0) {
-        load_row20(clv + (static_cast<size_t>(kid) * n + site) * row + c * S, x[k]);
-        sc += scl[static_cast<size_t>(kid) * n + site];
-      } else {
-        tip_vec<S>(tips[static_cast<size_t>(-kid - 1) * n + site], x[k]);
-      }
+      if (op.kid[k] < 0)
+        tip_vec<S>(tm[k], x[k]);
+      else
+        sc += sk[k];
     }
   }
   double v[S];
@@ -934,17 +943,30 @@ __global__ void __launch_bounds__(TILE *C)
   n_ops = min(n_ops - static_cast<int>(blockIdx.y), 1);
   for (int j = 0; j < n_ops; ++j) {
     const DevOp op = ops[j];
-    double a[S];
-    if (op.flags & OP_HAS_CONSUMERS) {
-      load_row20(adj + (static_cast<size_t>(op.out) * n + site) * row + c * S, a);
-    } else {
-#pragma unroll
-      for (int i = 0; i < S; ++i) a[i] = 0.0;
-    }
+    // load phase: every load issued whatever the op's kind (the adjoint
+    // row of an op without consumers, seed row 0, tip 0 and variant 0 stand
+    // in for absent inputs and are dropped below), so that no join copies a
+    // loaded register and waits for the loads issued before it
+    double araw[S], x[2][S];
+    load_row20(adj + (static_cast<size_t>(op.out) * n + site) * row + c * S, araw);
     const int kq = scl[static_cast<size_t>(op.out) * n + site];
+    const double sd = seed[static_cast<size_t>(op.root >= 0 ? op.root : 0) * n + site];
+    uint32_t tm[2];
+    int sk[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const int32_t kid = k < op.nk ? op.kid[k] : -1;
+      tm[k] = tips[static_cast<size_t>(kid < 0 ? -kid - 1 : 0) * n + site];
+      sk[k] = scl[static_cast<size_t>(kid < 0 ? 0 : kid) * n + site];
+      if (k < op.nk && kid >= 0) load_row20(clv + (static_cast<size_t>(kid) * n + site) * row + c * S, x[k]);
+    }
+    const bool has_adj = (op.flags & OP_HAS_CONSUMERS) != 0;
+    double a[S];
+#pragma unroll
+    for (int i = 0; i < S; ++i) a[i] = has_adj ? araw[i] : 0.0;
     if (op.root >= 0) {
       // seed beta * w * (1/C) pi_i / Ltilde at a displayed-tree root
-      const double f = beta[part * n_root + op.root] * seed[static_cast<size_t>(op.root) * n + site];
+      const double f = beta[part * n_root + op.root] * sd;
 #pragma unroll
       for (int i = 0; i < S; ++i) a[i] = fma(f, freqs[part * S + i], a[i]);
     }
@@ -953,20 +975,16 @@ __global__ void __launch_bounds__(TILE *C)
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
       if (k < op.nk) {
-        double x[S];
-        const int32_t kid = op.kid[k];
-        if (kid >= 0) {
-          load_row20(clv + (static_cast<size_t>(kid) * n + site) * row + c * S, x);
-          ks += scl[static_cast<size_t>(kid) * n + site];
-        } else {
-          tip_vec<S>(tips[static_cast<size_t>(-kid - 1) * n + site], x);
-        }
+        if (op.kid[k] < 0)
+          tip_vec<S>(tm[k], x[k]);
+        else
+          ks += sk[k];
         const cmat Pk = as_cmat(P + ((static_cast<size_t>(part) * L.n_edges + op.edge[k]) * C + c) * (S * S));
 #pragma unroll
         for (int i = 0; i < S; ++i) {
           double s = 0.0;
 #pragma unroll
-          for (int jj = 0; jj < S; ++jj) s = fma(Pk[i * S + jj], x[jj], s);
+          for (int jj = 0; jj < S; ++jj) s = fma(Pk[i * S + jj], x[k][jj], s);
           y[k][i] = s;
         }
       }
