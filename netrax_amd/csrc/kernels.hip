@@ -2588,7 +2588,9 @@ __global__ void __launch_bounds__(TILE *C)
   const int c = threadIdx.x % C;
   const int64_t site = static_cast<int64_t>(tile) * TILE + threadIdx.x / C;
   const size_t row = static_cast<size_t>(C * S);
-  for (int j = 0; j < n_paths; ++j) {
+  // one displayed tree's path per workgroup (grid tiles x paths): the path
+  // walks are independent, and one per workgroup fills the GPU
+  for (int j = blockIdx.y; j < min(n_paths, static_cast<int>(blockIdx.y) + 1); ++j) {
     double stv[S];
     path_sumtable<S, C, T>(steps, path_begin[j], path_begin[j + 1], ops_by_slot, L, part, site, c, clv, scl,
                            scale_unit, tips, P, V, Vinv, freqs, stv);
@@ -2651,7 +2653,8 @@ __global__ void __launch_bounds__(TILE *C)
     lr[k] = ev[part * S + k] * r;
     ex[k] = exp(lr[k] * brlen);
   }
-  for (int j = 0; j < n_paths; ++j) {
+  // one path per workgroup (grid tiles x paths)
+  for (int j = blockIdx.y; j < min(n_paths, static_cast<int>(blockIdx.y) + 1); ++j) {
     const double *in = st + (static_cast<size_t>(j) * n + site) * row + c * S;
     double stv[S], v[3];
 #pragma unroll
@@ -2697,7 +2700,8 @@ __global__ void __launch_bounds__(TILE *C)
     lr[k] = ev[part * S + k] * r;
     ex[k] = exp(lr[k] * brlen);
   }
-  for (int j = 0; j < n_paths; ++j) {
+  // one path per workgroup (grid tiles x the largest path count)
+  for (int j = blockIdx.y; j < min(n_paths, static_cast<int>(blockIdx.y) + 1); ++j) {
     const double *in = st + (static_cast<size_t>(j) * n + site) * row + c * S;
     double stv[S], v[3];
 #pragma unroll
@@ -2733,9 +2737,34 @@ __device__ __forceinline__ double nr_tree_logl(const double *sums, const DerivCl
   return ll;
 }
 
-__global__ void k_newton_step(const double *__restrict__ sums, int P, int nc, const DerivClassDev *__restrict__ cls,
-                              const double *__restrict__ base, int variant, int mode, NewtonState *__restrict__ ns) {
-  if (threadIdx.x != 0 || ns->done) return;
+// The class table, the base tree lnLs and the path sums are small: the
+// workgroup copies them into LDS when they fit (NR_LDS_BYTES), so the serial
+// mixture below reads LDS instead of making a memory round trip per value.
+constexpr int NR_LDS_BYTES = 32768;
+__device__ __forceinline__ void nr_stage(const double *&sums, const DerivClassDev *&cls, const double *&base, int P,
+                                         int nc, int n_paths) {
+  const size_t n_cls = static_cast<size_t>(nc), n_base = static_cast<size_t>(P) * nc,
+               n_sums = static_cast<size_t>(n_paths) * 3 * P;
+  if (sizeof(DerivClassDev) * n_cls + sizeof(double) * (n_base + n_sums) > NR_LDS_BYTES) return;
+  DerivClassDev *s_cls = dyn_lds<DerivClassDev>();
+  double *s_base = reinterpret_cast<double *>(s_cls + n_cls), *s_sums = s_base + n_base;
+  for (size_t i = threadIdx.x; i < n_cls; i += blockDim.x) s_cls[i] = cls[i];
+  for (size_t i = threadIdx.x; i < n_base; i += blockDim.x) s_base[i] = base[i];
+  for (size_t i = threadIdx.x; i < n_sums; i += blockDim.x) s_sums[i] = sums[i];
+  __syncthreads();
+  cls = s_cls;
+  base = s_base;
+  sums = s_sums;
+}
+
+__global__ void k_newton_step(const double *__restrict__ sums_g, int P, int nc,
+                              const DerivClassDev *__restrict__ cls_g, const double *__restrict__ base_g,
+                              int variant, int mode, NewtonState *__restrict__ ns) {
+  if (ns->done) return;  // uniform: every thread reads the same flag
+  const double *sums = sums_g, *base = base_g;
+  const DerivClassDev *cls = cls_g;
+  nr_stage(sums, cls, base, P, nc, ns->n_paths);
+  if (threadIdx.x != 0) return;
   // derivatives at ns->x, mixed per partition as the host does
   double D1 = 0.0, D2 = 0.0;
   for (int p = 0; p < P; ++p) {
@@ -2831,9 +2860,12 @@ __global__ void k_newton_step(const double *__restrict__ sums, int P, int nc, co
 // network lnL at the final length vs at the start (both from the same tree
 // lnLs); a worse length is given up (optimize_branch,
 // BranchLengthOptimization.cpp:317-330)
-__global__ void k_newton_accept(const double *__restrict__ sums, int P, int nc,
-                                const DerivClassDev *__restrict__ cls, const double *__restrict__ base,
+__global__ void k_newton_accept(const double *__restrict__ sums_g, int P, int nc,
+                                const DerivClassDev *__restrict__ cls_g, const double *__restrict__ base_g,
                                 int variant, NewtonState *__restrict__ ns) {
+  const double *sums = sums_g, *base = base_g;
+  const DerivClassDev *cls = cls_g;
+  nr_stage(sums, cls, base, P, nc, ns->n_paths);
   if (threadIdx.x != 0) return;
   double l_new = 0.0, l_old = 0.0;
   for (int p = 0; p < P; ++p) {
@@ -3313,14 +3345,14 @@ void launch_sumtables(Engine &e, int n_paths) {
     constexpr int S = decltype(s)::value, C = decltype(c)::value;
     using TT = typename TipType<S>::type;
     if (e.opt.precision == NRX_FP32) {
-      hipLaunchKernelGGL((k_sumtables<S, C, float>), dim3(e.L.n_tiles), dim3(TILE * C), 0,
+      hipLaunchKernelGGL((k_sumtables<S, C, float>), dim3(e.L.n_tiles, n_paths), dim3(TILE * C), 0,
                          e.stream, e.d_steps.as<PathStep>(), e.d_path_begin.as<int32_t>(), n_paths,
                          e.d_ops_by_slot.as<DevOp>(), e.L, e.d_tile_part.as<int32_t>(),
                          e.d_clv.as<float>(), e.d_scl.as<int32_t>(), 1, e.d_tips.as<TT>(),
                          e.d_P.as<double>(), e.d_V.as<double>(), e.d_Vinv.as<double>(),
                          e.d_freqs.as<double>(), e.d_st.as<double>());
     } else {
-      hipLaunchKernelGGL((k_sumtables<S, C, double>), dim3(e.L.n_tiles), dim3(TILE * C), 0,
+      hipLaunchKernelGGL((k_sumtables<S, C, double>), dim3(e.L.n_tiles, n_paths), dim3(TILE * C), 0,
                          e.stream, e.d_steps.as<PathStep>(), e.d_path_begin.as<int32_t>(), n_paths,
                          e.d_ops_by_slot.as<DevOp>(), e.L, e.d_tile_part.as<int32_t>(),
                          e.d_clv.as<double>(), e.d_scl.as<int32_t>(), 256, e.d_tips.as<TT>(),
@@ -3335,7 +3367,7 @@ void launch_sumtable_eval(Engine &e, int n_paths, double brlen) {
   if (n_paths == 0) return;
   dispatch_sc(e.S, e.C, [&](auto s, auto c) {
     constexpr int S = decltype(s)::value, C = decltype(c)::value;
-    hipLaunchKernelGGL((k_sumtable_eval<S, C>), dim3(e.L.n_tiles), dim3(TILE * C), 0, e.stream,
+    hipLaunchKernelGGL((k_sumtable_eval<S, C>), dim3(e.L.n_tiles, n_paths), dim3(TILE * C), 0, e.stream,
                        e.d_st.as<double>(), n_paths, e.L, e.d_tile_part.as<int32_t>(),
                        e.d_eval.as<double>(), e.d_rates.as<double>(), e.d_weights.as<double>(),
                        brlen, e.d_st_partials.as<double>());
@@ -3416,7 +3448,7 @@ void enqueue_newton(Engine &e, int variant, int mode, int rounds, int max_paths,
   auto eval_reduce = [&](int final_eval) {
     dispatch_sc(e.S, e.C, [&](auto s, auto c) {
       constexpr int S = decltype(s)::value, C = decltype(c)::value;
-      hipLaunchKernelGGL((k_sumtable_eval_nr<S, C>), dim3(e.L.n_tiles), dim3(TILE * C), 0, e.stream,
+      hipLaunchKernelGGL((k_sumtable_eval_nr<S, C>), dim3(e.L.n_tiles, max_paths), dim3(TILE * C), 0, e.stream,
                          e.d_st.as<double>(), e.L, e.d_tile_part.as<int32_t>(), e.d_eval.as<double>(),
                          e.d_rates.as<double>(), e.d_weights.as<double>(), st, final_eval,
                          e.d_st_partials.as<double>());
@@ -3429,13 +3461,13 @@ void enqueue_newton(Engine &e, int variant, int mode, int rounds, int max_paths,
   };
   for (int it = 0; it < rounds; ++it) {
     eval_reduce(0);
-    hipLaunchKernelGGL(k_newton_step, dim3(1), dim3(64), 0, e.stream, e.d_st_sums.as<double>(), e.P, nc, cls,
-                       base_logl, variant, mode, st);
+    hipLaunchKernelGGL(k_newton_step, dim3(1), dim3(64), NR_LDS_BYTES, e.stream, e.d_st_sums.as<double>(), e.P,
+                       nc, cls, base_logl, variant, mode, st);
     NRX_HIP(hipGetLastError());
   }
   eval_reduce(1);
-  hipLaunchKernelGGL(k_newton_accept, dim3(1), dim3(64), 0, e.stream, e.d_st_sums.as<double>(), e.P, nc, cls,
-                     base_logl, variant, st);
+  hipLaunchKernelGGL(k_newton_accept, dim3(1), dim3(64), NR_LDS_BYTES, e.stream, e.d_st_sums.as<double>(), e.P,
+                     nc, cls, base_logl, variant, st);
   NRX_HIP(hipGetLastError());
 }
 
