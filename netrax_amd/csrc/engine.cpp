@@ -103,6 +103,7 @@ Engine::~Engine() {
   destroy_graphs(graph_exec);
   if (nr_graph) (void)hipGraphExecDestroy(nr_graph);
   if (h_nr_state) (void)hipHostFree(h_nr_state);
+  if (h_stage) (void)hipHostFree(h_stage);
   if (h_result) (void)hipHostFree(h_result);
   if (h_brlen) (void)hipHostFree(h_brlen);
   if (ev_brlen) (void)hipEventDestroy(ev_brlen);
@@ -743,7 +744,33 @@ static void budget_adjoint(Engine &e) {
   }
 }
 
-static int evaluate(Engine &e, int variant, bool incremental, bool update_pm, bool gradient) {
+// Copies v to device buffer b through the engine's pinned staging memory,
+// after the uploads staged since the last reset (e.h_stage_off = 0, made
+// only when no staged copy can be pending), without waiting.
+template <typename T>
+static void stage_upload(Engine &e, DevBuf &b, const std::vector<T> &v) {
+  size_t &off = e.h_stage_off;
+  const size_t bytes = sizeof(T) * v.size();
+  b.alloc(std::max<size_t>(bytes, 16));
+  if (bytes == 0) return;
+  if (off + bytes > e.h_stage_bytes) {
+    NRX_HIP(hipStreamSynchronize(e.stream));  // earlier copies may still read the old staging
+    if (e.h_stage) NRX_HIP(hipHostFree(e.h_stage));
+    e.h_stage = nullptr;
+    e.h_stage_bytes = 0;
+    const size_t cap = std::max<size_t>(2 * (off + bytes), 1 << 16);
+    NRX_HIP(hipHostMalloc(reinterpret_cast<void **>(&e.h_stage), cap, hipHostMallocDefault));
+    e.h_stage_bytes = cap;
+  }
+  std::memcpy(e.h_stage + off, v.data(), bytes);
+  NRX_HIP(hipMemcpyAsync(b.p, e.h_stage + off, bytes, hipMemcpyHostToDevice, e.stream));
+  off += (bytes + 255) & ~size_t(255);
+}
+
+// `fetch` false: the results stay on the device (for a caller that reads
+// them there, such as the device Newton sweep) and no host wait is made;
+// the host copy is then stale, so the cache is marked invalid.
+static int evaluate(Engine &e, int variant, bool incremental, bool update_pm, bool gradient, bool fetch = true) {
   check_ready(e);
   if (variant != NRX_AVERAGE_DISPLAYED_TREES && variant != NRX_BEST_DISPLAYED_TREE)
     throw std::invalid_argument("unknown likelihood variant");
@@ -824,6 +851,11 @@ static int evaluate(Engine &e, int variant, bool incremental, bool update_pm, bo
     }
   }
   end_timing(e);
+  if (!fetch && !results_copied) {
+    e.probs_dirty = false;
+    e.cache_valid = false;
+    return NRX_OK;
+  }
   if (results_copied)
     NRX_HIP(hipStreamSynchronize(e.stream));
   else
@@ -909,6 +941,9 @@ static void require_resident(const Engine &e) {
 // Sumtables of branch `edge` (the device-side computePartitionSumtables).
 static void build_sumtables(Engine &e, int edge) {
   require_resident(e);
+  // every caller reaches here with no staged copy pending (each derivative
+  // call and each branch of a sweep ends with a wait on the stream)
+  e.h_stage_off = 0;
   const int nc = static_cast<int>(e.dag.classes.size());
   const int nv = static_cast<int>(e.dag.var.size());
   if (e.d_ops_by_slot.p == nullptr || e.d_ops_by_slot.bytes < sizeof(DevOp) * nv)
@@ -919,8 +954,8 @@ static void build_sumtables(Engine &e, int edge) {
   const int np = static_cast<int>(begin.size()) - 1;
   e.deriv_n_paths = np;
   if (np > 0) {
-    upload(e.d_steps, steps, e.stream);
-    upload(e.d_path_begin, begin, e.stream);
+    stage_upload(e, e.d_steps, steps);
+    stage_upload(e, e.d_path_begin, begin);
     e.d_st.alloc(sizeof(double) * e.clv_elems_per_slot * np);
     e.d_st_partials.alloc(sizeof(double) * np * 3 * static_cast<size_t>(e.L.n_tiles) * e.pw);
     e.d_st_sums.alloc(sizeof(double) * np * 3 * e.P);
@@ -1727,10 +1762,10 @@ static int optimize_branches_impl(Engine &e, int variant, int mode, const int32_
     e.d_st_sums.alloc(sizeof(double) * std::max(nc, 1) * 3 * e.P);
     e.d_nr_state.alloc(sizeof(NewtonState));
     e.d_nr_cls.alloc(sizeof(DerivClassDev) * std::max(nc, 1));
-    e.d_nr_base.alloc(sizeof(double) * std::max(nc, 1) * e.P);
     if (!e.h_nr_state)
       NRX_HIP(hipHostMalloc(reinterpret_cast<void **>(&e.h_nr_state), sizeof(NewtonState), hipHostMallocDefault));
     const void *bufs[3] = {e.d_st.p, e.d_st_partials.p, e.d_st_sums.p};
+    static_assert(sizeof(e.nr_graph_bufs) == sizeof(bufs), "graph key buffers");
     // Newton rounds per graph launch: most branches converge within one
     // launch; a branch that has not is launched again
     const int rounds = std::min<int>(max_iters, 6);
@@ -1744,10 +1779,15 @@ static int optimize_branches_impl(Engine &e, int variant, int mode, const int32_
     const double start = e.cached_total;
     const double tol = e.opt.brlen_min > 0.0 ? e.opt.brlen_min / 10.0 : 1e-7;
     std::vector<DerivClassDev> cls(nc);
+    const double *base_logl = e.d_result.as<double>() + e.rl.tree_logl;  // tree lnLs, read on the device
     for (int ed : cand) {
-      rc = evaluate(e, variant, true, true, false);
+      // the branch's evaluation leaves its results on the device: the
+      // Newton kernels read the tree lnLs there, so the only host wait of
+      // a branch is the one that reads its Newton state
+      rc = evaluate(e, variant, true, true, false, /*fetch=*/false);
       if (rc != NRX_OK) return rc;
       build_sumtables(e, ed);
+      e.deriv_valid = false;  // its host-side tree lnLs are stale (results stayed on the device)
       const int np = e.deriv_n_paths;
       if (np == 0) continue;  // the branch is in no displayed tree's path: derivative 0, length kept
       std::vector<char> has(nc, 0);
@@ -1757,10 +1797,7 @@ static int optimize_branches_impl(Engine &e, int variant, int mode, const int32_
         path_of[e.deriv_class[i]] = e.deriv_path[i];
       }
       for (int k = 0; k < nc; ++k) cls[k] = DerivClassDev{e.class_logprob[k], e.class_use[k], has[k], path_of[k], 0};
-      // (the host arrays outlive the copies: the branch ends with a wait)
-      NRX_HIP(hipMemcpyAsync(e.d_nr_cls.p, cls.data(), sizeof(DerivClassDev) * nc, hipMemcpyHostToDevice, e.stream));
-      NRX_HIP(hipMemcpyAsync(e.d_nr_base.p, e.deriv_base_logl.data(), sizeof(double) * e.deriv_base_logl.size(),
-                             hipMemcpyHostToDevice, e.stream));
+      stage_upload(e, e.d_nr_cls, cls);
       NewtonState &st = *e.h_nr_state;
       st = NewtonState{};
       st.lo = e.opt.brlen_min;
@@ -1775,7 +1812,7 @@ static int optimize_branches_impl(Engine &e, int variant, int mode, const int32_
       if (!e.nr_graph) {
         e.nr_graph = capture_graph(e.stream, [&]() {
           enqueue_newton(e, variant, mode, rounds, nc, e.d_nr_state.as<NewtonState>(),
-                         e.d_nr_cls.as<DerivClassDev>(), e.d_nr_base.as<double>());
+                         e.d_nr_cls.as<DerivClassDev>(), base_logl);
         });
         std::copy(key, key + 4, e.nr_graph_key);
         std::copy(bufs, bufs + 3, e.nr_graph_bufs);

@@ -105,7 +105,7 @@ struct Engine : nrx_engine {
   DevBuf d_ops_by_slot, d_steps, d_path_begin, d_st, d_st_partials, d_st_sums;
   // device Newton-Raphson: state, class table, tree lnLs; its captured
   // sequence, valid while the sumtable buffers keep their addresses
-  DevBuf d_nr_state, d_nr_cls, d_nr_base;
+  DevBuf d_nr_state, d_nr_cls;
   DevBuf d_mg_g, d_mg_r;  // model gradient partials
   // Memory budget of the FP64 form: with n_seg > 1 the CLV, adjoint and
   // scaler rows live in n_phys physical rows.  The level chunks are cut into
@@ -128,6 +128,8 @@ struct Engine : nrx_engine {
   int nr_graph_key[4] = {-1, -1, -1, -1};  // variant, mode, max_iters, paths
   const void *nr_graph_bufs[3] = {nullptr, nullptr, nullptr};
   NewtonState *h_nr_state = nullptr;  // pinned
+  char *h_stage = nullptr;             // pinned staging for per-branch uploads
+  size_t h_stage_bytes = 0, h_stage_off = 0;
   std::vector<double> h_st_sums;
 
   // dirty tracking
