@@ -400,6 +400,10 @@ static void build_schedule(Engine &e) {
     dc[k].use = e.class_use[k];
   }
   upload(e.d_ops, e.ops_all, e.stream);
+  {
+    std::vector<LevelChunk> walk(e.chunks.rbegin(), e.chunks.rend());
+    upload(e.d_adj_walk, walk, e.stream);
+  }
   upload(e.d_ops_dfs, e.ops_dfs, e.stream);
   upload(e.d_group_begin, e.group_begin, e.stream);
   upload(e.d_edge_use_begin, e.edge_use_begin, e.stream);
@@ -691,22 +695,23 @@ static void fetch_results(Engine &e) {
   NRX_HIP(hipStreamSynchronize(e.stream));
 }
 
-// Reverse sweep of level chunk c (plain form).  20 states: one launch per
-// run of ops without a common child, one op per workgroup, so that a level
-// fills the GPU.  4 states: one launch per chunk, a workgroup per tile
-// walking its ops (the chunk's runs are contiguous, so the walk is the
-// order the first-delivery flags were set in); per-run launches measured
-// slower there (fixed cost per launch against little work per op).
-static void adjoint_chunk(Engine &e, int c, const DevOp *ops) {
+// Reverse sweep of level chunks c_hi down to c_lo (plain form).  20
+// states: one launch per run of ops without a common child, one op per
+// workgroup, so that a level fills the GPU.  4 states: one launch for all
+// of them, a workgroup per tile walking the levels and their ops in order
+// (the order the first-delivery flags were set in); per-level launches
+// cost a fixed time each against little work per op.
+static void adjoint_chunks(Engine &e, int c_hi, int c_lo, const DevOp *ops) {
   if (e.S == AA_STATES) {
-    for (int r = e.adj_run_begin[c]; r < e.adj_run_begin[c + 1]; ++r) {
-      record_kernel_begin(e, 1);
-      launch_adjoint_level(e, e.adj_runs[r], ops);
-      record_kernel_end(e, 1);
-    }
-  } else {
+    for (int c = c_hi; c >= c_lo; --c)
+      for (int r = e.adj_run_begin[c]; r < e.adj_run_begin[c + 1]; ++r) {
+        record_kernel_begin(e, 1);
+        launch_adjoint_level(e, e.adj_runs[r], ops);
+        record_kernel_end(e, 1);
+      }
+  } else if (c_hi >= c_lo) {
     record_kernel_begin(e, 1);
-    launch_adjoint_level(e, e.chunks[c], ops);
+    launch_adjoint_levels(e, ops, c_hi, c_lo);
     record_kernel_end(e, 1);
   }
 }
@@ -740,7 +745,7 @@ static void budget_adjoint(Engine &e) {
         launch_forward_level(e, rec + bp.rec_begin[c], n, e.chunks[c]);
         record_kernel_end(e, 0);
       }
-    for (int c = bp.seg_chunk[j + 1] - 1; c >= bp.seg_chunk[j]; --c) adjoint_chunk(e, c, ops);
+    adjoint_chunks(e, bp.seg_chunk[j + 1] - 1, bp.seg_chunk[j], ops);
   }
 }
 
@@ -845,8 +850,7 @@ static int evaluate(Engine &e, int variant, bool incremental, bool update_pm, bo
     launch_mixture(e, variant, e.pw);
     if (gradient) {
       e.d_adj.alloc(sizeof(double) * e.clv_elems_per_slot * e.dag.var.size());
-      for (int c = static_cast<int>(e.chunks.size()) - 1; c >= 0; --c)
-        adjoint_chunk(e, c, e.d_ops.as<DevOp>());
+      adjoint_chunks(e, static_cast<int>(e.chunks.size()) - 1, 0, e.d_ops.as<DevOp>());
       launch_grad_reduce(e);
     }
   }

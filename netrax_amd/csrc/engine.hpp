@@ -238,9 +238,11 @@ void launch_root_partials(Engine &e, bool seeds);
 // root partials of n roots: physical rows `rows`, root indices `index`
 void launch_root_partials(Engine &e, bool seeds, const int32_t *rows, const int32_t *index, int n);
 void launch_mixture(Engine &e, int variant, int partials_per_tile);
-void launch_adjoint_level(Engine &e, const LevelChunk &ch);
-// the chunk's ops taken from `ops` (same positions as in the op list)
+// 20 states: one run of ops, taken from `ops` (same positions as in the
+// op list)
 void launch_adjoint_level(Engine &e, const LevelChunk &ch, const DevOp *ops);
+// 4 states: level chunks c_hi down to c_lo in one launch
+void launch_adjoint_levels(Engine &e, const DevOp *ops, int c_hi, int c_lo);
 void launch_grad_reduce(Engine &e);
 void launch_forward_persistent(Engine &e);
 void launch_adjoint_persistent(Engine &e);

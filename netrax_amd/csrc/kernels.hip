@@ -606,13 +606,12 @@ __global__ void __launch_bounds__(256)
 // own op runs, the first one writes and later ones accumulate.  The same
 // thread owns (site, category) in every op, so accumulation needs neither
 // atomics nor barriers and is deterministic.
-// One workgroup per tile walks the level's ops in order.  The inputs of op
-// j + 1 (its own adjoint, its children and their scalers) are loaded before
-// op j delivers: no input of an op of this level is a delivery target of the
-// level (deliveries go to lower levels; CLVs are read-only here), so the
-// loads overlap the previous op's arithmetic and stores, while deliveries to
-// a child shared by several ops of the level stay in order.  The P matrices
-// of the first n_staged ops are staged in LDS.
+// One workgroup per tile walks the levels of `walk` (level chunks, top
+// level first) and each level's ops in order; since every (site, category)
+// has one owning thread in every op, a level's deliveries are read back by
+// the thread that stored them, and a single launch can cover any number of
+// levels.  The P matrices of each level's first max_staged ops are staged
+// in LDS when the walk enters the level.
 template <int S>
 struct AdjIn {
   double a[S];
@@ -651,11 +650,13 @@ __device__ __forceinline__ void adjoint_inputs(const DevOp &op, int64_t site, in
   }
 }
 
-// 4 states: at least 3 waves per SIMD (the register count the kernel
-// needs anyway); 20 states: the compiler's choice (forcing 3 would spill)
+// At least 3 waves per SIMD (the register count the kernel needs anyway):
+// a launch has one wave per 64 (site, category) pairs, about 2.5 per SIMD
+// at 41k patterns, so every wave is resident from the start.
 template <int S, int C>
 __global__ void __launch_bounds__(TILE *C, S == 4 ? 3 : 1)
-    k_adjoint_level(const DevOp *__restrict__ ops, int n_ops, int n_staged, DevLayout L,
+    k_adjoint_level(const DevOp *__restrict__ ops, const LevelChunk *__restrict__ walk, int n_walk,
+                    int max_staged, DevLayout L,
                     const double *__restrict__ clv, const int32_t *__restrict__ scl,
                     double *__restrict__ adj, const typename TipType<S>::type *__restrict__ tips,
                     const double *__restrict__ P, const double *__restrict__ Qd,
@@ -670,10 +671,11 @@ __global__ void __launch_bounds__(TILE *C, S == 4 ? 3 : 1)
   if (tile >= L.n_tiles) return;  // grid padded to a multiple of 8 tiles
 
   const int part = tile_part[tile];
-  if constexpr (S == 4)
+  if constexpr (S == 4) {
     for (int i = threadIdx.x; i < C * S * S; i += blockDim.x)
       sQ[i] = Qd[static_cast<size_t>(part) * C * S * S + i];
-  stage_ops_p<S, C>(sP, P, ops, n_staged, part, L);  // ends with a barrier (4 states)
+    __syncthreads();
+  }
   const int c = threadIdx.x % C;
   const int64_t site = static_cast<int64_t>(tile) * TILE + threadIdx.x / C;
   const int64_t n = L.n_sites_pad;
@@ -686,129 +688,122 @@ __global__ void __launch_bounds__(TILE *C, S == 4 ? 3 : 1)
   if constexpr (S == 4)
 #pragma unroll
     for (int i = 0; i < 16; ++i) Qr[i] = Qc[i];
-  // no prefetch of the next op's inputs: it bought nothing measurable and
-  // its registers hold Q instead
-  constexpr bool PREFETCH = false;
-  AdjIn<S> cur, nxt;
-  if constexpr (PREFETCH) {
-    adjoint_inputs<S, C>(ops[0], site, n, c, clv, scl, adj, tips, seed, cur);
-    wait_loads();  // both ways into the loop then carry no pending load
-  }
-  for (int j = 0; j < n_ops; ++j) {
-    const DevOp op = ops[j];
-    if constexpr (PREFETCH) {
-      if (j + 1 < n_ops) adjoint_inputs<S, C>(ops[j + 1], site, n, c, clv, scl, adj, tips, seed, nxt);
-    } else {
+  for (int w = 0; w < n_walk; ++w) {
+    const LevelChunk lv = walk[w];
+    const DevOp *__restrict__ lops = ops + lv.op_begin;
+    const int n_ops = lv.op_end - lv.op_begin;
+    const int n_staged = min(n_ops, max_staged);
+    if (S == 4 && w > 0) __syncthreads();  // every lane is done with the previous level's staged P
+    stage_ops_p<S, C>(sP, P, lops, n_staged, part, L);  // ends with a barrier (4 states)
+    for (int j = 0; j < n_ops; ++j) {
+      const DevOp op = lops[j];
+      AdjIn<S> cur;
       adjoint_inputs<S, C>(op, site, n, c, clv, scl, adj, tips, seed, cur);
-    }
-    double a[S];
-    const bool has_adj = (op.flags & OP_HAS_CONSUMERS) != 0;
+      double a[S];
+      const bool has_adj = (op.flags & OP_HAS_CONSUMERS) != 0;
 #pragma unroll
-    for (int i = 0; i < S; ++i) a[i] = has_adj ? cur.a[i] : 0.0;
-    int ks = 0;
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-      if (k < op.nk) {
-        if (op.kid[k] < 0)
-          tip_vec<S>(cur.tm[k], cur.x[k]);
-        else
-          ks += cur.sk[k];
-      }
-    }
-    if (op.root >= 0) {
-      // seed beta * w * (1/C) pi_i / Ltilde at a displayed-tree root
-      const double f = beta[part * n_root + op.root] * cur.sd;
-#pragma unroll
-      for (int i = 0; i < S; ++i) a[i] = fma(f, freqs[part * S + i], a[i]);
-    }
-    double y[2][S];
-    const double F = (cur.kq - ks) > 0 ? SCALE_FACTOR : 1.0;
-    // arithmetic of both children first, memory traffic after
-    double back[2][S], g[2];
-    if constexpr (S == 4) {
-      // y_k = P_k x_k, z_0 = a F y_1 (or a F), z_1 = a F y_0, back_k = P_k^T
-      // z_k: P_0 is read once (row i gives y_0[i], z_0[i] and row i's share
-      // of back_0), P_1 twice -- three matrix reads per op instead of four
-      const double *P0 = op_p<S, C>(sP, n_staged, P, part, L, j, 0, op.edge[0], c);
-      // (selects, not op.edge[op.nk - 1]: a runtime index would put op on the stack)
-      const double *P1 = op.nk == 2 ? op_p<S, C>(sP, n_staged, P, part, L, j, 1, op.edge[1], c) : P0;
-      double z[2][S];
-      if (op.nk == 2) matvec<S>(P1, cur.x[1], y[1]);
-#pragma unroll
-      for (int i = 0; i < S; ++i) {
-        const double2 r01 = reinterpret_cast<const double2 *>(P0 + i * S)[0];
-        const double2 r23 = reinterpret_cast<const double2 *>(P0 + i * S)[1];
-        y[0][i] = fma(r23.y, cur.x[0][3], fma(r23.x, cur.x[0][2], fma(r01.y, cur.x[0][1], r01.x * cur.x[0][0])));
-        z[0][i] = a[i] * F * (op.nk == 2 ? y[1][i] : 1.0);
-        if (i == 0) {
-          back[0][0] = r01.x * z[0][0];
-          back[0][1] = r01.y * z[0][0];
-          back[0][2] = r23.x * z[0][0];
-          back[0][3] = r23.y * z[0][0];
-        } else {
-          back[0][0] = fma(r01.x, z[0][i], back[0][0]);
-          back[0][1] = fma(r01.y, z[0][i], back[0][1]);
-          back[0][2] = fma(r23.x, z[0][i], back[0][2]);
-          back[0][3] = fma(r23.y, z[0][i], back[0][3]);
-        }
-      }
-      if (op.nk == 2) {
-#pragma unroll
-        for (int i = 0; i < S; ++i) z[1][i] = a[i] * F * y[0][i];
-        if (op.kid[1] >= 0) matvec_t<S>(P1, z[1], back[1]);
-      }
+      for (int i = 0; i < S; ++i) a[i] = has_adj ? cur.a[i] : 0.0;
+      int ks = 0;
 #pragma unroll
       for (int k = 0; k < 2; ++k) {
         if (k < op.nk) {
-          double dx[S];
-          matvec<S>(Qr, y[k], dx);
-          double gk = 0.0;
-#pragma unroll
-          for (int i = 0; i < S; ++i) gk = fma(z[k][i], dx[i], gk);
-          g[k] = wave_sum(gk);
+          if (op.kid[k] < 0)
+            tip_vec<S>(cur.tm[k], cur.x[k]);
+          else
+            ks += cur.sk[k];
         }
       }
-    } else {
+      if (op.root >= 0) {
+        // seed beta * w * (1/C) pi_i / Ltilde at a displayed-tree root
+        const double f = beta[part * n_root + op.root] * cur.sd;
 #pragma unroll
-      for (int k = 0; k < 2; ++k)
-        if (k < op.nk) matvec<S>(op_p<S, C>(sP, n_staged, P, part, L, j, k, op.edge[k], c), cur.x[k], y[k]);
-#pragma unroll
-      for (int k = 0; k < 2; ++k) {
-        if (k < op.nk) {
-          double z[S], dx[S];
-#pragma unroll
-          for (int i = 0; i < S; ++i) z[i] = a[i] * F * (op.nk == 2 ? y[1 - k][i] : 1.0);
-          matvec<S>(Qc, y[k], dx);
-          double gk = 0.0;
-#pragma unroll
-          for (int i = 0; i < S; ++i) gk = fma(z[i], dx[i], gk);
-          g[k] = wave_sum(gk);
-          if (op.kid[k] >= 0)
-            matvec_t<S>(op_p<S, C>(sP, n_staged, P, part, L, j, k, op.edge[k], c), z, back[k]);
-        }
+        for (int i = 0; i < S; ++i) a[i] = fma(f, freqs[part * S + i], a[i]);
       }
-    }
-    // op j + 1's loads have had op j's arithmetic to arrive; waiting here,
-    // before op j's stores, leaves those stores off the critical path
-    if constexpr (PREFETCH) wait_loads();
+      double y[2][S];
+      const double F = (cur.kq - ks) > 0 ? SCALE_FACTOR : 1.0;
+      // arithmetic of both children first, memory traffic after
+      double back[2][S], g[2];
+      if constexpr (S == 4) {
+        // y_k = P_k x_k, z_0 = a F y_1 (or a F), z_1 = a F y_0, back_k = P_k^T
+        // z_k: P_0 is read once (row i gives y_0[i], z_0[i] and row i's share
+        // of back_0), P_1 twice -- three matrix reads per op instead of four
+        const double *P0 = op_p<S, C>(sP, n_staged, P, part, L, j, 0, op.edge[0], c);
+        // (selects, not op.edge[op.nk - 1]: a runtime index would put op on the stack)
+        const double *P1 = op.nk == 2 ? op_p<S, C>(sP, n_staged, P, part, L, j, 1, op.edge[1], c) : P0;
+        double z[2][S];
+        if (op.nk == 2) matvec<S>(P1, cur.x[1], y[1]);
 #pragma unroll
-    for (int k = 0; k < 2; ++k) {
-      if (k < op.nk) {
-        if ((threadIdx.x & 63) == 0) grad_partials[grad_index(tile, n_uses, op.gslot[k], PW, wave)] = g[k];
-        const int32_t kid = op.kid[k];
-        if (kid >= 0) {
-          double *dst = adj + (static_cast<size_t>(kid) * n + site) * row + c * S;
-          if (!(op.flags & (k == 0 ? OP_FIRST_KID0 : OP_FIRST_KID1))) {
-            double acc[S];
-            load_clv<S>(dst, acc);
-#pragma unroll
-            for (int i = 0; i < S; ++i) back[k][i] += acc[i];
+        for (int i = 0; i < S; ++i) {
+          const double2 r01 = reinterpret_cast<const double2 *>(P0 + i * S)[0];
+          const double2 r23 = reinterpret_cast<const double2 *>(P0 + i * S)[1];
+          y[0][i] = fma(r23.y, cur.x[0][3], fma(r23.x, cur.x[0][2], fma(r01.y, cur.x[0][1], r01.x * cur.x[0][0])));
+          z[0][i] = a[i] * F * (op.nk == 2 ? y[1][i] : 1.0);
+          if (i == 0) {
+            back[0][0] = r01.x * z[0][0];
+            back[0][1] = r01.y * z[0][0];
+            back[0][2] = r23.x * z[0][0];
+            back[0][3] = r23.y * z[0][0];
+          } else {
+            back[0][0] = fma(r01.x, z[0][i], back[0][0]);
+            back[0][1] = fma(r01.y, z[0][i], back[0][1]);
+            back[0][2] = fma(r23.x, z[0][i], back[0][2]);
+            back[0][3] = fma(r23.y, z[0][i], back[0][3]);
           }
-          store_clv<S>(dst, back[k]);
+        }
+        if (op.nk == 2) {
+#pragma unroll
+          for (int i = 0; i < S; ++i) z[1][i] = a[i] * F * y[0][i];
+          if (op.kid[1] >= 0) matvec_t<S>(P1, z[1], back[1]);
+        }
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+          if (k < op.nk) {
+            double dx[S];
+            matvec<S>(Qr, y[k], dx);
+            double gk = 0.0;
+#pragma unroll
+            for (int i = 0; i < S; ++i) gk = fma(z[k][i], dx[i], gk);
+            g[k] = wave_sum(gk);
+          }
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < 2; ++k)
+          if (k < op.nk) matvec<S>(op_p<S, C>(sP, n_staged, P, part, L, j, k, op.edge[k], c), cur.x[k], y[k]);
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+          if (k < op.nk) {
+            double z[S], dx[S];
+#pragma unroll
+            for (int i = 0; i < S; ++i) z[i] = a[i] * F * (op.nk == 2 ? y[1 - k][i] : 1.0);
+            matvec<S>(Qc, y[k], dx);
+            double gk = 0.0;
+#pragma unroll
+            for (int i = 0; i < S; ++i) gk = fma(z[i], dx[i], gk);
+            g[k] = wave_sum(gk);
+            if (op.kid[k] >= 0)
+              matvec_t<S>(op_p<S, C>(sP, n_staged, P, part, L, j, k, op.edge[k], c), z, back[k]);
+          }
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        if (k < op.nk) {
+          if ((threadIdx.x & 63) == 0) grad_partials[grad_index(tile, n_uses, op.gslot[k], PW, wave)] = g[k];
+          const int32_t kid = op.kid[k];
+          if (kid >= 0) {
+            double *dst = adj + (static_cast<size_t>(kid) * n + site) * row + c * S;
+            if (!(op.flags & (k == 0 ? OP_FIRST_KID0 : OP_FIRST_KID1))) {
+              double acc[S];
+              load_clv<S>(dst, acc);
+#pragma unroll
+              for (int i = 0; i < S; ++i) back[k][i] += acc[i];
+            }
+            store_clv<S>(dst, back[k]);
+          }
         }
       }
     }
-    if constexpr (PREFETCH) cur = nxt;
   }
 }
 
@@ -3157,8 +3152,6 @@ void launch_mixture(Engine &e, int variant, int pw) {
   NRX_HIP(hipGetLastError());
 }
 
-void launch_adjoint_level(Engine &e, const LevelChunk &ch) { launch_adjoint_level(e, ch, e.d_ops.as<DevOp>()); }
-
 void launch_adjoint_level(Engine &e, const LevelChunk &ch, const DevOp *ops) {
   const int n_ops = ch.op_end - ch.op_begin;
   if (n_ops == 0) return;
@@ -3175,17 +3168,31 @@ void launch_adjoint_level(Engine &e, const LevelChunk &ch, const DevOp *ops) {
     NRX_HIP(hipGetLastError());
     return;
   }
+  throw std::logic_error("launch_adjoint_level: 4-state sweeps go through launch_adjoint_levels");
+}
+
+void launch_adjoint_levels(Engine &e, const DevOp *ops, int c_hi, int c_lo) {
+  const int nch = static_cast<int>(e.chunks.size());
+  if (e.S == AA_STATES || c_lo < 0 || c_hi >= nch || c_lo > c_hi)
+    throw std::logic_error("launch_adjoint_levels: 4-state level chunks expected");
+  int widest = 0;
+  for (int c = c_lo; c <= c_hi; ++c) widest = std::max(widest, e.chunks[c].op_end - e.chunks[c].op_begin);
+  if (widest == 0) return;
+  // the walk list holds the chunks top level first: chunk c at nch - 1 - c
+  const LevelChunk *walk = e.d_adj_walk.as<LevelChunk>() + (nch - 1 - c_hi);
+  const int n_walk = c_hi - c_lo + 1;
   dispatch_sc(e.S, e.C, [&](auto s, auto c) {
     constexpr int S = decltype(s)::value, C = decltype(c)::value;
     using TT = typename TipType<S>::type;
-    const int n_staged = S == 4 ? std::min(n_ops, plain_staged_ops(C)) : 0;
-    const size_t lds = S == 4 ? sizeof(double) * (n_staged * 2 + 1) * C * S * S : 0;
-    hipLaunchKernelGGL((k_adjoint_level<S, C>), dim3(e.L.n_tiles_grid), dim3(TILE * C), lds, e.stream,
-                       ops + ch.op_begin, n_ops, n_staged, e.L, e.d_clv.as<double>(),
-                       e.d_scl.as<int32_t>(), e.d_adj.as<double>(), e.d_tips.as<TT>(),
-                       e.d_P.as<double>(), e.d_Qd.as<double>(), e.d_freqs.as<double>(),
-                       e.d_site_seed.as<double>(), e.d_tile_part.as<int32_t>(), e.d_beta.as<double>(),
-                       static_cast<int>(e.root_slots.size()), e.n_uses, e.d_grad_partials.as<double>());
+    const int max_staged = S == 4 ? std::min(widest, plain_staged_ops(C)) : 0;
+    const size_t lds = S == 4 ? sizeof(double) * (max_staged * 2 + 1) * C * S * S : 0;
+    if constexpr (S != AA_STATES)  // 20 states take k_adjoint_level_aa (refused above)
+      hipLaunchKernelGGL((k_adjoint_level<S, C>), dim3(e.L.n_tiles_grid), dim3(TILE * C), lds, e.stream,
+                         ops, walk, n_walk, max_staged, e.L, e.d_clv.as<double>(),
+                         e.d_scl.as<int32_t>(), e.d_adj.as<double>(), e.d_tips.as<TT>(),
+                         e.d_P.as<double>(), e.d_Qd.as<double>(), e.d_freqs.as<double>(),
+                         e.d_site_seed.as<double>(), e.d_tile_part.as<int32_t>(), e.d_beta.as<double>(),
+                         static_cast<int>(e.root_slots.size()), e.n_uses, e.d_grad_partials.as<double>());
   });
   NRX_HIP(hipGetLastError());
 }
