@@ -150,6 +150,7 @@ static void compute_class_probs(Engine &e) {
 }
 
 static void upload_netdesc(Engine &e);
+static void stream_sync(Engine &e);
 
 // Bytes of one CLV row with its adjoint and scaler rows (FP64 form).
 static size_t budget_row_bytes(const Engine &e) {
@@ -403,6 +404,7 @@ static void build_schedule(Engine &e) {
   {
     std::vector<LevelChunk> walk(e.chunks.rbegin(), e.chunks.rend());
     upload(e.d_adj_walk, walk, e.stream);
+    upload(e.d_fwd_walk, e.chunks, e.stream);
   }
   upload(e.d_ops_dfs, e.ops_dfs, e.stream);
   upload(e.d_group_begin, e.group_begin, e.stream);
@@ -434,7 +436,7 @@ static void build_schedule(Engine &e) {
   e.d_site_seed.alloc(sizeof(double) * std::max<size_t>(1, rs.size()) * e.L.n_sites_pad);
   e.d_root_sums.alloc(sizeof(double) * std::max<size_t>(1, rs.size()) * e.P);
   e.d_use_sums.alloc(sizeof(double) * std::max(1, e.n_uses) * static_cast<size_t>(e.P));
-  NRX_HIP(hipStreamSynchronize(e.stream));
+  stream_sync(e);
   upload_netdesc(e);
   plan_budget(e);
   e.schedule_version++;
@@ -498,7 +500,7 @@ static void upload_classes_only(Engine &e) {
   }
   upload(e.d_classes, dc, e.stream);
   upload(e.d_class_dlogprob, e.class_dlogprob, e.stream);
-  NRX_HIP(hipStreamSynchronize(e.stream));
+  stream_sync(e);
 }
 
 static void set_probs(Engine &e, const double *p) {
@@ -625,6 +627,35 @@ static void record_kernel_end(Engine &e, int kind) {
   e.n_k[kind]++;
 }
 
+// Waits for the engine's stream.  Every copy staged through the pinned
+// staging memory has then been read, so the staging starts over.
+static void stream_sync(Engine &e) {
+  NRX_HIP(hipStreamSynchronize(e.stream));
+  e.h_stage_off = 0;
+}
+
+// Copies v to device buffer b through the engine's pinned staging memory,
+// after the copies staged since the last stream_sync, without waiting.
+template <typename T>
+static void stage_upload(Engine &e, DevBuf &b, const std::vector<T> &v) {
+  size_t &off = e.h_stage_off;
+  const size_t bytes = sizeof(T) * v.size();
+  b.alloc(std::max<size_t>(bytes, 16));
+  if (bytes == 0) return;
+  if (off + bytes > e.h_stage_bytes) {
+    const size_t cap = std::max<size_t>(2 * (off + bytes), 1 << 16);
+    stream_sync(e);  // earlier copies may still read the old staging
+    if (e.h_stage) NRX_HIP(hipHostFree(e.h_stage));
+    e.h_stage = nullptr;
+    e.h_stage_bytes = 0;
+    NRX_HIP(hipHostMalloc(reinterpret_cast<void **>(&e.h_stage), cap, hipHostMallocDefault));
+    e.h_stage_bytes = cap;
+  }
+  std::memcpy(e.h_stage + off, v.data(), bytes);
+  NRX_HIP(hipMemcpyAsync(b.p, e.h_stage + off, bytes, hipMemcpyHostToDevice, e.stream));
+  off += (bytes + 255) & ~size_t(255);
+}
+
 static void forward(Engine &e, bool incremental) {
   if (!incremental) std::fill(e.clv_dirty.begin(), e.clv_dirty.end(), 1);
   bool all = true;
@@ -633,30 +664,47 @@ static void forward(Engine &e, bool incremental) {
       all = false;
       break;
     }
+  // 4 states: one launch walking the levels (a workgroup per tile); 20
+  // states: one launch per level (one op per workgroup, so a level fills
+  // the GPU)
+  const bool walk = e.S != AA_STATES;
   if (all) {
-    for (const LevelChunk &ch : e.chunks) {
+    if (walk) {
       record_kernel_begin(e, 0);
-      launch_forward_level(e, e.d_ops.as<DevOp>() + ch.op_begin, ch.op_end - ch.op_begin, ch);
+      launch_forward_walk(e, e.d_ops.as<DevOp>(), e.d_fwd_walk.as<LevelChunk>(), static_cast<int>(e.chunks.size()));
       record_kernel_end(e, 0);
+    } else {
+      for (const LevelChunk &ch : e.chunks) {
+        record_kernel_begin(e, 0);
+        launch_forward_level(e, e.d_ops.as<DevOp>() + ch.op_begin, ch.op_end - ch.op_begin, ch);
+        record_kernel_end(e, 0);
+      }
     }
   } else {
-    // only the invalid CLVs of each launch, with the launch's edge table
+    // only the invalid CLVs, level by level
     std::vector<DevOp> dirty;
-    std::vector<int> begin(e.chunks.size() + 1, 0);
-    for (size_t c = 0; c < e.chunks.size(); ++c) {
-      begin[c] = static_cast<int>(dirty.size());
-      for (int i = e.chunks[c].op_begin; i < e.chunks[c].op_end; ++i)
+    std::vector<LevelChunk> lv;
+    for (const LevelChunk &ch : e.chunks) {
+      LevelChunk d = ch;
+      d.op_begin = static_cast<int>(dirty.size());
+      for (int i = ch.op_begin; i < ch.op_end; ++i)
         if (e.clv_dirty[e.ops_all[i].out]) dirty.push_back(e.ops_all[i]);
+      d.op_end = static_cast<int>(dirty.size());
+      if (d.op_end > d.op_begin) lv.push_back(d);
     }
-    begin[e.chunks.size()] = static_cast<int>(dirty.size());
     if (!dirty.empty()) {
-      upload(e.d_ops_dirty, dirty, e.stream);
-      for (size_t c = 0; c < e.chunks.size(); ++c) {
-        const int n = begin[c + 1] - begin[c];
-        if (n == 0) continue;
+      stage_upload(e, e.d_ops_dirty, dirty);
+      if (walk) {
+        stage_upload(e, e.d_dirty_walk, lv);
         record_kernel_begin(e, 0);
-        launch_forward_level(e, e.d_ops_dirty.as<DevOp>() + begin[c], n, e.chunks[c]);
+        launch_forward_walk(e, e.d_ops_dirty.as<DevOp>(), e.d_dirty_walk.as<LevelChunk>(), static_cast<int>(lv.size()));
         record_kernel_end(e, 0);
+      } else {
+        for (const LevelChunk &d : lv) {
+          record_kernel_begin(e, 0);
+          launch_forward_level(e, e.d_ops_dirty.as<DevOp>() + d.op_begin, d.op_end - d.op_begin, d);
+          record_kernel_end(e, 0);
+        }
       }
     }
   }
@@ -692,7 +740,7 @@ static void end_timing(Engine &e) {
 static void fetch_results(Engine &e) {
   NRX_HIP(hipMemcpyAsync(e.h_result, e.d_result.p, sizeof(double) * e.rl.size,
                          hipMemcpyDeviceToHost, e.stream));
-  NRX_HIP(hipStreamSynchronize(e.stream));
+  stream_sync(e);
 }
 
 // Reverse sweep of level chunks c_hi down to c_lo (plain form).  20
@@ -747,29 +795,6 @@ static void budget_adjoint(Engine &e) {
       }
     adjoint_chunks(e, bp.seg_chunk[j + 1] - 1, bp.seg_chunk[j], ops);
   }
-}
-
-// Copies v to device buffer b through the engine's pinned staging memory,
-// after the uploads staged since the last reset (e.h_stage_off = 0, made
-// only when no staged copy can be pending), without waiting.
-template <typename T>
-static void stage_upload(Engine &e, DevBuf &b, const std::vector<T> &v) {
-  size_t &off = e.h_stage_off;
-  const size_t bytes = sizeof(T) * v.size();
-  b.alloc(std::max<size_t>(bytes, 16));
-  if (bytes == 0) return;
-  if (off + bytes > e.h_stage_bytes) {
-    NRX_HIP(hipStreamSynchronize(e.stream));  // earlier copies may still read the old staging
-    if (e.h_stage) NRX_HIP(hipHostFree(e.h_stage));
-    e.h_stage = nullptr;
-    e.h_stage_bytes = 0;
-    const size_t cap = std::max<size_t>(2 * (off + bytes), 1 << 16);
-    NRX_HIP(hipHostMalloc(reinterpret_cast<void **>(&e.h_stage), cap, hipHostMallocDefault));
-    e.h_stage_bytes = cap;
-  }
-  std::memcpy(e.h_stage + off, v.data(), bytes);
-  NRX_HIP(hipMemcpyAsync(b.p, e.h_stage + off, bytes, hipMemcpyHostToDevice, e.stream));
-  off += (bytes + 255) & ~size_t(255);
 }
 
 // `fetch` false: the results stay on the device (for a caller that reads
@@ -861,7 +886,7 @@ static int evaluate(Engine &e, int variant, bool incremental, bool update_pm, bo
     return NRX_OK;
   }
   if (results_copied)
-    NRX_HIP(hipStreamSynchronize(e.stream));
+    stream_sync(e);
   else
     fetch_results(e);
   e.probs_dirty = false;
@@ -945,9 +970,6 @@ static void require_resident(const Engine &e) {
 // Sumtables of branch `edge` (the device-side computePartitionSumtables).
 static void build_sumtables(Engine &e, int edge) {
   require_resident(e);
-  // every caller reaches here with no staged copy pending (each derivative
-  // call and each branch of a sweep ends with a wait on the stream)
-  e.h_stage_off = 0;
   const int nc = static_cast<int>(e.dag.classes.size());
   const int nv = static_cast<int>(e.dag.var.size());
   if (e.d_ops_by_slot.p == nullptr || e.d_ops_by_slot.bytes < sizeof(DevOp) * nv)
@@ -1653,7 +1675,7 @@ int nrx_brlen_derivatives(nrx_engine *eng, int32_t variant, int32_t edge, int32_
       launch_sumtable_eval(e, np, brlen);
       NRX_HIP(hipMemcpyAsync(e.h_st_sums.data(), e.d_st_sums.p, sizeof(double) * np * 3 * e.P,
                              hipMemcpyDeviceToHost, e.stream));
-      NRX_HIP(hipStreamSynchronize(e.stream));
+      stream_sync(e);
     }
     mix_brlen_derivatives(e, variant, mode, e.deriv_base_logl, e.deriv_class, e.deriv_path, e.h_st_sums.data(),
                           d1, d2, part_d1, part_d2);
@@ -1723,7 +1745,7 @@ int nrx_brlen_derivatives_noalloc(nrx_engine *eng, int32_t variant, int32_t edge
                          reinterpret_cast<double *>(base + o_part), reinterpret_cast<double *>(base + o_sums));
       NRX_HIP(hipMemcpyAsync(sums.data(), base + o_sums, sizeof(double) * sums.size(), hipMemcpyDeviceToHost,
                              e.stream));
-      NRX_HIP(hipStreamSynchronize(e.stream));
+      stream_sync(e);
     }
     const std::vector<double> base_logl(e.h_result + e.rl.tree_logl, e.h_result + e.rl.tree_logl + e.P * nc);
     mix_brlen_derivatives(e, variant, mode, base_logl, cls, path, sums.data(), d1, d2, part_d1, part_d2);
@@ -1825,7 +1847,7 @@ static int optimize_branches_impl(Engine &e, int variant, int mode, const int32_
         NRX_HIP(hipGraphLaunch(e.nr_graph, e.stream));
         NRX_HIP(hipMemcpyAsync(e.h_nr_state, e.d_nr_state.p, sizeof(NewtonState), hipMemcpyDeviceToHost,
                                e.stream));
-        NRX_HIP(hipStreamSynchronize(e.stream));  // one wait per launch (usually one per branch)
+        stream_sync(e);  // one wait per launch (usually one per branch)
       } while (!st.done);
       const double x = st.accepted ? st.x : st.x0;
       if (x != e.brlen[ed]) {
@@ -1877,7 +1899,7 @@ int nrx_model_gradient(nrx_engine *eng, int32_t variant, double *total, double *
     std::vector<double> G(ng), Rr(nrr);
     NRX_HIP(hipMemcpyAsync(G.data(), e.d_mg_g.p, sizeof(double) * ng, hipMemcpyDeviceToHost, e.stream));
     NRX_HIP(hipMemcpyAsync(Rr.data(), e.d_mg_r.p, sizeof(double) * nrr, hipMemcpyDeviceToHost, e.stream));
-    NRX_HIP(hipStreamSynchronize(e.stream));
+    stream_sync(e);
     for (int p = 0; p < P; ++p) {
       const PartInfo &pi = e.parts[p];
       const EigenModel &m = pi.model;

@@ -234,6 +234,9 @@ int adjoint_pass_slots(int C, int G, int TS, int H, int E, int64_t n_work);
 // Kernel launchers (kernels.hip)
 void launch_pmatrices(Engine &e);
 void launch_forward_level(Engine &e, const DevOp *ops, int n_ops, const LevelChunk &ch);
+// 4 states: the levels of `walk` (device array of op ranges into `ops`,
+// lowest level first) in one launch
+void launch_forward_walk(Engine &e, const DevOp *ops, const LevelChunk *walk, int n_walk);
 void launch_root_partials(Engine &e, bool seeds);
 // root partials of n roots: physical rows `rows`, root indices `index`
 void launch_root_partials(Engine &e, bool seeds, const int32_t *rows, const int32_t *index, int n);

@@ -325,21 +325,14 @@ __device__ __forceinline__ const double *op_p(const double *sP, int n_staged,
   return P + ((static_cast<size_t>(part) * L.n_edges + edge) * C + c) * (S * S);
 }
 
-// Workgroup (tile, g) computes ops [gG, gG + G) of the launch for one tile;
-// all children of its ops are loaded before any is combined.
+// ops[0, nb) of one pruning level for one tile: all children of the ops are
+// loaded before any is combined; their P matrices are staged in sP.
 template <int S, int C, int G>
-__global__ void __launch_bounds__(TILE *C)
-    k_forward_level(const DevOp *__restrict__ ops, int n_ops, DevLayout L,
-                    double *__restrict__ clv, int32_t *__restrict__ scl,
-                    const typename TipType<S>::type *__restrict__ tips,
-                    const double *__restrict__ P, const int32_t *__restrict__ tile_part) {
-  __shared__ double sP[S == 4 ? G * 2 * C * S * S : 1];
-  const int tile = blockIdx.x;
-  if (tile >= L.n_tiles) return;  // grid padded to a multiple of 8 tiles
-  const int g0 = blockIdx.y * G;
-  const int nb = min(G, n_ops - g0);
-  const int part = tile_part[tile];
-  stage_ops_p<S, C>(sP, P, ops + g0, nb, part, L);
+__device__ __forceinline__ void forward_ops(const DevOp *__restrict__ ops, int nb, const DevLayout &L, int part,
+                                            int tile, double *__restrict__ clv, int32_t *__restrict__ scl,
+                                            const typename TipType<S>::type *__restrict__ tips,
+                                            const double *__restrict__ P, double *sP) {
+  stage_ops_p<S, C>(sP, P, ops, nb, part, L);  // ends with a barrier (4 states)
   const int c = threadIdx.x % C;
   const int64_t site = static_cast<int64_t>(tile) * TILE + threadIdx.x / C;
   const int64_t n = L.n_sites_pad;
@@ -354,7 +347,7 @@ __global__ void __launch_bounds__(TILE *C)
 #pragma unroll
       for (int k = 0; k < 2; ++k) {
         if (j < nb) {
-          const DevOp op = ops[g0 + j];
+          const DevOp op = ops[j];
           const int32_t kid = op.kid[k];
           if (k < op.nk && kid >= 0) {
             load_clv<S>(clv + (static_cast<size_t>(kid) * n + site) * (C * S) + c * S, x[j][k]);
@@ -368,7 +361,7 @@ __global__ void __launch_bounds__(TILE *C)
 #pragma unroll
       for (int k = 0; k < 2; ++k) {
         if (j < nb) {
-          const DevOp op = ops[g0 + j];
+          const DevOp op = ops[j];
           const int32_t kid = op.kid[k];
           if (k < op.nk && kid < 0) tm[j][k] = tips[static_cast<size_t>(-kid - 1) * n + site];
         }
@@ -378,7 +371,7 @@ __global__ void __launch_bounds__(TILE *C)
 #pragma unroll
     for (int j = 0; j < G; ++j) {
       if (j < nb) {
-        const DevOp op = ops[g0 + j];
+        const DevOp op = ops[j];
         int sc_j = 0;
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
@@ -414,6 +407,46 @@ __global__ void __launch_bounds__(TILE *C)
         store_clv<S>(clv + (static_cast<size_t>(op.out) * n + site) * (C * S) + c * S, v);
         if (c == 0) scl[static_cast<size_t>(op.out) * n + site] = scj;
       }
+    }
+  }
+}
+
+// Workgroup (tile, g) computes ops [gG, gG + G) of the launch for one tile.
+template <int S, int C, int G>
+__global__ void __launch_bounds__(TILE *C)
+    k_forward_level(const DevOp *__restrict__ ops, int n_ops, DevLayout L,
+                    double *__restrict__ clv, int32_t *__restrict__ scl,
+                    const typename TipType<S>::type *__restrict__ tips,
+                    const double *__restrict__ P, const int32_t *__restrict__ tile_part) {
+  __shared__ double sP[S == 4 ? G * 2 * C * S * S : 1];
+  const int tile = blockIdx.x;
+  if (tile >= L.n_tiles) return;  // grid padded to a multiple of 8 tiles
+  const int g0 = blockIdx.y * G;
+  forward_ops<S, C, G>(ops + g0, min(G, n_ops - g0), L, tile_part[tile], tile, clv, scl, tips, P, sP);
+}
+
+// The whole forward pass (or its invalid CLVs) in one launch: a workgroup
+// per tile walks the levels of `walk` (op ranges of `ops`, lowest level
+// first) and each level's ops G at a time.  A CLV and its scaler are read
+// back only by the lanes of the wave that stored them (a site's C lanes
+// share a wave), so no level boundary needs a launch.
+template <int S, int C, int G>
+__global__ void __launch_bounds__(TILE *C, 3)
+    k_forward_walk(const DevOp *__restrict__ ops, const LevelChunk *__restrict__ walk, int n_walk, DevLayout L,
+                   double *__restrict__ clv, int32_t *__restrict__ scl,
+                   const typename TipType<S>::type *__restrict__ tips,
+                   const double *__restrict__ P, const int32_t *__restrict__ tile_part) {
+  __shared__ double sP[S == 4 ? G * 2 * C * S * S : 1];
+  const int tile = blockIdx.x;
+  if (tile >= L.n_tiles) return;  // grid padded to a multiple of 8 tiles
+  const int part = tile_part[tile];
+  bool staged = false;
+  for (int w = 0; w < n_walk; ++w) {
+    const LevelChunk lv = walk[w];
+    for (int g0 = lv.op_begin; g0 < lv.op_end; g0 += G) {
+      if (S == 4 && staged) __syncthreads();  // every lane is done with the previous group's staged P
+      staged = true;
+      forward_ops<S, C, G>(ops + g0, min(G, lv.op_end - g0), L, part, tile, clv, scl, tips, P, sP);
     }
   }
 }
@@ -3118,6 +3151,21 @@ void launch_forward_level(Engine &e, const DevOp *ops, int n_ops, const LevelChu
                        dim3(TILE * C), 0, e.stream, ops, n_ops, e.L, e.d_clv.as<double>(),
                        e.d_scl.as<int32_t>(), e.d_tips.as<TT>(), e.d_P.as<double>(),
                        e.d_tile_part.as<int32_t>());
+  });
+  NRX_HIP(hipGetLastError());
+}
+
+void launch_forward_walk(Engine &e, const DevOp *ops, const LevelChunk *walk, int n_walk) {
+  if (e.S == AA_STATES) throw std::logic_error("launch_forward_walk: 20-state levels go through launch_forward_level");
+  if (n_walk == 0) return;
+  dispatch_sc(e.S, e.C, [&](auto s, auto c) {
+    constexpr int S = decltype(s)::value, C = decltype(c)::value;
+    constexpr int G = S == 4 ? PLAIN_G : 1;
+    using TT = typename TipType<S>::type;
+    if constexpr (S != AA_STATES)
+      hipLaunchKernelGGL((k_forward_walk<S, C, G>), dim3(e.L.n_tiles_grid), dim3(TILE * C), 0, e.stream, ops, walk,
+                         n_walk, e.L, e.d_clv.as<double>(), e.d_scl.as<int32_t>(), e.d_tips.as<TT>(),
+                         e.d_P.as<double>(), e.d_tile_part.as<int32_t>());
   });
   NRX_HIP(hipGetLastError());
 }
