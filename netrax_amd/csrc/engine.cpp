@@ -902,25 +902,6 @@ static int evaluate(Engine &e, int variant, bool incremental, bool update_pm, bo
 }
 
 
-// Every op by variant slot, for walking root-to-branch paths on the device.
-static std::vector<DevOp> ops_by_slot_table(const Engine &e) {
-  const int nv = static_cast<int>(e.dag.var.size());
-  std::vector<DevOp> all(nv);
-  for (int s = 0; s < nv; ++s) {
-    const Variant &v = e.dag.var[s];
-    DevOp op{};
-    op.out = s;
-    op.nk = v.nkids;
-    for (int k = 0; k < 2; ++k) {
-      op.kid[k] = k < v.nkids ? v.kid[k] : NO_CHILD;
-      op.edge[k] = k < v.nkids ? v.edge[k] : 0;
-    }
-    op.root = -1;
-    all[s] = op;
-  }
-  return all;
-}
-
 // Root-to-branch paths of branch `edge` for every used displayed tree that
 // contains it (alive edge set, as isActiveBranch + the target variant
 // existing, LikelihoodDerivatives.cpp:313-343).  cls: those classes;
@@ -944,7 +925,14 @@ static void make_deriv_paths(const Engine &e, int edge, std::vector<int> &cls, s
     std::function<bool(int)> find = [&](int slot) -> bool {
       const Variant &v = e.dag.var[slot];
       for (int c = 0; c < v.nkids; ++c) {
-        p.push_back({slot, c});
+        PathStep ps{};
+        ps.out = slot;
+        ps.child = v.kid[c];
+        ps.edge_child = v.edge[c];
+        ps.sib = v.nkids == 2 ? v.kid[1 - c] : NO_CHILD;
+        ps.edge_sib = v.nkids == 2 ? v.edge[1 - c] : 0;
+        ps.nk = v.nkids;
+        p.push_back(ps);
         if (v.edge[c] == edge) return true;
         if (v.kid[c] >= 0 && find(v.kid[c])) return true;
         p.pop_back();
@@ -952,6 +940,10 @@ static void make_deriv_paths(const Engine &e, int edge, std::vector<int> &cls, s
       return false;
     };
     if (find(e.dag.classes[k].dtroot_slot)) {
+      // the kernels take each step's own scaler from the previous step's
+      // child: the path must run down through the children it names
+      for (size_t i = 1; i < p.size(); ++i)
+        if (p[i].out != p[i - 1].child) throw std::logic_error("derivative path is not a chain of ops");
       path.push_back(static_cast<int>(begin.size()) - 1);
       steps.insert(steps.end(), p.begin(), p.end());
       begin.push_back(static_cast<int32_t>(steps.size()));
@@ -959,6 +951,12 @@ static void make_deriv_paths(const Engine &e, int edge, std::vector<int> &cls, s
       path.push_back(-1);
     }
   }
+}
+
+static int max_path_steps(const std::vector<int32_t> &begin) {
+  int m = 0;
+  for (size_t i = 1; i < begin.size(); ++i) m = std::max(m, begin[i] - begin[i - 1]);
+  return m;
 }
 
 // Branch derivatives and model gradients read the CLVs of every op after
@@ -971,9 +969,6 @@ static void require_resident(const Engine &e) {
 static void build_sumtables(Engine &e, int edge) {
   require_resident(e);
   const int nc = static_cast<int>(e.dag.classes.size());
-  const int nv = static_cast<int>(e.dag.var.size());
-  if (e.d_ops_by_slot.p == nullptr || e.d_ops_by_slot.bytes < sizeof(DevOp) * nv)
-    upload(e.d_ops_by_slot, ops_by_slot_table(e), e.stream);
   std::vector<PathStep> steps;
   std::vector<int32_t> begin;
   make_deriv_paths(e, edge, e.deriv_class, e.deriv_path, steps, begin);
@@ -986,7 +981,7 @@ static void build_sumtables(Engine &e, int edge) {
     e.d_st_partials.alloc(sizeof(double) * np * 3 * static_cast<size_t>(e.L.n_tiles) * e.pw);
     e.d_st_sums.alloc(sizeof(double) * np * 3 * e.P);
     e.h_st_sums.assign(static_cast<size_t>(np) * 3 * e.P, 0.0);
-    launch_sumtables(e, np);
+    launch_sumtables(e, np, max_path_steps(begin));
   }
   e.deriv_base_logl.assign(e.h_result + e.rl.tree_logl, e.h_result + e.rl.tree_logl + e.P * nc);
   e.deriv_edge = edge;
@@ -1705,13 +1700,11 @@ int nrx_brlen_derivatives_noalloc(nrx_engine *eng, int32_t variant, int32_t edge
     make_deriv_paths(e, edge, cls, path, steps, begin);
     const int np = static_cast<int>(begin.size()) - 1;
     const int nc = static_cast<int>(e.dag.classes.size());
-    // layout of the fixed scratch: op table, path steps and bounds, per-wave
-    // partials, per-(path, partition) sums
+    // layout of the fixed scratch: path steps and bounds, per-wave partials,
+    // per-(path, partition) sums
     require_resident(e);
-    const std::vector<DevOp> table = ops_by_slot_table(e);
     auto aligned = [](size_t b) { return (b + 255) & ~size_t(255); };
-    const size_t o_ops = 0;
-    const size_t o_steps = aligned(o_ops + sizeof(DevOp) * table.size());
+    const size_t o_steps = 0;
     const size_t o_begin = aligned(o_steps + sizeof(PathStep) * steps.size());
     const size_t o_part = aligned(o_begin + sizeof(int32_t) * begin.size());
     const size_t o_sums = aligned(o_part + sizeof(double) * np * 3 * static_cast<size_t>(e.L.n_tiles) * e.pw);
@@ -1733,15 +1726,12 @@ int nrx_brlen_derivatives_noalloc(nrx_engine *eng, int32_t variant, int32_t edge
     if (np > 0) {
       NRX_HIP(hipSetDevice(e.opt.device));
       char *base = static_cast<char *>(noalloc_scratch());
-      NRX_HIP(hipMemcpyAsync(base + o_ops, table.data(), sizeof(DevOp) * table.size(), hipMemcpyHostToDevice,
-                             e.stream));
       NRX_HIP(hipMemcpyAsync(base + o_steps, steps.data(), sizeof(PathStep) * steps.size(),
                              hipMemcpyHostToDevice, e.stream));
       NRX_HIP(hipMemcpyAsync(base + o_begin, begin.data(), sizeof(int32_t) * begin.size(),
                              hipMemcpyHostToDevice, e.stream));
       launch_deriv_fused(e, reinterpret_cast<const PathStep *>(base + o_steps),
-                         reinterpret_cast<const int32_t *>(base + o_begin), np,
-                         reinterpret_cast<const DevOp *>(base + o_ops), brlen,
+                         reinterpret_cast<const int32_t *>(base + o_begin), np, max_path_steps(begin), brlen,
                          reinterpret_cast<double *>(base + o_part), reinterpret_cast<double *>(base + o_sums));
       NRX_HIP(hipMemcpyAsync(sums.data(), base + o_sums, sizeof(double) * sums.size(), hipMemcpyDeviceToHost,
                              e.stream));

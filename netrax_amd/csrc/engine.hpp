@@ -117,11 +117,20 @@ constexpr int PLAIN_G = 4;  // forward ops per workgroup
 // adjoint: ops of a level whose P matrices are staged in LDS (48 KiB)
 constexpr int plain_staged_ops(int C) { return 49152 / (2 * C * 16 * 8); }
 
-// One step of a root-to-branch path: op slot and the child index taken.
+// One step of a root-to-branch path: the op (its output slot), the child
+// the path continues into and the other child (NO_CHILD for a one-child
+// op), with their edges; children are slots or tip codes as in DevOp.
 struct PathStep {
-  int32_t op;
-  int32_t k;
+  int32_t out, child, sib;
+  int32_t edge_child, edge_sib;
+  int32_t nk;
 };
+// P blocks of a path's first PATH_STAGED_STEPS steps are staged in LDS by
+// the path kernels (4 states)
+constexpr int PATH_STAGED_STEPS = 24;
+inline size_t path_lds_bytes(int S, int C, int max_steps) {
+  return S == 4 ? sizeof(double) * 2 * C * S * S * static_cast<size_t>(std::min(max_steps, PATH_STAGED_STEPS)) : 0;
+}
 
 // Device-side Newton-Raphson of one branch length (nrx_optimize_branches):
 // the safeguarded iteration's state, and per displayed-tree class what the
@@ -252,7 +261,7 @@ void launch_adjoint_persistent(Engine &e);
 void launch_mfma_tables(Engine &e);
 void launch_forward_mfma(Engine &e);
 void launch_adjoint_mfma(Engine &e);
-void launch_sumtables(Engine &e, int n_paths);
+void launch_sumtables(Engine &e, int n_paths, int max_steps);
 // FP64 4-state: per use, chunk and category the summed z x^T (dlnL/dP);
 // per root CLV and chunk the frequencies' direct share (a chunk is one of
 // n_split slices of a partition's tiles, chunk = partition * n_split + slice)
@@ -269,8 +278,9 @@ void enqueue_newton(Engine &e, int variant, int mode, int rounds, int max_paths,
 // bytes per GPU holds every scratch buffer of nrx_brlen_derivatives_noalloc
 constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(16) << 20;
 void *noalloc_scratch();  // its device address on the current device
-void launch_deriv_fused(Engine &e, const PathStep *steps, const int32_t *path_begin, int n_paths,
-                        const DevOp *ops_by_slot, double brlen, double *partials, double *sums);
+// max_steps: the longest path's step count (sizes the staged P in LDS)
+void launch_deriv_fused(Engine &e, const PathStep *steps, const int32_t *path_begin, int n_paths, int max_steps,
+                        double brlen, double *partials, double *sums);
 void launch_sumtable_eval(Engine &e, int n_paths, double brlen);
 
 struct HipCheck {

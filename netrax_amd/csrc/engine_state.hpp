@@ -102,7 +102,7 @@ struct Engine : nrx_engine {
   std::vector<int> deriv_path;        // per such class: sumtable row, or -1 (edge above the root)
   int deriv_n_paths = 0;
   std::vector<double> deriv_base_logl;  // [P][nc] tree lnLs when the sumtables were built
-  DevBuf d_ops_by_slot, d_steps, d_path_begin, d_st, d_st_partials, d_st_sums;
+  DevBuf d_steps, d_path_begin, d_st, d_st_partials, d_st_sums;
   // device Newton-Raphson: state, class table, tree lnLs; its captured
   // sequence, valid while the sumtable buffers keep their addresses
   DevBuf d_nr_state, d_nr_cls;

@@ -2528,30 +2528,57 @@ __device__ __forceinline__ void load_any(const float *p, double (&x)[S]) {
   for (int i = 0; i < S; ++i) x[i] = static_cast<double>(p[i]);
 }
 
-// Sumtable row of path j at (site, category): the path adjoint z walked
+// The P blocks (child edge, sibling edge; every category) of a path's first
+// n_staged steps into LDS, [step][2][C][S][S]; ends with a barrier.
+template <int S, int C>
+__device__ __forceinline__ void stage_path_p(double *sPp, const PathStep *__restrict__ steps, int n_staged,
+                                             const double *__restrict__ P, int part, const DevLayout &L) {
+  if constexpr (S == 4) {
+    constexpr int ROW = C * S * S;
+    for (int i = threadIdx.x; i < n_staged * 2 * ROW; i += blockDim.x) {
+      const int blk = i / ROW, r = i % ROW;
+      const PathStep &ps = steps[blk >> 1];
+      const int edge = (blk & 1) ? ps.edge_sib : ps.edge_child;
+      sPp[i] = P[(static_cast<size_t>(part) * L.n_edges + edge) * ROW + r];
+    }
+    __syncthreads();
+  }
+}
+
+// Sumtable row of a path at (site, category): the path adjoint z walked
 // from the displayed tree's root to the branch, then (z^T V)_k (V^-1 x)_k
-// with x the CLV below the branch.
+// with x the CLV below the branch.  The steps go in batches of PB: every
+// load of a batch (sibling rows, tip masks, scalers; slot 0 and tip 0 stand
+// in for absent ones, so no load sits on one path of a branch) is issued
+// before its arithmetic, which then only waits on LDS and registers.  A
+// step's own scaler is the previous step's child scaler.
 template <int S, int C, typename T>
 __device__ __forceinline__ void path_sumtable(const PathStep *__restrict__ steps, int b0, int b1,
-                                              const DevOp *__restrict__ ops_by_slot, const DevLayout &L,
+                                              const double *sPp, int n_staged, const DevLayout &L,
                                               int part, int64_t site, int c, const T *__restrict__ clv,
                                               const int32_t *__restrict__ scl, int scale_unit,
                                               const typename TipType<S>::type *__restrict__ tips,
                                               const double *__restrict__ P, const double *__restrict__ V,
                                               const double *__restrict__ Vinv,
                                               const double *__restrict__ freqs, double (&st)[S]) {
+  using TT = typename TipType<S>::type;
+  constexpr int PB = S == 4 ? 4 : 1;
   const int64_t n = L.n_sites_pad;
   const size_t row = static_cast<size_t>(C * S);
-  const double *Vp = V + static_cast<size_t>(part) * S * S;
-  const double *Wp = Vinv + static_cast<size_t>(part) * S * S;
-  auto prow = [&](int edge) {
+  auto slot_of = [](int32_t x) { return x >= 0 ? x : 0; };
+  auto tip_of = [](int32_t x) { return (x < 0 && x != NO_CHILD) ? -x - 1 : 0; };
+  auto prow = [&](int j, int which, int edge) -> const double * {
+    if (S == 4 && j < n_staged) return sPp + ((j * 2 + which) * C + c) * (S * S);
     return P + ((static_cast<size_t>(part) * L.n_edges + edge) * C + c) * (S * S);
   };
-  const int root = steps[b0].op;
+  const int32_t root = steps[b0].out, low = steps[b1 - 1].child;
+  double xr[S], xc[S];
+  load_any<S>(clv + (static_cast<size_t>(root) * n + site) * row + c * S, xr);
+  int kq = scl[static_cast<size_t>(root) * n + site];
+  load_any<S>(clv + (static_cast<size_t>(slot_of(low)) * n + site) * row + c * S, xc);
+  const TT tmc = tips[static_cast<size_t>(tip_of(low)) * n + site];
   double a[S];
   {
-    double xr[S];
-    load_any<S>(clv + (static_cast<size_t>(root) * n + site) * row + c * S, xr);
     double t = 0.0;
 #pragma unroll
     for (int i = 0; i < S; ++i) t = fma(freqs[part * S + i], xr[i], t);
@@ -2559,44 +2586,54 @@ __device__ __forceinline__ void path_sumtable(const PathStep *__restrict__ steps
 #pragma unroll
     for (int i = 0; i < S; ++i) a[i] = freqs[part * S + i] / (C * t);
   }
-  for (int sidx = b0; sidx < b1; ++sidx) {
-    const DevOp op = ops_by_slot[steps[sidx].op];
-    const int k = steps[sidx].k;
-    const int kq = scl[static_cast<size_t>(op.out) * n + site];
-    const int32_t child = op.kid[k];
-    const int kc = child >= 0 ? scl[static_cast<size_t>(child) * n + site] : 0;
-    double y[S];
-    int ks = 0;
-    if (op.nk == 2) {
-      double xs[S];
-      const int32_t sib = op.kid[1 - k];
-      if (sib >= 0) {
-        load_any<S>(clv + (static_cast<size_t>(sib) * n + site) * row + c * S, xs);
-        ks = scl[static_cast<size_t>(sib) * n + site];
-      } else {
-        tip_vec<S>(tips[static_cast<size_t>(-sib - 1) * n + site], xs);
+  for (int s0 = b0; s0 < b1; s0 += PB) {
+    double xs[PB][S];
+    int kc[PB], ks[PB];
+    TT tm[PB];
+#pragma unroll
+    for (int u = 0; u < PB; ++u) {
+      if (s0 + u < b1) {
+        const PathStep &ps = steps[s0 + u];
+        kc[u] = scl[static_cast<size_t>(slot_of(ps.child)) * n + site];
+        ks[u] = scl[static_cast<size_t>(slot_of(ps.sib)) * n + site];
+        tm[u] = tips[static_cast<size_t>(tip_of(ps.sib)) * n + site];
+        load_any<S>(clv + (static_cast<size_t>(slot_of(ps.sib)) * n + site) * row + c * S, xs[u]);
       }
-      matvec<S>(prow(op.edge[1 - k]), xs, y);
-    } else {
-#pragma unroll
-      for (int i = 0; i < S; ++i) y[i] = 1.0;
     }
-    const double F = ldexp(1.0, scale_unit * (kq - kc - ks));
-    double z[S];
 #pragma unroll
-    for (int i = 0; i < S; ++i) z[i] = a[i] * F * y[i];
-    if (sidx + 1 < b1) {
-      matvec_t<S>(prow(op.edge[k]), z, a);  // adjoint of the on-path child
-    } else {
-      double xc[S], u[S], v[S];
-      if (child >= 0)
-        load_any<S>(clv + (static_cast<size_t>(child) * n + site) * row + c * S, xc);
-      else
-        tip_vec<S>(tips[static_cast<size_t>(-child - 1) * n + site], xc);
-      matvec_t<S>(Vp, z, u);  // u_k = sum_i z_i V[i][k]
-      matvec<S>(Wp, xc, v);   // v_k = sum_j Vinv[k][j] x_j
+    for (int u = 0; u < PB; ++u) {
+      if (s0 + u < b1) {
+        const PathStep &ps = steps[s0 + u];
+        const int j = s0 + u - b0;
+        double y[S];
+        int ksu = 0;
+        if (ps.nk == 2) {
+          if (ps.sib < 0)
+            tip_vec<S>(tm[u], xs[u]);
+          else
+            ksu = ks[u];
+          matvec<S>(prow(j, 1, ps.edge_sib), xs[u], y);
+        } else {
 #pragma unroll
-      for (int kk = 0; kk < S; ++kk) st[kk] = u[kk] * v[kk];
+          for (int i = 0; i < S; ++i) y[i] = 1.0;
+        }
+        const int kcu = ps.child >= 0 ? kc[u] : 0;
+        const double F = ldexp(1.0, scale_unit * (kq - kcu - ksu));
+        double z[S];
+#pragma unroll
+        for (int i = 0; i < S; ++i) z[i] = a[i] * F * y[i];
+        if (s0 + u + 1 < b1) {
+          matvec_t<S>(prow(j, 0, ps.edge_child), z, a);  // adjoint of the on-path child
+          kq = kcu;
+        } else {
+          double uu[S], v[S];
+          if (low < 0) tip_vec<S>(tmc, xc);
+          matvec_t<S>(V + static_cast<size_t>(part) * S * S, z, uu);  // u_k = sum_i z_i V[i][k]
+          matvec<S>(Vinv + static_cast<size_t>(part) * S * S, xc, v);  // v_k = sum_j Vinv[k][j] x_j
+#pragma unroll
+          for (int kk = 0; kk < S; ++kk) st[kk] = uu[kk] * v[kk];
+        }
+      }
     }
   }
 }
@@ -2604,13 +2641,13 @@ __device__ __forceinline__ void path_sumtable(const PathStep *__restrict__ steps
 template <int S, int C, typename T>
 __global__ void __launch_bounds__(TILE *C)
     k_sumtables(const PathStep *__restrict__ steps, const int32_t *__restrict__ path_begin,
-                int n_paths, const DevOp *__restrict__ ops_by_slot, DevLayout L,
-                const int32_t *__restrict__ tile_part, const T *__restrict__ clv,
+                int n_paths, DevLayout L, const int32_t *__restrict__ tile_part, const T *__restrict__ clv,
                 const int32_t *__restrict__ scl, int scale_unit,
                 const typename TipType<S>::type *__restrict__ tips,
                 const double *__restrict__ P, const double *__restrict__ V,
                 const double *__restrict__ Vinv, const double *__restrict__ freqs,
                 double *__restrict__ st) {
+  double *sPp = dyn_lds<double>();
   const int tile = blockIdx.x;
   const int part = tile_part[tile];
   const int c = threadIdx.x % C;
@@ -2618,14 +2655,17 @@ __global__ void __launch_bounds__(TILE *C)
   const size_t row = static_cast<size_t>(C * S);
   // one displayed tree's path per workgroup (grid tiles x paths): the path
   // walks are independent, and one per workgroup fills the GPU
-  for (int j = blockIdx.y; j < min(n_paths, static_cast<int>(blockIdx.y) + 1); ++j) {
-    double stv[S];
-    path_sumtable<S, C, T>(steps, path_begin[j], path_begin[j + 1], ops_by_slot, L, part, site, c, clv, scl,
-                           scale_unit, tips, P, V, Vinv, freqs, stv);
-    double *out = st + (static_cast<size_t>(j) * L.n_sites_pad + site) * row + c * S;
+  const int j = blockIdx.y;
+  if (j >= n_paths) return;
+  const int b0 = path_begin[j], b1 = path_begin[j + 1];
+  const int n_staged = S == 4 ? min(b1 - b0, PATH_STAGED_STEPS) : 0;
+  stage_path_p<S, C>(sPp, steps + b0, n_staged, P, part, L);
+  double stv[S];
+  path_sumtable<S, C, T>(steps, b0, b1, sPp, n_staged, L, part, site, c, clv, scl, scale_unit, tips, P, V, Vinv,
+                         freqs, stv);
+  double *out = st + (static_cast<size_t>(j) * L.n_sites_pad + site) * row + c * S;
 #pragma unroll
-    for (int kk = 0; kk < S; ++kk) out[kk] = stv[kk];
-  }
+  for (int kk = 0; kk < S; ++kk) out[kk] = stv[kk];
 }
 
 // Per displayed tree j and wave: sums over sites of w log rho(b),
@@ -2929,7 +2969,7 @@ __global__ void k_newton_accept(const double *__restrict__ sums_g, int P, int nc
 template <int S, int C, typename T>
 __global__ void __launch_bounds__(TILE *C)
     k_deriv_fused(const PathStep *__restrict__ steps, const int32_t *__restrict__ path_begin, int n_paths,
-                  const DevOp *__restrict__ ops_by_slot, DevLayout L, const int32_t *__restrict__ tile_part,
+                  DevLayout L, const int32_t *__restrict__ tile_part,
                   const T *__restrict__ clv, const int32_t *__restrict__ scl, int scale_unit,
                   const typename TipType<S>::type *__restrict__ tips, const double *__restrict__ P,
                   const double *__restrict__ V, const double *__restrict__ Vinv,
@@ -2951,10 +2991,15 @@ __global__ void __launch_bounds__(TILE *C)
     lr[k] = ev[part * S + k] * r;
     ex[k] = exp(lr[k] * brlen);
   }
+  double *sPp = dyn_lds<double>();
   for (int j = 0; j < n_paths; ++j) {
     double stv[S], v[3];
-    path_sumtable<S, C, T>(steps, path_begin[j], path_begin[j + 1], ops_by_slot, L, part, site, c, clv, scl,
-                           scale_unit, tips, P, V, Vinv, freqs, stv);
+    const int b0 = path_begin[j], b1 = path_begin[j + 1];
+    const int n_staged = S == 4 ? min(b1 - b0, PATH_STAGED_STEPS) : 0;
+    if (S == 4 && j > 0) __syncthreads();  // every lane is done with the previous path's staged P
+    stage_path_p<S, C>(sPp, steps + b0, n_staged, P, part, L);
+    path_sumtable<S, C, T>(steps, b0, b1, sPp, n_staged, L, part, site, c, clv, scl, scale_unit, tips, P, V,
+                           Vinv, freqs, stv);
     sumtable_partials<S, C>(stv, ex, lr, w, c, v);
     if ((threadIdx.x & 63) == 0)
 #pragma unroll
@@ -3394,22 +3439,23 @@ void launch_adjoint_mfma(Engine &e) {
   NRX_HIP(hipGetLastError());
 }
 
-void launch_sumtables(Engine &e, int n_paths) {
+void launch_sumtables(Engine &e, int n_paths, int max_steps) {
   if (n_paths == 0) return;
+  const size_t lds = path_lds_bytes(e.S, e.C, max_steps);
   dispatch_sc(e.S, e.C, [&](auto s, auto c) {
     constexpr int S = decltype(s)::value, C = decltype(c)::value;
     using TT = typename TipType<S>::type;
     if (e.opt.precision == NRX_FP32) {
-      hipLaunchKernelGGL((k_sumtables<S, C, float>), dim3(e.L.n_tiles, n_paths), dim3(TILE * C), 0,
+      hipLaunchKernelGGL((k_sumtables<S, C, float>), dim3(e.L.n_tiles, n_paths), dim3(TILE * C), lds,
                          e.stream, e.d_steps.as<PathStep>(), e.d_path_begin.as<int32_t>(), n_paths,
-                         e.d_ops_by_slot.as<DevOp>(), e.L, e.d_tile_part.as<int32_t>(),
+                         e.L, e.d_tile_part.as<int32_t>(),
                          e.d_clv.as<float>(), e.d_scl.as<int32_t>(), 1, e.d_tips.as<TT>(),
                          e.d_P.as<double>(), e.d_V.as<double>(), e.d_Vinv.as<double>(),
                          e.d_freqs.as<double>(), e.d_st.as<double>());
     } else {
-      hipLaunchKernelGGL((k_sumtables<S, C, double>), dim3(e.L.n_tiles, n_paths), dim3(TILE * C), 0,
+      hipLaunchKernelGGL((k_sumtables<S, C, double>), dim3(e.L.n_tiles, n_paths), dim3(TILE * C), lds,
                          e.stream, e.d_steps.as<PathStep>(), e.d_path_begin.as<int32_t>(), n_paths,
-                         e.d_ops_by_slot.as<DevOp>(), e.L, e.d_tile_part.as<int32_t>(),
+                         e.L, e.d_tile_part.as<int32_t>(),
                          e.d_clv.as<double>(), e.d_scl.as<int32_t>(), 256, e.d_tips.as<TT>(),
                          e.d_P.as<double>(), e.d_V.as<double>(), e.d_Vinv.as<double>(),
                          e.d_freqs.as<double>(), e.d_st.as<double>());
@@ -3456,21 +3502,22 @@ void *noalloc_scratch() {
   return ptr;
 }
 
-void launch_deriv_fused(Engine &e, const PathStep *steps, const int32_t *path_begin, int n_paths,
-                        const DevOp *ops_by_slot, double brlen, double *partials, double *sums) {
+void launch_deriv_fused(Engine &e, const PathStep *steps, const int32_t *path_begin, int n_paths, int max_steps,
+                        double brlen, double *partials, double *sums) {
   if (n_paths == 0) return;
+  const size_t lds = path_lds_bytes(e.S, e.C, max_steps);
   dispatch_sc(e.S, e.C, [&](auto s, auto c) {
     constexpr int S = decltype(s)::value, C = decltype(c)::value;
     using TT = typename TipType<S>::type;
     if (e.opt.precision == NRX_FP32)
-      hipLaunchKernelGGL((k_deriv_fused<S, C, float>), dim3(e.L.n_tiles), dim3(TILE * C), 0, e.stream, steps,
-                         path_begin, n_paths, ops_by_slot, e.L, e.d_tile_part.as<int32_t>(), e.d_clv.as<float>(),
+      hipLaunchKernelGGL((k_deriv_fused<S, C, float>), dim3(e.L.n_tiles), dim3(TILE * C), lds, e.stream, steps,
+                         path_begin, n_paths, e.L, e.d_tile_part.as<int32_t>(), e.d_clv.as<float>(),
                          e.d_scl.as<int32_t>(), 1, e.d_tips.as<TT>(), e.d_P.as<double>(), e.d_V.as<double>(),
                          e.d_Vinv.as<double>(), e.d_freqs.as<double>(), e.d_eval.as<double>(),
                          e.d_rates.as<double>(), e.d_weights.as<double>(), brlen, partials);
     else
-      hipLaunchKernelGGL((k_deriv_fused<S, C, double>), dim3(e.L.n_tiles), dim3(TILE * C), 0, e.stream, steps,
-                         path_begin, n_paths, ops_by_slot, e.L, e.d_tile_part.as<int32_t>(),
+      hipLaunchKernelGGL((k_deriv_fused<S, C, double>), dim3(e.L.n_tiles), dim3(TILE * C), lds, e.stream, steps,
+                         path_begin, n_paths, e.L, e.d_tile_part.as<int32_t>(),
                          e.d_clv.as<double>(), e.d_scl.as<int32_t>(), 256, e.d_tips.as<TT>(),
                          e.d_P.as<double>(), e.d_V.as<double>(), e.d_Vinv.as<double>(), e.d_freqs.as<double>(),
                          e.d_eval.as<double>(), e.d_rates.as<double>(), e.d_weights.as<double>(), brlen,
