@@ -52,8 +52,8 @@ def run_form(w, precision, steps, warmup, use_graph=False, tune=False):
     Throughput (`evals_per_s`) is measured on an engine without timing
     events; kernel durations for the roofline come from a second engine of
     the same shape whose every pruning / adjoint launch is bracketed by HIP
-    events on its stream (events between the per-level launches of the plain
-    form would otherwise add their own gaps to its throughput).  With `tune`
+    events on its stream (events between launches would otherwise add their
+    own gaps to the throughput).  With `tune`
     (FP32 form) the kernel shape is autotuned first, outside the timed
     region (or set from --shape)."""
     ann = make_engine(w, precision, timing=False, use_graph=use_graph)
@@ -139,8 +139,8 @@ def roofline(res, ncat=4):
         split = step == "adjoint" and t.get("row_lanes", 1) == 2
         kname = (f"k_adjoint_split<{ncat}, {t['group_width']}, {t['tile_sites']}>" if split else
                  f"k_{step}_persistent<{ncat}, {t['group_width']}, {t['tile_sites']}>")
-    else:  # plain form: one launch per pruning level
-        kname = f"k_{step}_level<4, {ncat}>" if step == "adjoint" else f"k_forward_level<4, {ncat}, 4>"
+    else:  # plain form: one forward and one reverse-sweep launch walking every level
+        kname = f"k_adjoint_level<4, {ncat}>" if step == "adjoint" else f"k_forward_walk<4, {ncat}, 4>"
     traffic = pmc_traffic(kname)
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic, "kernel": kname,
@@ -430,8 +430,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--form", choices=["batched", "plain"], default="batched",
                     help="headline form: batched = FP32 CLVs, persistent site-tile kernels; "
-                         "plain = FP64 CLVs, one launch per pruning level")
-    ap.add_argument("--only", choices=["batched", "plain", "protein", "candidates"], default=None,
+                         "plain = FP64 CLVs, one launch per pass walking every pruning level")
+    ap.add_argument("--only", choices=["batched", "plain", "protein", "candidates", "sweep"], default=None,
                     help="time only this form or section (profiling runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-oracle-check", action="store_true")
@@ -490,6 +490,9 @@ def main():
     w = workload_50taxa_50k(seed=3)
     n_pat = sum(p.n_sites for p in w.partitions)
     log(f"workload ready in {time.perf_counter() - t0:.1f}s: {n_pat} patterns")
+    if args.only == "sweep":
+        print(json.dumps({"newton_sweep": bench_newton_sweep(w)}), flush=True)
+        return 0
 
     forms = {}
     for name, prec in (("plain", 0), ("batched", 1)):
@@ -502,7 +505,8 @@ def main():
     head_name = args.only or args.form
     head = forms[head_name]
     st = head["stats"]
-    desc = {"plain": "plain FP64 (FP64 CLVs, one launch per pruning level, FP64 adjoint)",
+    desc = {"plain": "plain FP64 (FP64 CLVs and adjoint; one forward and one reverse-sweep launch, a "
+                     "workgroup per site tile walking every pruning level)",
             "batched": "batched FP32 (FP32 CLVs + per-site power-of-two exponents, one persistent "
                        "site-tile launch for all ops and trees, FP32 adjoint with per-site "
                        "exponents, FP64 site sums / mixture / gradient reduction; each evaluation "
