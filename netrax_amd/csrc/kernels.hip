@@ -332,7 +332,6 @@ __device__ __forceinline__ void forward_ops(const DevOp *__restrict__ ops, int n
                                             int tile, double *__restrict__ clv, int32_t *__restrict__ scl,
                                             const typename TipType<S>::type *__restrict__ tips,
                                             const double *__restrict__ P, double *sP) {
-  stage_ops_p<S, C>(sP, P, ops, nb, part, L);  // ends with a barrier (4 states)
   const int c = threadIdx.x % C;
   const int64_t site = static_cast<int64_t>(tile) * TILE + threadIdx.x / C;
   const int64_t n = L.n_sites_pad;
@@ -367,6 +366,8 @@ __device__ __forceinline__ void forward_ops(const DevOp *__restrict__ ops, int n
         }
       }
     }
+    // the P matrices after the children's loads: one round trip for both
+    stage_ops_p<S, C>(sP, P, ops, nb, part, L);  // ends with a barrier (4 states)
     wait_loads();
 #pragma unroll
     for (int j = 0; j < G; ++j) {
@@ -2825,69 +2826,116 @@ __device__ __forceinline__ void nr_stage(const double *&sums, const DerivClassDe
   sums = s_sums;
 }
 
-__global__ void k_newton_step(const double *__restrict__ sums_g, int P, int nc,
-                              const DerivClassDev *__restrict__ cls_g, const double *__restrict__ base_g,
-                              int variant, int mode, NewtonState *__restrict__ ns) {
+// Butterfly reductions over the 64 lanes of the (one-wave) Newton kernels;
+// every lane ends with the result.
+__device__ __forceinline__ double nr_wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+__device__ __forceinline__ double nr_wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  return v;
+}
+__device__ __forceinline__ int nr_wave_isum(int v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+__device__ __forceinline__ int nr_wave_imax(int v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v = max(v, __shfl_xor(v, o));
+  return v;
+}
+
+// One wave: lane l takes the classes l, l + 64, ...; the class loops of the
+// mixture become wave reductions.
+__global__ void __launch_bounds__(64)
+    k_newton_step(const double *__restrict__ sums_g, int P, int nc,
+                  const DerivClassDev *__restrict__ cls_g, const double *__restrict__ base_g,
+                  int variant, int mode, NewtonState *__restrict__ ns) {
   if (ns->done) return;  // uniform: every thread reads the same flag
   const double *sums = sums_g, *base = base_g;
   const DerivClassDev *cls = cls_g;
   nr_stage(sums, cls, base, P, nc, ns->n_paths);
-  if (threadIdx.x != 0) return;
+  const int lane = threadIdx.x;
+  auto member = [&](int k) { return cls[k].use && (mode == NRX_DERIV_TRUE || cls[k].has); };
   // derivatives at ns->x, mixed per partition as the host does
   double D1 = 0.0, D2 = 0.0;
   for (int p = 0; p < P; ++p) {
-    int n_members = 0, only = -1;
-    for (int k = 0; k < nc; ++k)
-      if (cls[k].use && (mode == NRX_DERIV_TRUE || cls[k].has)) {
-        ++n_members;
-        only = k;
-      }
-    auto l1 = [&](int k) {
+    auto d1 = [&](int k) {
       return cls[k].has && cls[k].path >= 0 ? sums[(static_cast<size_t>(cls[k].path) * 3 + 1) * P + p] : 0.0;
     };
-    auto l2 = [&](int k) {
+    auto d2 = [&](int k) {
       return cls[k].has && cls[k].path >= 0 ? sums[(static_cast<size_t>(cls[k].path) * 3 + 2) * P + p] : 0.0;
     };
+    int cnt = 0, last = -1;
+    for (int k = lane; k < nc; k += 64)
+      if (member(k)) {
+        ++cnt;
+        last = k;
+      }
+    const int n_members = nr_wave_isum(cnt), only = nr_wave_imax(last);
     double pd1 = 0.0, pd2 = 0.0;
     if (n_members == 1 && mode == NRX_DERIV_REFERENCE) {
-      pd1 = l1(only);
-      pd2 = l2(only);
+      pd1 = d1(only);  // single-tree mode
+      pd2 = d2(only);
     } else if (n_members > 0) {
       if (variant == NRX_AVERAGE_DISPLAYED_TREES) {
-        double mx = -INFINITY;
-        for (int k = 0; k < nc; ++k)
-          if (cls[k].use && (mode == NRX_DERIV_TRUE || cls[k].has))
-            mx = fmax(mx, cls[k].logprob + nr_tree_logl(sums, cls[k], base, P, nc, k, p));
+        double m = -INFINITY;
+        for (int k = lane; k < nc; k += 64)
+          if (member(k)) m = fmax(m, cls[k].logprob + nr_tree_logl(sums, cls[k], base, P, nc, k, p));
+        const double mx = nr_wave_max(m);
         double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-        for (int k = 0; k < nc; ++k) {
-          if (!(cls[k].use && (mode == NRX_DERIV_TRUE || cls[k].has))) continue;
+        for (int k = lane; k < nc; k += 64) {
+          if (!member(k)) continue;
           const double wgt = exp(cls[k].logprob + nr_tree_logl(sums, cls[k], base, P, nc, k, p) - mx);
-          const double a1 = l1(k);
+          const double a1 = d1(k);
           s0 += wgt;
           s1 += wgt * a1;
-          s2 += wgt * (a1 * a1 + l2(k));
+          s2 += wgt * (a1 * a1 + d2(k));
         }
+        s0 = nr_wave_sum(s0);
+        s1 = nr_wave_sum(s1);
+        s2 = nr_wave_sum(s2);
         pd1 = s1 / s0;
         pd2 = (s2 * s0 - s1 * s1) / (s0 * s0);
       } else {
+        // the first class (lowest index) with the highest score, as the
+        // host's strict comparison in class order picks it
         double best = -INFINITY;
-        for (int k = 0; k < nc; ++k) {
-          if (!(cls[k].use && (mode == NRX_DERIV_TRUE || cls[k].has))) continue;
+        int kb = -1;
+        for (int k = lane; k < nc; k += 64) {
+          if (!member(k)) continue;
           const double ll = nr_tree_logl(sums, cls[k], base, P, nc, k, p);
           // reference mode: the tree maximising tree_logl * tree_prob
           // (LikelihoodDerivatives.cpp:161-167)
           const double score = mode == NRX_DERIV_REFERENCE ? ll * exp(cls[k].logprob) : cls[k].logprob + ll;
           if (score > best) {
             best = score;
-            pd1 = l1(k);
-            pd2 = l2(k);
+            kb = k;
           }
+        }
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+          const double ob = __shfl_xor(best, o);
+          const int ok = __shfl_xor(kb, o);
+          if (ok >= 0 && (kb < 0 || ob > best || (ob == best && ok < kb))) {
+            best = ob;
+            kb = ok;
+          }
+        }
+        if (kb >= 0) {
+          pd1 = d1(kb);
+          pd2 = d2(kb);
         }
       }
     }
     D1 += pd1;
     D2 += pd2;
   }
+  if (lane != 0) return;
   ns->d1 = D1;
   ns->d2 = D2;
   ns->iters += 1;
@@ -2928,35 +2976,39 @@ __global__ void k_newton_step(const double *__restrict__ sums_g, int P, int nc,
 // network lnL at the final length vs at the start (both from the same tree
 // lnLs); a worse length is given up (optimize_branch,
 // BranchLengthOptimization.cpp:317-330)
-__global__ void k_newton_accept(const double *__restrict__ sums_g, int P, int nc,
-                                const DerivClassDev *__restrict__ cls_g, const double *__restrict__ base_g,
-                                int variant, NewtonState *__restrict__ ns) {
+__global__ void __launch_bounds__(64)
+    k_newton_accept(const double *__restrict__ sums_g, int P, int nc,
+                    const DerivClassDev *__restrict__ cls_g, const double *__restrict__ base_g,
+                    int variant, NewtonState *__restrict__ ns) {
   const double *sums = sums_g, *base = base_g;
   const DerivClassDev *cls = cls_g;
   nr_stage(sums, cls, base, P, nc, ns->n_paths);
-  if (threadIdx.x != 0) return;
+  const int lane = threadIdx.x;
   double l_new = 0.0, l_old = 0.0;
   for (int p = 0; p < P; ++p) {
     double mn = -INFINITY, mo = -INFINITY;
-    for (int k = 0; k < nc; ++k) {
+    for (int k = lane; k < nc; k += 64) {
       if (!cls[k].use) continue;
       mn = fmax(mn, cls[k].logprob + nr_tree_logl(sums, cls[k], base, P, nc, k, p));
       mo = fmax(mo, cls[k].logprob + base[static_cast<size_t>(p) * nc + k]);
     }
+    mn = nr_wave_max(mn);
+    mo = nr_wave_max(mo);
     if (variant == NRX_BEST_DISPLAYED_TREE) {
       l_new += mn;
       l_old += mo;
       continue;
     }
     double sn = 0.0, so = 0.0;
-    for (int k = 0; k < nc; ++k) {
+    for (int k = lane; k < nc; k += 64) {
       if (!cls[k].use) continue;
       sn += exp(cls[k].logprob + nr_tree_logl(sums, cls[k], base, P, nc, k, p) - mn);
       so += exp(cls[k].logprob + base[static_cast<size_t>(p) * nc + k] - mo);
     }
-    l_new += mn + log(sn);
-    l_old += mo + log(so);
+    l_new += mn + log(nr_wave_sum(sn));
+    l_old += mo + log(nr_wave_sum(so));
   }
+  if (lane != 0) return;
   ns->lnl_new = l_new;
   ns->lnl_old = l_old;
   ns->accepted = l_new >= l_old;
