@@ -168,7 +168,8 @@ def bench_protein(steps, warmup):
            "patterns": int(sum(p.n_sites for p in w.partitions))}
     forms = {}
     for name, prec in (("plain_fp64", 0), ("fp32_mfma", 1)):
-        r = run_form(w, prec, steps, warmup, tune=prec == 1)
+        # the FP32 form replays one captured graph per evaluation, as the headline does
+        r = run_form(w, prec, steps, warmup, use_graph=prec == 1, tune=prec == 1)
         forms[name] = r
         st = r["stats"]
         d = {"evals_per_s": round(r["evals_per_s"], 3),
