@@ -56,7 +56,8 @@ extern "C" {
 #define NRX_BEST_DISPLAYED_TREE 1
 
 /* arithmetic of the CLV products */
-#define NRX_FP64 0 /* plain form: FP64 CLVs, one launch per pruning level */
+#define NRX_FP64 0 /* plain form: FP64 CLVs; one forward and one reverse-sweep
+                      launch walking the pruning levels per site tile */
 #define NRX_FP32 1 /* batched form: FP32 CLVs + per-site power-of-two
                       exponents, persistent site-tile kernels, FP64 sums */
 

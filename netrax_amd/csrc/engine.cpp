@@ -321,7 +321,8 @@ static void build_schedule(Engine &e) {
     return op;
   };
 
-  // plain form: one launch per pruning level, ops in slot order
+  // plain form: level chunks (one per pruning level), ops in slot order;
+  // the 4-state passes walk them in one launch, 20 states launch per level
   e.ops_all.clear();
   e.chunks.clear();
   for (int l = 1; l <= max_level; ++l) {

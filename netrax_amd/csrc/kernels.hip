@@ -9,10 +9,10 @@
 // Every site-parallel kernel works on tiles of TILE = 64 sites; a workgroup
 // holds TILE * C threads, thread = (site, rate category), so the C lanes of
 // a site are adjacent and a CLV row of a workgroup is one contiguous
-// TILE*C*S*sizeof(real) span.  Workgroup b of a per-level launch handles
-// tile b % n_tiles_grid of op b / n_tiles_grid; n_tiles_grid is a multiple
-// of 8, so a tile is always dispatched to the same XCD (round-robin over
-// workgroup ids) and its children's CLVs stay in that XCD's L2.
+// TILE*C*S*sizeof(real) span.  Workgroup b of a launch over (tile, op)
+// handles tile b % n_tiles_grid of op b / n_tiles_grid; n_tiles_grid is a
+// multiple of 8, so a tile is always dispatched to the same XCD (round-robin
+// over workgroup ids) and its children's CLVs stay in that XCD's L2.
 #include <cfloat>
 #include <cmath>
 #include <map>
@@ -291,8 +291,10 @@ __global__ void k_pmatrices(DevLayout L, const double *__restrict__ V,
 }
 
 // ------------------------------------------------------- forward (FP64)
-// Plain form: one launch per pruning level, a 2-D grid of (tile, group of
-// ops of the level).  blockIdx.x is the tile and gridDim.x a multiple of 8,
+// Plain form: a workgroup per tile walks the levels in one launch
+// (k_forward_walk), or, under a memory budget, one launch per level on a
+// 2-D grid of (tile, group of ops of the level).  In the latter blockIdx.x
+// is the tile and gridDim.x a multiple of 8,
 // so every workgroup of a tile lands on the same XCD (workgroup ids are
 // dealt round-robin to the 8 XCDs) and finds the children it reads, written
 // by the same tile in earlier levels, in that XCD's L2.  For 4-state data
