@@ -312,6 +312,137 @@ def cpu_model() -> str:
     return platform.processor() or "unknown"
 
 
+def _median_ms(fn, reps):
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(1e3 * (time.perf_counter() - t0))
+    return statistics.median(ts)
+
+
+def bench_widened(w):
+    """The calls around the hot path on the headline workload (plain FP64
+    engine unless noted), host-inclusive wall times:
+
+    * model gradient (nrx_model_gradient: dlnL/d exchangeabilities,
+      frequencies, Gamma shape for both partitions) against the central
+      differences it replaces (2 full lnL evaluations per parameter);
+    * reticulation probabilities: d lnL/dp and d2 lnL/dp2 with the CLVs
+      reused, and the Newton optimiser from perturbed probabilities;
+    * lnL + gradient under a memory budget (the smallest of 50-90 % of the
+      resident CLV rows that a segment plan fits) against the resident
+      evaluation (bit-identical results);
+    * per-candidate optimisation of 32 candidate networks
+      (nrx_batch_optimize: branches around the move and every reticulation,
+      then one batched scoring) against the same steps one engine at a time."""
+    from netrax_amd import AnnotatedNetwork, NetraxOptions, NetworkBatch, Precision
+    from netrax_amd.simulate import candidate_batch
+
+    out = {}
+    ann = AnnotatedNetwork(w.network.copy(), w.partitions, NetraxOptions())
+    b0 = w.network.brlen.copy()
+    flip = [0]
+
+    def full_grad():
+        flip[0] ^= 1
+        ann.set_branch_lengths(b0 * (1.0 + 1e-7 * flip[0]))
+        return ann.gradient()
+
+    def full_lnl():
+        flip[0] ^= 1
+        ann.set_branch_lengths(b0 * (1.0 + 1e-7 * flip[0]))
+        return ann.loglikelihood(incremental=0)
+
+    full_grad()
+    grad_ms = _median_ms(full_grad, 7)
+    lnl_ms = _median_ms(full_lnl, 7)
+    mg_ms = _median_ms(ann.model_gradient, 7)
+    n_par = sum(len(p.model.subst_rates) - 1 + len(p.model.freqs) - 1 + 1 for p in w.partitions)
+    out["model_gradient"] = {"ms": round(mg_ms, 3), "parameters": n_par,
+                             "central_difference_ms": round(2 * n_par * lnl_ms, 3),
+                             "full_lnl_ms": round(lnl_ms, 3), "lnl_grad_ms": round(grad_ms, 3)}
+    pd_ms = _median_ms(ann.prob_derivatives, 7)
+    rng = np.random.default_rng(12)
+    p0 = np.clip(w.network.retic_prob * rng.uniform(0.5, 1.5, w.network.n_retic), 0.05, 0.95)
+    ann.set_reticulation_probs(p0)
+    l0 = ann.loglikelihood()
+    t0 = time.perf_counter()
+    l1 = ann.optimize_reticulations(10)
+    opt_ms = 1e3 * (time.perf_counter() - t0)
+    out["reticulation_probs"] = {"derivatives_ms": round(pd_ms, 3), "optimise_ms": round(opt_ms, 3),
+                                 "lnl_start": l0, "lnl_end": l1, "reticulations": int(w.network.n_retic)}
+    ann.set_reticulation_probs(w.network.retic_prob)
+    # memory budget: the smallest of 50-90 % of the resident rows that a
+    # segment plan fits
+    from netrax_amd import _lib
+    ann.set_branch_lengths(b0)
+    ref = ann.gradient()
+    plan0 = ann.memory_plan()
+    frac = None
+    for f in (0.5, 0.6, 0.7, 0.8, 0.9):
+        try:
+            ann.set_memory_budget(int(f * plan0["resident_rows"] * plan0["row_bytes"]))
+            frac = f
+            break
+        except _lib.NrxError:
+            continue
+    plan = ann.memory_plan()
+    ann.set_branch_lengths(b0)
+    got = ann.gradient()
+    same = bool(got[0] == ref[0] and np.array_equal(got[1], ref[1]) and np.array_equal(got[2], ref[2]))
+    bud_ms = _median_ms(full_grad, 7)
+    ann.set_memory_budget(0)
+    out["memory_budget"] = {"budget_fraction_of_resident": frac,
+                            "lnl_grad_ms": round(bud_ms, 3), "resident_lnl_grad_ms": round(grad_ms, 3),
+                            "rows": plan["rows"], "resident_rows": plan0["resident_rows"],
+                            "segments": plan["segments"], "row_bytes": plan["row_bytes"],
+                            "bit_identical": same}
+    ann.close()
+    # per-candidate optimisation inside a batch
+    cands = candidate_batch(8, 4, seed=5)
+
+    def move_edges(net):
+        es = {0}
+        for r in range(net.n_retic):
+            es.add(int(net.retic_first_edge[r]))
+            es.add(int(net.retic_second_edge[r]))
+        return sorted(es)
+
+    edges = [move_edges(n) for n, _p in cands]
+    retics = [list(range(n.n_retic)) for n, _p in cands]
+    opts = NetraxOptions(precision=Precision.FP32, use_graph=True)
+    anns = [AnnotatedNetwork(n.copy(), p, opts) for n, p in cands]
+    for a in anns:
+        a.loglikelihood()
+    batch = NetworkBatch(anns)
+    t0 = time.perf_counter()
+    lnl_b, st = batch.optimize(edges, retics, brlen_iters=32, prob_iters=10, n_threads=8)
+    batch_ms = 1e3 * (time.perf_counter() - t0)
+    batch.close()
+    for a in anns:
+        a.close()
+    twins = [AnnotatedNetwork(n.copy(), p, opts) for n, p in cands]
+    for a in twins:
+        a.loglikelihood()
+    t0 = time.perf_counter()
+    lnl_s = []
+    for i, a in enumerate(twins):
+        a.optimize_branches(edges[i], max_iters=32)
+        lnl_s.append(a.optimize_reticulations(10) if cands[i][0].n_retic else a.loglikelihood())
+    seq_ms = 1e3 * (time.perf_counter() - t0)
+    for a in twins:
+        a.close()
+    out["candidate_optimisation"] = {
+        "candidates": len(cands), "batch_ms": round(batch_ms, 2), "one_at_a_time_ms": round(seq_ms, 2),
+        "host_workers": 8, "statuses_ok": bool(np.all(st == 0)),
+        "max_rel_lnl_diff_vs_one_at_a_time": float(np.max(np.abs(lnl_b - np.array(lnl_s)) / np.abs(lnl_s)))}
+    log(f"widened: model gradient {mg_ms:.2f} ms (central differences {2 * n_par * lnl_ms:.1f} ms), "
+        f"prob optimiser {opt_ms:.1f} ms, budget {bud_ms:.2f} vs {grad_ms:.2f} ms, "
+        f"candidate optimisation {batch_ms:.1f} vs {seq_ms:.1f} ms")
+    return out
+
+
 def cpu_baseline(w, sample_patterns, threads, budget_s):
     """The oracle (C FP64 restatement, OpenMP over sites) timed on the host
     cores: lnL + full gradient of the whole workload (every pattern), with
@@ -432,7 +563,8 @@ def main():
     ap.add_argument("--form", choices=["batched", "plain"], default="batched",
                     help="headline form: batched = FP32 CLVs, persistent site-tile kernels; "
                          "plain = FP64 CLVs, one launch per pass walking every pruning level")
-    ap.add_argument("--only", choices=["batched", "plain", "protein", "candidates", "sweep"], default=None,
+    ap.add_argument("--only", choices=["batched", "plain", "protein", "candidates", "sweep", "widened"],
+                    default=None,
                     help="time only this form or section (profiling runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-oracle-check", action="store_true")
@@ -493,6 +625,9 @@ def main():
     log(f"workload ready in {time.perf_counter() - t0:.1f}s: {n_pat} patterns")
     if args.only == "sweep":
         print(json.dumps({"newton_sweep": bench_newton_sweep(w)}), flush=True)
+        return 0
+    if args.only == "widened":
+        print(json.dumps({"widened": bench_widened(w)}), flush=True)
         return 0
 
     forms = {}
@@ -558,6 +693,7 @@ def main():
         }
     if not args.no_extras and not args.only:
         out["newton_sweep"] = bench_newton_sweep(w)
+        out["widened"] = bench_widened(w)
         out["protein"] = bench_protein(max(5, args.steps // 2), args.warmup)
         out["candidate_batch"] = bench_candidates(args.candidates, 3, 1)
     if not args.no_oracle_check:
