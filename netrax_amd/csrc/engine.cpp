@@ -798,6 +798,51 @@ static void budget_adjoint(Engine &e) {
   }
 }
 
+// FP32 form, incremental evaluation: the ops whose CLVs are invalid, in the
+// forward order and groups of the full list (a subset of a group of
+// independent ops is independent), as a device descriptor of their own.
+// A group is flagged independent of the one before it when it reads none of
+// that group's outputs, as the kernels' early-load rule requires.  Returns
+// the number of groups (0: nothing to recompute), or -1 when every op is
+// invalid (the full list, and its graph, is used instead); `desc` points to
+// the staged descriptor.
+static int stage_invalid_ops(Engine &e, const NetDesc *&desc) {
+  const int ng = static_cast<int>(e.group_begin.size()) - 1;
+  std::vector<DevOp> ops;
+  std::vector<int32_t> gb(1, 0);
+  std::vector<int32_t> prev, cur;
+  for (int g = 0; g < ng; ++g) {
+    cur.clear();
+    const size_t first = ops.size();
+    for (int i = e.group_begin[g]; i < e.group_begin[g + 1]; ++i) {
+      const DevOp &op = e.ops_dfs[i];
+      if (!e.clv_dirty[op.out]) continue;
+      ops.push_back(op);
+      cur.push_back(op.out);
+    }
+    if (cur.empty()) continue;
+    bool indep = !prev.empty();
+    for (size_t i = first; i < ops.size() && indep; ++i)
+      for (int k = 0; k < ops[i].nk; ++k)
+        if (std::find(prev.begin(), prev.end(), ops[i].kid[k]) != prev.end()) indep = false;
+    ops[first].flags = indep ? (ops[first].flags | OP_GROUP_INDEP) : (ops[first].flags & ~OP_GROUP_INDEP);
+    gb.push_back(static_cast<int32_t>(ops.size()));
+    prev.swap(cur);
+  }
+  if (ops.size() == e.ops_dfs.size()) return -1;
+  const int n = static_cast<int>(gb.size()) - 1;
+  if (n == 0) return 0;
+  stage_upload(e, e.d_ops_inc, ops);
+  stage_upload(e, e.d_group_begin_inc, gb);
+  NetDesc d = make_netdesc(e);
+  d.ops = e.d_ops_inc.as<DevOp>();
+  d.group_begin = e.d_group_begin_inc.as<int32_t>();
+  d.n_groups = n;
+  stage_upload(e, e.d_netdesc_inc, std::vector<NetDesc>{d});
+  desc = e.d_netdesc_inc.as<NetDesc>();
+  return n;
+}
+
 // `fetch` false: the results stay on the device (for a caller that reads
 // them there, such as the device Newton sweep) and no host wait is made;
 // the host copy is then stale, so the cache is marked invalid.
@@ -819,16 +864,25 @@ static int evaluate(Engine &e, int variant, bool incremental, bool update_pm, bo
   }
   bool results_copied = false;
   if (e.opt.precision == NRX_FP32) {
-    // batched form: the whole op list in one persistent launch, root site
-    // sums fused into it; every CLV is recomputed
+    // batched form: the op list in one persistent launch, root site sums
+    // fused into it.  A full evaluation recomputes every CLV (one captured
+    // graph with use_graph); an incremental one with some CLVs still valid
+    // runs the invalid ops only, through a descriptor of their own, eagerly
     const bool aa32 = e.S == AA_STATES;
-    auto enqueue = [&e, aa32, variant, gradient, pm_in_graph](bool timed) {
+    const NetDesc *inc = nullptr;
+    int n_inc_groups = -1;
+    if (incremental) n_inc_groups = stage_invalid_ops(e, inc);
+    const bool partial = n_inc_groups >= 0;
+    auto enqueue = [&e, aa32, variant, gradient, pm_in_graph, inc, n_inc_groups](bool timed) {
       if (pm_in_graph) enqueue_pmatrices(e);
       if (timed) record_kernel_begin(e, 0);
-      if (aa32)
-        launch_forward_mfma(e);
-      else
-        launch_forward_persistent(e);
+      if (n_inc_groups == 0) {
+        // no CLV to recompute (a probability change, or a repeated call)
+      } else if (aa32) {
+        launch_forward_mfma(e, inc);
+      } else {
+        launch_forward_persistent(e, inc);
+      }
       if (timed) record_kernel_end(e, 0);
       launch_mixture(e, variant, e.pw);
       if (gradient) {
@@ -841,7 +895,7 @@ static int evaluate(Engine &e, int variant, bool incremental, bool update_pm, bo
         launch_grad_reduce(e);
       }
     };
-    if (e.opt.use_graph) {
+    if (e.opt.use_graph && !partial) {
       // one graph launch per evaluation; re-captured after a schedule change
       if (e.graph_version != e.schedule_version) {
         destroy_graphs(e.graph_exec);

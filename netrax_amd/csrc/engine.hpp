@@ -256,10 +256,11 @@ void launch_adjoint_level(Engine &e, const LevelChunk &ch, const DevOp *ops);
 // 4 states: level chunks c_hi down to c_lo in one launch
 void launch_adjoint_levels(Engine &e, const DevOp *ops, int c_hi, int c_lo);
 void launch_grad_reduce(Engine &e);
-void launch_forward_persistent(Engine &e);
+// nets: a descriptor other than the engine's own (its invalid ops only)
+void launch_forward_persistent(Engine &e, const NetDesc *nets = nullptr);
 void launch_adjoint_persistent(Engine &e);
 void launch_mfma_tables(Engine &e);
-void launch_forward_mfma(Engine &e);
+void launch_forward_mfma(Engine &e, const NetDesc *nets = nullptr);
 void launch_adjoint_mfma(Engine &e);
 void launch_sumtables(Engine &e, int n_paths, int max_steps);
 // FP64 4-state: per use, chunk and category the summed z x^T (dlnL/dP);

@@ -153,6 +153,9 @@ struct Engine : nrx_engine {
   DevBuf d_adj_walk;  // the level chunks top level first (the reverse sweep's walk)
   DevBuf d_fwd_walk;  // the level chunks lowest level first (the forward pass's walk)
   DevBuf d_dirty_walk;  // op ranges of d_ops_dirty per level (incremental forward)
+  // FP32 form, incremental evaluation: the invalid ops, their groups and
+  // their descriptor
+  DevBuf d_ops_inc, d_group_begin_inc, d_netdesc_inc;
   DevBuf d_ops_dfs, d_group_begin, d_edge_use_begin, d_edge_use_list;
   DevBuf d_netdesc, d_work;  // FP32 form: this engine's NetDesc and (0, tile) work table
   DevBuf d_classes, d_class_dlogprob, d_root_slots;

@@ -3380,9 +3380,9 @@ void dispatch_shape(int C, int G, int TS, F &&f) {
   });
 }
 
-void launch_forward_persistent(Engine &e) {
+void launch_forward_persistent(Engine &e, const NetDesc *nets) {
   launch_forward_persistent_multi(e.stream, e.C, e.tune_group, e.tune_tile,
-                                  e.d_netdesc.as<NetDesc>(), e.d_work.as<WorkItem>(),
+                                  nets ? nets : e.d_netdesc.as<NetDesc>(), e.d_work.as<WorkItem>(),
                                   static_cast<int>(e.L.n_sites_pad / e.tune_tile),
                                   sizeof(float) * e.E * e.C * 16);
 }
@@ -3466,13 +3466,13 @@ void launch_mfma_tables(Engine &e) {
 }
 
 // one workgroup per 32-site tile: the work table holds 2 n_tiles items
-void launch_forward_mfma(Engine &e) {
+void launch_forward_mfma(Engine &e, const NetDesc *nets) {
   dispatch_c(e.C, [&](auto c) {
     constexpr int CC = decltype(c)::value;
     auto go = [&](auto g) {
       constexpr int GG = decltype(g)::value;
       hipLaunchKernelGGL((k_forward_mfma<CC, GG>), dim3(2 * e.L.n_tiles), dim3(64 * CC), 0,
-                         e.stream, e.d_netdesc.as<NetDesc>(), e.d_work.as<WorkItem>());
+                         e.stream, nets ? nets : e.d_netdesc.as<NetDesc>(), e.d_work.as<WorkItem>());
     };
     switch (e.tune_group) {
       case 1: go(std::integral_constant<int, 1>{}); break;
