@@ -2700,6 +2700,28 @@ __device__ __forceinline__ void sumtable_partials(const double (&in)[S], const d
   out3[2] = wave_sum(v2);
 }
 
+// lambda_k r_c and e^(lambda_k r_c b) of the workgroup's partition for every
+// (category, state): site-independent, so computed once per workgroup (the
+// exponentials would otherwise dominate the evaluation's arithmetic) and
+// handed to every thread through LDS; the values are the ones each thread
+// computed itself before.  Ends with a barrier.
+template <int S, int C>
+__device__ __forceinline__ void stage_exps(const double *__restrict__ ev, const double *__restrict__ rates, int part,
+                                           double brlen, int c, double (&ex)[S], double (&lr)[S]) {
+  __shared__ double s_lr[C * S], s_ex[C * S];
+  for (int i = threadIdx.x; i < C * S; i += blockDim.x) {
+    const double l = ev[part * S + i % S] * rates[part * C + i / S];
+    s_lr[i] = l;
+    s_ex[i] = exp(l * brlen);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < S; ++k) {
+    lr[k] = s_lr[c * S + k];
+    ex[k] = s_ex[c * S + k];
+  }
+}
+
 template <int S, int C>
 __global__ void __launch_bounds__(TILE *C)
     k_sumtable_eval(const double *__restrict__ st, int n_paths, DevLayout L,
@@ -2707,6 +2729,7 @@ __global__ void __launch_bounds__(TILE *C)
                     const double *__restrict__ ev, const double *__restrict__ rates,
                     const double *__restrict__ weights, double brlen,
                     double *__restrict__ partials) {
+  if (static_cast<int>(blockIdx.y) >= n_paths) return;
   constexpr int PW = TILE * C / 64;
   const int tile = blockIdx.x;
   const int part = tile_part[tile];
@@ -2717,13 +2740,8 @@ __global__ void __launch_bounds__(TILE *C)
   const size_t n_wt = static_cast<size_t>(L.n_tiles) * PW;
   const int wt = tile * PW + threadIdx.x / 64;
   const double w = weights[site];
-  const double r = rates[part * C + c];
   double ex[S], lr[S];
-#pragma unroll
-  for (int k = 0; k < S; ++k) {
-    lr[k] = ev[part * S + k] * r;
-    ex[k] = exp(lr[k] * brlen);
-  }
+  stage_exps<S, C>(ev, rates, part, brlen, c, ex, lr);
   // one path per workgroup (grid tiles x paths)
   for (int j = blockIdx.y; j < min(n_paths, static_cast<int>(blockIdx.y) + 1); ++j) {
     const double *in = st + (static_cast<size_t>(j) * n + site) * row + c * S;
@@ -2754,6 +2772,7 @@ __global__ void __launch_bounds__(TILE *C)
   if (!final_eval && ns->done) return;
   constexpr int PW = TILE * C / 64;
   const int n_paths = ns->n_paths;
+  if (static_cast<int>(blockIdx.y) >= n_paths) return;  // uniform: the grid covers the largest path count
   const double brlen = ns->x;
   const int tile = blockIdx.x;
   const int part = tile_part[tile];
@@ -2764,13 +2783,8 @@ __global__ void __launch_bounds__(TILE *C)
   const size_t n_wt = static_cast<size_t>(L.n_tiles) * PW;
   const int wt = tile * PW + threadIdx.x / 64;
   const double w = weights[site];
-  const double r = rates[part * C + c];
   double ex[S], lr[S];
-#pragma unroll
-  for (int k = 0; k < S; ++k) {
-    lr[k] = ev[part * S + k] * r;
-    ex[k] = exp(lr[k] * brlen);
-  }
+  stage_exps<S, C>(ev, rates, part, brlen, c, ex, lr);
   // one path per workgroup (grid tiles x the largest path count)
   for (int j = blockIdx.y; j < min(n_paths, static_cast<int>(blockIdx.y) + 1); ++j) {
     const double *in = st + (static_cast<size_t>(j) * n + site) * row + c * S;
@@ -3038,13 +3052,8 @@ __global__ void __launch_bounds__(TILE *C)
   const size_t n_wt = static_cast<size_t>(L.n_tiles) * PW;
   const int wt = tile * PW + threadIdx.x / 64;
   const double w = weights[site];
-  const double r = rates[part * C + c];
   double ex[S], lr[S];
-#pragma unroll
-  for (int k = 0; k < S; ++k) {
-    lr[k] = ev[part * S + k] * r;
-    ex[k] = exp(lr[k] * brlen);
-  }
+  stage_exps<S, C>(ev, rates, part, brlen, c, ex, lr);
   double *sPp = dyn_lds<double>();
   for (int j = 0; j < n_paths; ++j) {
     double stv[S], v[3];
