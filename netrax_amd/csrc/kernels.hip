@@ -340,6 +340,7 @@ __device__ __forceinline__ void forward_ops(const DevOp *__restrict__ ops, int n
   {
     // load phase: raw values only (CLV rows, scaler words, tip masks), so no
     // load has to wait for the ones issued before it
+    __builtin_amdgcn_s_setprio(1);
     double x[G][2][S];
     int sk[G][2];
     uint32_t tm[G][2];
@@ -369,6 +370,7 @@ __device__ __forceinline__ void forward_ops(const DevOp *__restrict__ ops, int n
       }
     }
     // the P matrices after the children's loads: one round trip for both
+    __builtin_amdgcn_s_setprio(0);
     stage_ops_p<S, C>(sP, P, ops, nb, part, L);  // ends with a barrier (4 states)
     wait_loads();
 #pragma unroll
@@ -734,7 +736,9 @@ __global__ void __launch_bounds__(TILE *C, S == 4 ? 3 : 1)
     for (int j = 0; j < n_ops; ++j) {
       const DevOp op = lops[j];
       AdjIn<S> cur;
+      __builtin_amdgcn_s_setprio(1);  // the op's loads go out ahead of other waves' arithmetic
       adjoint_inputs<S, C>(op, site, n, c, clv, scl, adj, tips, seed, cur);
+      __builtin_amdgcn_s_setprio(0);
       double a[S];
       const bool has_adj = (op.flags & OP_HAS_CONSUMERS) != 0;
 #pragma unroll
@@ -979,6 +983,7 @@ __global__ void __launch_bounds__(TILE *C)
     // in for absent inputs and are dropped below), so that no join copies a
     // loaded register and waits for the loads issued before it
     double araw[S], x[2][S];
+    __builtin_amdgcn_s_setprio(1);  // the op's loads go out ahead of other waves' arithmetic
     load_row20(adj + (static_cast<size_t>(op.out) * n + site) * row + c * S, araw);
     const int kq = scl[static_cast<size_t>(op.out) * n + site];
     const double sd = seed[static_cast<size_t>(op.root >= 0 ? op.root : 0) * n + site];
@@ -991,6 +996,7 @@ __global__ void __launch_bounds__(TILE *C)
       sk[k] = scl[static_cast<size_t>(kid < 0 ? 0 : kid) * n + site];
       if (k < op.nk && kid >= 0) load_row20(clv + (static_cast<size_t>(kid) * n + site) * row + c * S, x[k]);
     }
+    __builtin_amdgcn_s_setprio(0);
     const bool has_adj = (op.flags & OP_HAS_CONSUMERS) != 0;
     double a[S];
 #pragma unroll
@@ -1430,7 +1436,11 @@ __device__ __forceinline__ void forward_persistent_body(
       nn = group_begin[gi + 2] - n0;
       early = (ops[n0].flags & OP_GROUP_INDEP) != 0;
     }
-    if (early) fwd_load<C, G>(ops, n0, nn, clv, scl, tips, n, site, c, nxt);
+    if (early) {
+      __builtin_amdgcn_s_setprio(1);  // the loads go out ahead of other waves' arithmetic
+      fwd_load<C, G>(ops, n0, nn, clv, scl, tips, n, site, c, nxt);
+      __builtin_amdgcn_s_setprio(0);
+    }
     float v[G][4];
     int kk_[G];
     double rv[G];
@@ -1489,7 +1499,9 @@ __device__ __forceinline__ void forward_persistent_body(
     }
     if (has_next) {
       if (!early) {
+        __builtin_amdgcn_s_setprio(1);
         fwd_load<C, G>(ops, n0, nn, clv, scl, tips, n, site, c, nxt);
+        __builtin_amdgcn_s_setprio(0);
         wait_loads();
       }
       cur = nxt;
@@ -1624,6 +1636,7 @@ __device__ __forceinline__ void adjoint_persistent_body(
     const int nb = group_begin[gi + 1] - g0;
     // load the adjoints and children of the whole group first, as raw
     // values: nothing in this phase waits for an earlier load
+    __builtin_amdgcn_s_setprio(1);
     float aq[G][4], x[G][2][4], xq[G][4], acc[G][2][4];
     int gq[G], kq[G], sk[G][2], eacc[G][2];
     uint32_t tm[G][2];
@@ -1679,6 +1692,7 @@ __device__ __forceinline__ void adjoint_persistent_body(
         }
       }
     }
+    __builtin_amdgcn_s_setprio(0);
     wait_loads();
     int ks[G];
 #pragma unroll
