@@ -146,6 +146,11 @@ __device__ __forceinline__ double cat_sum(double m) {
 // path with no pending load, so the compiler needs no conservative waits
 // before the next phase's loads.
 __device__ __forceinline__ void wait_loads() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+// Load phases are bracketed by s_setprio 1 / s_setprio 0.  Measured 1-1.5 %
+// faster in the batched and plain passes; swapping the two priorities
+// measured the same, so the gain comes from the fixed instruction order the
+// pair imposes (the compiler moves nothing across an s_setprio), not from
+// the wave arbitration itself.
 
 __device__ __forceinline__ double readlane(double v, int lane) {
   return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
@@ -736,7 +741,7 @@ __global__ void __launch_bounds__(TILE *C, S == 4 ? 3 : 1)
     for (int j = 0; j < n_ops; ++j) {
       const DevOp op = lops[j];
       AdjIn<S> cur;
-      __builtin_amdgcn_s_setprio(1);  // the op's loads go out ahead of other waves' arithmetic
+      __builtin_amdgcn_s_setprio(1);  // see the note at wait_loads
       adjoint_inputs<S, C>(op, site, n, c, clv, scl, adj, tips, seed, cur);
       __builtin_amdgcn_s_setprio(0);
       double a[S];
@@ -983,7 +988,7 @@ __global__ void __launch_bounds__(TILE *C)
     // in for absent inputs and are dropped below), so that no join copies a
     // loaded register and waits for the loads issued before it
     double araw[S], x[2][S];
-    __builtin_amdgcn_s_setprio(1);  // the op's loads go out ahead of other waves' arithmetic
+    __builtin_amdgcn_s_setprio(1);  // see the note at wait_loads
     load_row20(adj + (static_cast<size_t>(op.out) * n + site) * row + c * S, araw);
     const int kq = scl[static_cast<size_t>(op.out) * n + site];
     const double sd = seed[static_cast<size_t>(op.root >= 0 ? op.root : 0) * n + site];
@@ -1437,7 +1442,7 @@ __device__ __forceinline__ void forward_persistent_body(
       early = (ops[n0].flags & OP_GROUP_INDEP) != 0;
     }
     if (early) {
-      __builtin_amdgcn_s_setprio(1);  // the loads go out ahead of other waves' arithmetic
+      __builtin_amdgcn_s_setprio(1);  // see the note at wait_loads
       fwd_load<C, G>(ops, n0, nn, clv, scl, tips, n, site, c, nxt);
       __builtin_amdgcn_s_setprio(0);
     }
