@@ -904,6 +904,7 @@ __global__ void __launch_bounds__(TILE *C)
   double x[2][S];
   uint32_t tm[2];
   int sk[2];
+  __builtin_amdgcn_s_setprio(1);  // see the note at wait_loads
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
     const int32_t kid = k < op.nk ? op.kid[k] : -1;
@@ -911,6 +912,7 @@ __global__ void __launch_bounds__(TILE *C)
     sk[k] = scl[static_cast<size_t>(kid < 0 ? 0 : kid) * n + site];
     if (k < op.nk && kid >= 0) load_row20(clv + (static_cast<size_t>(kid) * n + site) * row + c * S, x[k]);
   }
+  __builtin_amdgcn_s_setprio(0);
   int sc = 0;
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
