@@ -600,7 +600,8 @@ def main():
         return 0
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; the site-sharded multi-GPU form is "
-              "design only (DESIGN.md). Run with --gpus 1.", flush=True)
+              "a design only (partitions split by site tiles, one all-reduce of the per-root site sums). "
+              "Run with --gpus 1.", flush=True)
         return 1
     threads = args.cpu_threads or min(16, os.cpu_count() or 1)
     # torch's HIP runtime and this library's do not share one process, so the
