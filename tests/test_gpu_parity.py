@@ -3,7 +3,7 @@
 Mirrors the reference's likelihood tests (test/src/LikelihoodTest.cpp):
 repeated evaluation is bit-identical, incremental == full recomputation,
 and every lnL/gradient is compared with the CPU oracle at 1e-9 relative
-(the north-star bar for the plain FP64 form).
+(the parity bar of the plain FP64 form).
 """
 import math
 import sys
