@@ -195,7 +195,9 @@ static int check_plain(const Dag &dag) {
       }
       for (int kid : mine) delivered[kid]++;
     }
-  if (errors == 0) std::printf("ok plain_runs %zu plain_chunks %zu\n", runs.size(), chunks.size());
+  int widest = 0;
+  for (const LevelChunk &ch : chunks) widest = std::max(widest, ch.op_end - ch.op_begin);
+  if (errors == 0) std::printf("ok plain_runs %zu plain_chunks %zu plain_widest %d\n", runs.size(), chunks.size(), widest);
   return errors;
 }
 

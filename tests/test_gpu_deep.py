@@ -127,3 +127,23 @@ def test_deep_network_device_sweep_matches_host_sweep():
     assert ld >= l0 - 1e-6 * abs(l0)
     assert np.allclose(bd, bh, rtol=1e-6, atol=1e-9)
     assert abs(ld - lh) <= 1e-9 * abs(lh)
+
+
+def test_wide_network_gradient_matches_oracle():
+    """30 taxa, 7 reticulations (128 displayed trees): pruning levels of up
+    to 96 ops, past the 48 whose P blocks the plain reverse sweep stages in
+    LDS per level, so the rest read P from memory."""
+    from netrax_amd.simulate import make_workload
+    w = make_workload("wide", 30, 7, [800], seed=2)
+    net, parts = w.network, w.partitions
+    for prec, tol in ((Precision.FP64, 1e-9), (Precision.FP32, 2e-6)):
+        for var in (0, 1):
+            say("wide network", prec.name, "variant", var)
+            ann = AnnotatedNetwork(net.copy(), parts, NetraxOptions(precision=prec, likelihood_variant=var))
+            lnl, g, gp = ann.gradient()
+            ol, og, ogp = O.gradient(net, parts, var)[:3]
+            assert abs(lnl - ol) <= tol * abs(ol), (prec.name, var, lnl, ol)
+            gt = 1e-7 if prec == Precision.FP64 else 2e-4
+            assert np.allclose(g, og, rtol=gt, atol=gt * max(1.0, np.abs(og).max())), (prec.name, var)
+            assert np.allclose(gp, ogp, rtol=gt, atol=gt * max(1.0, np.abs(ogp).max())), (prec.name, var)
+            ann.close()
