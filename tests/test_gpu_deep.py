@@ -147,3 +147,29 @@ def test_wide_network_gradient_matches_oracle():
             assert np.allclose(g, og, rtol=gt, atol=gt * max(1.0, np.abs(og).max())), (prec.name, var)
             assert np.allclose(gp, ogp, rtol=gt, atol=gt * max(1.0, np.abs(ogp).max())), (prec.name, var)
             ann.close()
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_wide_network_device_sweep_matches_host_sweep(variant):
+    """128 displayed-tree classes: the device Newton step and acceptance
+    spread them over a 64-lane wave, two per lane."""
+    from netrax_amd.simulate import make_workload
+    w = make_workload("wide", 30, 7, [800], seed=2)
+    net, parts = w.network, w.partitions
+    rng = np.random.default_rng(6)
+    edges = sorted(int(e) for e in rng.choice(net.n_edges, 8, replace=False))
+    start = net.brlen * rng.uniform(0.5, 2.0, net.n_edges)
+    out = []
+    for device in (True, False):
+        n = net.copy()
+        n.brlen = start.copy()
+        ann = AnnotatedNetwork(n, parts, NetraxOptions(likelihood_variant=variant))
+        l0 = ann.loglikelihood()
+        l1 = ann.optimize_branches(edges) if device else optimize_branches(ann, candidates=edges)
+        out.append((l0, l1, ann.network.brlen.copy()))
+        ann.close()
+    (l0, ld, bd), (_, lh, bh) = out
+    say("wide sweep, variant", variant, l0, "->", ld, lh)
+    assert ld >= l0 - 1e-6 * abs(l0)
+    assert np.allclose(bd, bh, rtol=1e-6, atol=1e-9)
+    assert abs(ld - lh) <= 1e-9 * abs(lh)
