@@ -173,3 +173,32 @@ def test_wide_network_device_sweep_matches_host_sweep(variant):
     assert ld >= l0 - 1e-6 * abs(l0)
     assert np.allclose(bd, bh, rtol=1e-6, atol=1e-9)
     assert abs(ld - lh) <= 1e-9 * abs(lh)
+
+
+@pytest.mark.parametrize("shape", [(16, 9, 5), (24, 11, 7)])
+def test_many_reticulations_match_oracle(shape):
+    """9 and 11 reticulations (512 and 2,048 configurations, 648 distinct
+    displayed-tree classes for the larger): the mixture's inputs no longer
+    fit its LDS staging and run from memory; lnL, branch and probability
+    gradient of both forms and variants against the oracle."""
+    from netrax_amd.simulate import make_workload
+    nt, nr, seed = shape
+    w = make_workload("many", nt, nr, [300], seed=seed)
+    net, parts = w.network, w.partitions
+    for prec, tol in ((Precision.FP64, 1e-9), (Precision.FP32, 2e-6)):
+        for var in (0, 1):
+            say("many reticulations", nt, nr, prec.name, "variant", var)
+            ann = AnnotatedNetwork(net.copy(), parts, NetraxOptions(precision=prec, likelihood_variant=var))
+            st = ann.stats()
+            staged = st.n_root_clvs * len(parts) + 2 * st.n_classes + st.n_classes * len(parts) \
+                + st.n_classes * net.n_retic
+            say("classes", st.n_classes, "root CLVs", st.n_root_clvs, "mixture doubles", staged)
+            if nr == 11:
+                assert staged > 6144  # past the mixture's LDS staging
+            lnl, g, gp = ann.gradient()
+            ol, og, ogp = O.gradient(net, parts, var)[:3]
+            assert abs(lnl - ol) <= tol * abs(ol), (prec.name, var, lnl, ol)
+            gt = 1e-7 if prec == Precision.FP64 else 2e-4
+            assert np.allclose(g, og, rtol=gt, atol=gt * max(1.0, np.abs(og).max())), (prec.name, var)
+            assert np.allclose(gp, ogp, rtol=gt, atol=gt * max(1.0, np.abs(ogp).max())), (prec.name, var)
+            ann.close()
