@@ -202,3 +202,31 @@ def test_many_reticulations_match_oracle(shape):
             assert np.allclose(g, og, rtol=gt, atol=gt * max(1.0, np.abs(og).max())), (prec.name, var)
             assert np.allclose(gp, ogp, rtol=gt, atol=gt * max(1.0, np.abs(ogp).max())), (prec.name, var)
             ann.close()
+
+
+def test_many_classes_device_sweep_matches_host_sweep():
+    """2,304 displayed-tree classes (12 reticulations): the Newton kernels'
+    class table alone exceeds their LDS staging, so it, the tree lnLs and the
+    path sums are read from memory."""
+    from netrax_amd.simulate import make_workload
+    w = make_workload("many", 24, 12, [300], seed=8)
+    net, parts = w.network, w.partitions
+    rng = np.random.default_rng(8)
+    edges = sorted(int(e) for e in rng.choice(net.n_edges, 5, replace=False))
+    start = net.brlen * rng.uniform(0.5, 2.0, net.n_edges)
+    out = []
+    for device in (True, False):
+        n = net.copy()
+        n.brlen = start.copy()
+        ann = AnnotatedNetwork(n, parts, NetraxOptions())
+        st = ann.stats()
+        assert 24 * st.n_classes > 32768  # the class table alone is past the Newton kernels' LDS staging
+        l0 = ann.loglikelihood()
+        l1 = ann.optimize_branches(edges) if device else optimize_branches(ann, candidates=edges)
+        out.append((l0, l1, ann.network.brlen.copy()))
+        ann.close()
+    (l0, ld, bd), (_, lh, bh) = out
+    say("many-class sweep", l0, "->", ld, lh)
+    assert ld >= l0 - 1e-6 * abs(l0)
+    assert np.allclose(bd, bh, rtol=1e-6, atol=1e-9)
+    assert abs(ld - lh) <= 1e-9 * abs(lh)
