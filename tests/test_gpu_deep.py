@@ -230,3 +230,24 @@ def test_many_classes_device_sweep_matches_host_sweep():
     assert ld >= l0 - 1e-6 * abs(l0)
     assert np.allclose(bd, bh, rtol=1e-6, atol=1e-9)
     assert abs(ld - lh) <= 1e-9 * abs(lh)
+
+
+@pytest.mark.parametrize("protein", [False, True])
+def test_ragged_partitions_match_oracle(protein):
+    """Partitions of 1, 65 and 7 sites (each padded to whole 64-site tiles,
+    one of them a single site): lnL and gradient of both forms against the
+    oracle."""
+    from netrax_amd.simulate import make_workload
+    w = make_workload("ragged", 12, 2, [1, 65, 7], seed=9, protein=protein)
+    net, parts = w.network, w.partitions
+    assert min(p.n_sites for p in parts) == 1
+    for prec, tol in ((Precision.FP64, 1e-9), (Precision.FP32, 5e-6)):
+        for var in (0, 1):
+            ann = AnnotatedNetwork(net.copy(), parts, NetraxOptions(precision=prec, likelihood_variant=var))
+            lnl, g, gp = ann.gradient()
+            ol, og, ogp = O.gradient(net, parts, var)[:3]
+            assert abs(lnl - ol) <= tol * abs(ol), (protein, prec.name, var, lnl, ol)
+            gt = 1e-7 if prec == Precision.FP64 else 5e-4
+            assert np.allclose(g, og, rtol=gt, atol=gt * max(1.0, np.abs(og).max())), (protein, prec.name, var)
+            assert np.allclose(gp, ogp, rtol=gt, atol=gt * max(1.0, np.abs(ogp).max())), (protein, prec.name, var)
+            ann.close()
