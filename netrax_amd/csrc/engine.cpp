@@ -472,8 +472,7 @@ NetDesc make_netdesc(const Engine &e) {
   // is handed to the device
   const bool aa = e.S == AA_STATES;
   const void *need[] = {d.ops, d.group_begin, d.clv, d.scl, d.adj, d.adj_exp, d.tips,
-                        aa ? static_cast<const void *>(d.Pa) : d.Pf,
-                        aa ? static_cast<const void *>(d.Pa) : d.Qf,
+                        aa ? static_cast<const void *>(d.Pa) : d.Pf, d.Qf,
                         d.tile_part, d.freqs, d.weights, d.beta, d.root_partials, d.grad_partials};
   for (const void *p : need)
     if (!p) throw std::logic_error("device descriptor built before its buffers were allocated");

@@ -23,7 +23,7 @@ constexpr int MAX_CATS = 8;
 constexpr int AA_STATES = 20;
 constexpr int AA_TILE = 32;
 constexpr int AA_TABLE = 64 * 12;
-constexpr int AA_KINDS = 3;
+constexpr int AA_KINDS = 2;  // operand tables per (partition, edge, category): P, P^T
 
 // ---------------------------------------------------------------- model
 struct EigenModel {

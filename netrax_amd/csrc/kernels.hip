@@ -2132,9 +2132,9 @@ __global__ void __launch_bounds__(TS *C * 2)
 // CLVs and adjoints are stored per tile as [C][20 states][32 sites], so the
 // B operand of step t (state 2t + h) and every D register (one state per
 // half wave) is one coalesced pair of 128-byte rows; no transpose is
-// needed between a CLV's producer and its consumers.  The A operands (P,
-// P^T and P' = r_c Q P per partition, edge and category) are laid out in
-// lane order by k_mfma_tables.  A workgroup is C waves, wave c = rate
+// needed between a CLV's producer and its consumers.  The A operands (P
+// and P^T per partition, edge and category) are laid out in lane order by
+// k_mfma_tables; r_c Q, the same for every edge, is read once per sweep.  A workgroup is C waves, wave c = rate
 // category c of one 32-site tile.  CLV scalers are per site, as in the
 // 4-state form: the waves combine their site maxima through LDS once per
 // group of independent ops.  Adjoint exponents are per (site, category),
@@ -2179,10 +2179,9 @@ __device__ __forceinline__ f32x16 mfma_chain(const float (&a)[AA_STEPS], const f
 }
 
 // Operand tables: Pa[((p E + e) C + c) 3 + kind][lane][12], entry t < 10 of
-// lane (h, col) = M[col][2t + h] with M = P (kind 0), P^T (1), P' (2); zero
+// lane (h, col) = M[col][2t + h] with M = P (kind 0), P^T (1); zero
 // for col >= 20 and t >= 10.
-__global__ void k_mfma_tables(DevLayout L, const double *__restrict__ P,
-                              const double *__restrict__ dP, float *__restrict__ Pa) {
+__global__ void k_mfma_tables(DevLayout L, const double *__restrict__ P, float *__restrict__ Pa) {
   const int64_t total = static_cast<int64_t>(L.n_parts) * L.n_edges * L.C * AA_KINDS * AA_TABLE;
   const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (idx >= total) return;
@@ -2194,7 +2193,7 @@ __global__ void k_mfma_tables(DevLayout L, const double *__restrict__ P,
   float v = 0.0f;
   if (t < AA_STEPS && col < AA_STATES) {
     const int k = 2 * t + h;
-    const double *M = (kind == 2 ? dP : P) + mat * (AA_STATES * AA_STATES);
+    const double *M = P + mat * (AA_STATES * AA_STATES);
     v = static_cast<float>(kind == 1 ? M[k * AA_STATES + col] : M[col * AA_STATES + k]);
   }
   Pa[idx] = v;
@@ -2363,9 +2362,10 @@ __device__ __forceinline__ void deliver_adjoint_mfma(GAS float *__restrict__ dst
 }
 
 // Reverse sweep of the 20-state FP32 form (push form, as k_adjoint_persistent):
-// y_k = P_k x_k and P'_k x_k share the child's B operand; z_k = Abar * y_other
-// is turned from the D layout into a B operand through a wave-private LDS
-// block, and P_k^T z_k is delivered to the child.
+// y_k = P_k x_k from the child's B operand; P'_k x_k = r_c Q y_k and
+// P_k^T z_k (z_k = Abar * y_other) take y_k and z_k from the D layout into
+// B operands through wave-private LDS blocks, and P_k^T z_k is delivered to
+// the child.
 template <int C>
 __device__ __forceinline__ void adjoint_mfma_body(
     const CAS DevOp *__restrict__ ops, const CAS int32_t *__restrict__ group_begin, int n_groups,
@@ -2373,10 +2373,10 @@ __device__ __forceinline__ void adjoint_mfma_body(
     GAS float *__restrict__ adj, GAS int32_t *__restrict__ adj_exp,
     const GAS uint32_t *__restrict__ tips, const GAS double *__restrict__ freqs,
     const GAS int32_t *__restrict__ tile_part, const GAS float *__restrict__ Pa_all,
-    const GAS double *__restrict__ weights, const GAS double *__restrict__ beta, int n_root, int n_uses,
-    GAS double *__restrict__ grad_partials, int tile) {
+    const GAS float *__restrict__ Qf, const GAS double *__restrict__ weights, const GAS double *__restrict__ beta,
+    int n_root, int n_uses, GAS double *__restrict__ grad_partials, int tile) {
   __shared__ double rsum[2][C][AA_TILE];
-  __shared__ float zb[C][AA_STATES * AA_TILE];
+  __shared__ float zb[C][2][AA_STATES * AA_TILE];
   const int part = tile_part[tile >> 1];
   const int c = threadIdx.x >> 6, lane = threadIdx.x & 63, h = lane >> 5, col = lane & 31;
   const int64_t n = L.n_sites_pad;
@@ -2390,7 +2390,15 @@ __device__ __forceinline__ void adjoint_mfma_body(
   const GAS float *__restrict__ Pa =
       Pa_all + static_cast<size_t>(part) * L.n_edges * C * AA_KINDS * AA_TABLE;
   const double w = weights[site];
-  float *zw = zb[c];
+  float *zw = zb[c][0], *yw = zb[c][1];
+  // A operand of r_c Q (the partition's and this wave's category), held for
+  // the whole sweep: P' x = r_c Q (P x) reuses y = P x instead of a P' table
+  float qa[AA_STEPS];
+  {
+    const GAS float *q = Qf + (static_cast<size_t>(part) * C + c) * (AA_STATES * AA_STATES);
+#pragma unroll
+    for (int t = 0; t < AA_STEPS; ++t) qa[t] = col < AA_STATES ? q[col * AA_STATES + 2 * t + h] : 0.0f;
+  }
   int rb = 0;
   for (int gi = n_groups - 1; gi >= 0; --gi) {
     const int g0 = group_begin[gi];
@@ -2423,16 +2431,15 @@ __device__ __forceinline__ void adjoint_mfma_body(
         skraw[kk] = scl[static_cast<size_t>(kid < 0 ? 0 : kid) * n + site];
         if (kk < op.nk && kid >= 0) load_b(clv + kid * slot_stride + tile_off, h, b[kk]);
       }
-      // every A operand of the op (P, P' and, for inner children, P^T) is
+      // every A operand of the op (P and, for inner children, P^T) is
       // requested here with the op's other loads, so the chains below wait
       // for one round trip per op instead of one per table
-      float pa[2][AA_STEPS], pd[2][AA_STEPS], pt[2][AA_STEPS];
+      float pa[2][AA_STEPS], pt[2][AA_STEPS];
 #pragma unroll
       for (int kk = 0; kk < 2; ++kk) {
         if (kk < op.nk) {
           const GAS float *tab = Pa + (static_cast<size_t>(op.edge[kk]) * C + c) * AA_KINDS * AA_TABLE;
           load_a(tab, lane, pa[kk]);
-          load_a(tab + 2 * AA_TABLE, lane, pd[kk]);
           if (op.kid[kk] >= 0) load_a(tab + AA_TABLE, lane, pt[kk]);
         }
       }
@@ -2491,7 +2498,17 @@ __device__ __forceinline__ void adjoint_mfma_body(
           float z[AA_ROWS];
 #pragma unroll
           for (int r = 0; r < AA_ROWS; ++r) z[r] = op.nk == 2 ? a[r] * y[1 - kk][r] : a[r];
-          const f32x16 dy = mfma_chain(pd[kk], b[kk]);
+          // y_k from the D layout into a B operand (rows 2t + h) through the
+          // wave's LDS block, then P'_k x_k = r_c Q y_k
+#pragma unroll
+          for (int r = 0; r < AA_ROWS; ++r)
+            if (aa_valid(r, h)) yw[aa_row(r, h) * AA_TILE + col] = y[kk][r];
+          wave_sync();
+          float yb[AA_STEPS];
+#pragma unroll
+          for (int t = 0; t < AA_STEPS; ++t) yb[t] = yw[(2 * t + h) * AA_TILE + col];
+          wave_sync();
+          const f32x16 dy = mfma_chain(qa, yb);
           float gd = 0.0f;
 #pragma unroll
           for (int r = 0; r < AA_ROWS; ++r) gd = fmaf(z[r], dy[r], gd);
@@ -2528,7 +2545,7 @@ __global__ void __launch_bounds__(64 * C)
   adjoint_mfma_body<C>(to_constant(nd.ops), to_constant(nd.group_begin), nd.n_groups, nd.L,
                        to_global(nd.clv), to_global(nd.scl), to_global(nd.adj), to_global(nd.adj_exp),
                        to_global(reinterpret_cast<const uint32_t *>(nd.tips)), to_global(nd.freqs),
-                       to_global(nd.tile_part), to_global(nd.Pa), to_global(nd.weights),
+                       to_global(nd.tile_part), to_global(nd.Pa), to_global(nd.Qf), to_global(nd.weights),
                        to_global(nd.beta), nd.n_root, nd.n_uses, to_global(nd.grad_partials), wi.tile);
 }
 
@@ -3491,7 +3508,7 @@ void launch_mfma_tables(Engine &e) {
   const int64_t total = static_cast<int64_t>(e.P) * e.E * e.C * AA_KINDS * AA_TABLE;
   const int nb = static_cast<int>((total + 255) / 256);
   hipLaunchKernelGGL(k_mfma_tables, dim3(nb), dim3(256), 0, e.stream, e.L, e.d_P.as<double>(),
-                     e.d_dP.as<double>(), e.d_Pa.as<float>());
+                     e.d_Pa.as<float>());
   NRX_HIP(hipGetLastError());
 }
 
