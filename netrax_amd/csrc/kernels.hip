@@ -660,8 +660,7 @@ struct AdjIn {
   double a[S];
   double x[2][S];  // inner children; tips are expanded from tm at use
   double sd;       // seed factor w / (C Ltilde) of a displayed-tree root's site
-  uint32_t tm[2];
-  int kq, sk[2];
+  int kq, sk[2];   // sk: a child's scaler, or a tip child's code (one register for both)
 };
 
 template <int S, int C>
@@ -689,7 +688,7 @@ __device__ __forceinline__ void adjoint_inputs(const DevOp &op, int64_t site, in
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
     const int32_t kid = op.kid[k];
-    if (k < op.nk && kid < 0) in.tm[k] = tips[static_cast<size_t>(-kid - 1) * n + site];
+    if (k < op.nk && kid < 0) in.sk[k] = static_cast<int>(tips[static_cast<size_t>(-kid - 1) * n + site]);
   }
 }
 
@@ -753,7 +752,7 @@ __global__ void __launch_bounds__(TILE *C, S == 4 ? 3 : 1)
       for (int k = 0; k < 2; ++k) {
         if (k < op.nk) {
           if (op.kid[k] < 0)
-            tip_vec<S>(cur.tm[k], cur.x[k]);
+            tip_vec<S>(static_cast<uint32_t>(cur.sk[k]), cur.x[k]);
           else
             ks += cur.sk[k];
         }
@@ -1645,8 +1644,8 @@ __device__ __forceinline__ void adjoint_persistent_body(
     // values: nothing in this phase waits for an earlier load
     __builtin_amdgcn_s_setprio(1);
     float aq[G][4], x[G][2][4], xq[G][4], acc[G][2][4];
+    // sk: a child's scaler, or a tip child's code (one register for both)
     int gq[G], kq[G], sk[G][2], eacc[G][2];
-    uint32_t tm[G][2];
 #pragma unroll
     for (int j = 0; j < G; ++j) {
       if (j < nb) {
@@ -1695,7 +1694,7 @@ __device__ __forceinline__ void adjoint_persistent_body(
         if (j < nb) {
           const DevOp op = ops[g0 + j];
           const int32_t kid = op.kid[kk];
-          if (kk < op.nk && kid < 0) tm[j][kk] = *at(tips + (-kid - 1) * n, static_cast<uint32_t>(site));
+          if (kk < op.nk && kid < 0) sk[j][kk] = *at(tips + (-kid - 1) * n, static_cast<uint32_t>(site));
         }
       }
     }
@@ -1711,7 +1710,7 @@ __device__ __forceinline__ void adjoint_persistent_body(
         for (int kk = 0; kk < 2; ++kk) {
           if (kk < op.nk) {
             if (op.kid[kk] < 0)
-              tip_vec4(tm[j][kk], x[j][kk]);
+              tip_vec4(static_cast<uint32_t>(sk[j][kk]), x[j][kk]);
             else
               ks[j] += sk[j][kk];
           }
