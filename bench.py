@@ -447,6 +447,33 @@ def bench_widened(w):
     return out
 
 
+def bench_tile_sweep(tile_counts, steps, warmup):
+    """Forward and reverse-sweep time of both forms against the number of
+    64-site tiles: the headline network (50 taxa, 4 reticulations, seed 3)
+    with two uncompressed partitions holding 2/5 and 3/5 of the tiles, the
+    batched form at a fixed shape (3 ops per group, 64-site tiles, or
+    --shape).  A time that stays nearly flat while the tiles per CU grow,
+    then jumps when the tiles outnumber the resident workgroups, marks a
+    walk bound by its per-tile dependency chain, not by HBM."""
+    global SHAPE
+    from netrax_amd.simulate import make_workload
+
+    shape = SHAPE or (3, 64, 1)
+    SHAPE = shape  # run_form reads it for the batched form
+    rows = []
+    for tiles in tile_counts:
+        k1 = (2 * tiles) // 5
+        w = make_workload("tile_sweep", 50, 4, [64 * k1, 64 * (tiles - k1)], 3, compress=False)
+        row = {"tiles": tiles}
+        for name, prec in (("batched", 1), ("plain", 0)):
+            r = run_form(w, prec, steps, warmup, use_graph=prec == 1, tune=prec == 1)
+            row[f"{name}_fwd_ms"] = round(r["fwd_ms_per_eval"], 4)
+            row[f"{name}_adj_ms"] = round(r["adj_ms_per_eval"], 4)
+        log(f"tile sweep: {row}")
+        rows.append(row)
+    return {"shape": list(shape), "rows": rows}
+
+
 def cpu_baseline(w, sample_patterns, threads, budget_s):
     """The oracle (C FP64 restatement, OpenMP over sites) timed on the host
     cores: lnL + full gradient of the whole workload (every pattern), with
@@ -584,6 +611,9 @@ def main():
                     help="fixed FP32 kernel shape (ops per group x sites per workgroup x lanes "
                          "per row in the reverse sweep) instead of autotuning, e.g. 3x64x2 "
                          "(profiling runs)")
+    ap.add_argument("--tile-sweep", default=None, metavar="T1,T2,...",
+                    help="only time both forms on the headline network at these numbers of "
+                         "64-site tiles (uncompressed synthetic partitions)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device copy of BYTES (PMC counter calibration)")
     args = ap.parse_args()
@@ -597,6 +627,10 @@ def main():
         return 0
     if args.stream_reference:
         print(json.dumps(stream_reference()), flush=True)
+        return 0
+    if args.tile_sweep:
+        counts = [int(v) for v in args.tile_sweep.split(",")]
+        print(json.dumps({"tile_sweep": bench_tile_sweep(counts, args.steps, args.warmup)}), flush=True)
         return 0
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; the site-sharded multi-GPU form is "
