@@ -447,31 +447,38 @@ def bench_widened(w):
     return out
 
 
-def bench_tile_sweep(tile_counts, steps, warmup):
+def bench_tile_sweep(tile_counts, steps, warmup, protein=False):
     """Forward and reverse-sweep time of both forms against the number of
     64-site tiles: the headline network (50 taxa, 4 reticulations, seed 3)
     with two uncompressed partitions holding 2/5 and 3/5 of the tiles, the
     batched form at a fixed shape (3 ops per group, 64-site tiles, or
     --shape).  A time that stays nearly flat while the tiles per CU grow,
     then jumps when the tiles outnumber the resident workgroups, marks a
-    walk bound by its per-tile dependency chain, not by HBM."""
+    walk bound by its per-tile dependency chain, not by HBM.  With
+    `protein`: the 40-taxon, 2-reticulation protein network (seed 4), one
+    uncompressed partition, plain FP64 form and the FP32 matrix-core form
+    (4 ops per group; two 32-site workgroups per tile)."""
     global SHAPE
     from netrax_amd.simulate import make_workload
 
-    shape = SHAPE or (3, 64, 1)
-    SHAPE = shape  # run_form reads it for the batched form
+    shape = SHAPE or ((4, 32, 1) if protein else (3, 64, 1))
+    SHAPE = shape  # run_form reads it for the FP32 form
+    forms = (("plain", 0), ("mfma", 1)) if protein else (("batched", 1), ("plain", 0))
     rows = []
     for tiles in tile_counts:
-        k1 = (2 * tiles) // 5
-        w = make_workload("tile_sweep", 50, 4, [64 * k1, 64 * (tiles - k1)], 3, compress=False)
+        if protein:
+            w = make_workload("tile_sweep_aa", 40, 2, [64 * tiles], 4, protein=True, compress=False)
+        else:
+            k1 = (2 * tiles) // 5
+            w = make_workload("tile_sweep", 50, 4, [64 * k1, 64 * (tiles - k1)], 3, compress=False)
         row = {"tiles": tiles}
-        for name, prec in (("batched", 1), ("plain", 0)):
+        for name, prec in forms:
             r = run_form(w, prec, steps, warmup, use_graph=prec == 1, tune=prec == 1)
             row[f"{name}_fwd_ms"] = round(r["fwd_ms_per_eval"], 4)
             row[f"{name}_adj_ms"] = round(r["adj_ms_per_eval"], 4)
         log(f"tile sweep: {row}")
         rows.append(row)
-    return {"shape": list(shape), "rows": rows}
+    return {"shape": list(shape), "states": 20 if protein else 4, "rows": rows}
 
 
 def cpu_baseline(w, sample_patterns, threads, budget_s):
@@ -614,6 +621,8 @@ def main():
     ap.add_argument("--tile-sweep", default=None, metavar="T1,T2,...",
                     help="only time both forms on the headline network at these numbers of "
                          "64-site tiles (uncompressed synthetic partitions)")
+    ap.add_argument("--tile-sweep-protein", action="store_true",
+                    help="with --tile-sweep: the 20-state protein network instead")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device copy of BYTES (PMC counter calibration)")
     args = ap.parse_args()
@@ -630,7 +639,8 @@ def main():
         return 0
     if args.tile_sweep:
         counts = [int(v) for v in args.tile_sweep.split(",")]
-        print(json.dumps({"tile_sweep": bench_tile_sweep(counts, args.steps, args.warmup)}), flush=True)
+        print(json.dumps({"tile_sweep": bench_tile_sweep(counts, args.steps, args.warmup,
+                                                         protein=args.tile_sweep_protein)}), flush=True)
         return 0
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; the site-sharded multi-GPU form is "
